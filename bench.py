@@ -1,0 +1,191 @@
+"""Benchmark of the CLUE engine on one MI355X.
+
+Headline: Mpoints/s of one plain clustering of the 2D 1,048,576-point workload
+(200 Gaussian blobs, centres U[0,1000]^2, sigma U[1,4], 20% noise, weights
+U[0.5,1.5]; dc=2, rhoc=8, dm=4, GaussianKernel(mean 0, sigma 1, amplitude 1)),
+input resident in HBM.  One step = one full make_clusters pass (bounding box,
+tile sort, density, nearest-higher, seeds, labels).
+
+Prints ONE JSON line.  Extra keys: per-stage times (HIP events), the dominant
+kernel's HBM roofline, whole-pipeline HBM fraction, in-radius pair throughput,
+a label-parity check against the oracle, and the CPU baseline (the oracle,
+serial on a bounded spatial window of the same workload, and OpenMP on all of
+it).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mpoints/sec at 1M 2D points + batched events/sec with p99 latency, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="2d_1m_gauss")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--window-fraction", type=float, default=0.25,
+                    help="side of the serial CPU sample window, as a fraction of the box side")
+    return ap.parse_args()
+
+
+def cpu_baseline(ps, prm, args, gpu_labels):
+    """The oracle (CPU restatement of the reference's serial backend) on the box's
+    host cores: serial on a spatial window (same local density), OpenMP on all."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    p = oracle.make_params(prm["dc"], prm["rhoc"], prm["dm"], kernel=prm["kernel"])
+    lo = ps.coords.min(axis=1)
+    hi = ps.coords.max(axis=1)
+    side = (hi - lo) * args.window_fraction
+    mask = np.all((ps.coords >= lo[:, None]) & (ps.coords < (lo + side)[:, None]), axis=0)
+    sub_c = np.ascontiguousarray(ps.coords[:, mask])
+    sub_w = np.ascontiguousarray(ps.weights[mask])
+    t0 = time.perf_counter()
+    oracle.make_clusters(sub_c, sub_w, p, with_tiles=False)
+    t_serial = time.perf_counter() - t0
+    out = {
+        "value": round(sub_c.shape[1] / t_serial / 1e6, 4),
+        "unit": "Mpoints/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"serial oracle on the points of the same workload inside a "
+                  f"{args.window_fraction:.2f}x{args.window_fraction:.2f} corner window of the box "
+                  f"({sub_c.shape[1]} points, same local density), {t_serial:.2f} s",
+    }
+    threads = args.cpu_threads
+    t0 = time.perf_counter()
+    r = oracle.make_clusters_omp(ps.coords, ps.weights, p, threads)
+    t_omp = time.perf_counter() - t0
+    out["omp"] = {"value": round(ps.n / t_omp / 1e6, 4), "unit": "Mpoints/s", "cores": threads,
+                  "sample": f"OpenMP oracle, full workload, {t_omp:.2f} s"}
+    parity = {"labels_equal_oracle": bool(np.array_equal(gpu_labels, r.cluster)),
+              "n_clusters_oracle": int(r.stats["n_stored_seeds"]),
+              "inradius_pairs_density": int(r.stats["inradius_pairs_density"]),
+              "inradius_pairs_nh": int(r.stats["inradius_pairs_nh"]),
+              "candidate_pairs_density": int(r.stats["candidate_pairs_density"]),
+              "candidate_pairs_nh": int(r.stats["candidate_pairs_nh"])}
+    try:
+        out["host_cpu"] = open("/proc/cpuinfo").read().split("model name")[1].split("\n")[0].strip(" :\t")
+    except Exception:
+        pass
+    return out, parity
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU per process; multi-GPU figures are not "
+                                   "produced (events shard across GPUs without communication, see DESIGN.md)",
+                          "n_gpus": args.gpus}), flush=True)
+        sys.exit(2)
+
+    import cluestering_alpaka3_amd as clue
+    from cluestering_alpaka3_amd import datasets
+
+    if clue.device_count() < 1:
+        raise SystemExit("no GPU visible")
+    ps = datasets.workload(args.workload)
+    prm = datasets.WORKLOADS[args.workload]
+    D, N = ps.ndim, ps.n
+
+    stream = clue.Stream(0)
+    plan = clue.Plan(0)
+    plan.reserve(D, N)
+    dcoords = clue.DeviceArray.from_host(ps.coords)
+    dw = clue.DeviceArray.from_host(ps.weights)
+    out = {k: clue.DeviceArray((N,), dt) for k, dt in
+           (("cluster", np.int32), ("is_seed", np.int32), ("rho", np.float32), ("nh", np.int32))}
+    params = clue.make_params(prm["dc"], prm["rhoc"], prm["dm"])
+    kd = clue.kernel_desc(prm["kernel"])
+    md = clue.metric_desc("euclidean")
+    cptrs = [dcoords.offset(d * N) for d in range(D)]
+
+    def step():
+        plan.make_clusters_device(stream, params, D, N, cptrs, dw.ptr, kd, md, None, out["cluster"].ptr,
+                                  out["is_seed"].ptr, out["rho"].ptr, out["nh"].ptr)
+
+    for _ in range(args.warmup):
+        step()
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    stream.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed / args.steps * 1e3
+    mpts = N * args.steps / elapsed / 1e6
+
+    # per-stage device time, HIP events on the launch stream (second pass)
+    plan.set_timing(True)
+    acc = {}
+    for _ in range(args.steps):
+        step()
+        for k, v in plan.stage_times().items():
+            acc[k] = acc.get(k, 0.0) + v
+    plan.set_timing(False)
+    stages = {k: round(v / args.steps, 4) for k, v in acc.items()}
+    stats = plan.stats()
+    labels = out["cluster"].download(stream)
+
+    # dominant kernel roofline: algorithmic bytes per point (density: read D
+    # coords + weight, write rho = 4D+8; nearest-higher: read D coords + rho,
+    # write nh + delta = 4D+12)
+    per_pt = {"density": 4 * D + 8, "nearest_higher": 4 * D + 12}
+    dom = max(per_pt, key=lambda k: stages.get(k, 0.0))
+    t_dom = stages[dom] * 1e-3
+    achieved = N * per_pt[dom] / t_dom / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get(dom)
+    b_plain = N * (24 * D + 92) + 12 * (plan.grid()["ntiles"] + 1)
+    result = {
+        "metric": METRIC,
+        "value": round(mpts, 3),
+        "unit": "Mpoints/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded 2D Gaussian blobs + uniform noise (splitmix64/Box-Muller generator)",
+        "config": {"workload": args.workload, "ndim": D, "n_points": N, "dc": prm["dc"], "rhoc": prm["rhoc"],
+                   "dm": prm["dm"], "kernel": list(prm["kernel"]), "points_per_tile": 128,
+                   "parallelism": "single GPU"},
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+                     "algorithmic_bytes_per_launch": N * per_pt[dom], "avg_launch_ms": stages[dom],
+                     "note": "the neighbour loops are VALU-bound; see pairs_per_s"},
+        "pipeline_hbm": {"bytes": b_plain, "frac": round(b_plain / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 6)},
+        "stages_ms": stages,
+        "stats": stats,
+    }
+    if not args.no_cpu_baseline:
+        cb, parity = cpu_baseline(ps, prm, args, labels)
+        result["cpu_baseline"] = cb
+        result["parity"] = parity
+        pairs = parity["inradius_pairs_density"] + parity["inradius_pairs_nh"]
+        cands = parity["candidate_pairs_density"] + parity["candidate_pairs_nh"]
+        result["pairs_per_s"] = {"inradius": round(pairs / (ms_per_step * 1e-3), 1),
+                                 "candidates": round(cands / (ms_per_step * 1e-3), 1)}
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,58 @@
+"""Build recipe of the engine library (hipcc, gfx950) and of the oracle.
+
+`python -m cluestering_alpaka3_amd.build` compiles everything in-tree.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+ARCH = os.environ.get("CLUE_OFFLOAD_ARCH", "gfx950")
+
+# -ffp-contract=off: no multiply-add fusion, so every float operation rounds
+# exactly as the reference's serial backend writes it.  Packed FP32 is disabled
+# as a target feature.
+HIP_FLAGS = [
+    "-std=c++20", "-O3", f"--offload-arch={ARCH}", "-fPIC", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", os.path.join(ROOT, "include"),
+]
+
+
+def run(cmd, cwd=None):
+    print("+", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l and "hip-link" not in l)
+    if out.strip():
+        print(out, flush=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}")
+
+
+def build_engine():
+    src = os.path.join(HERE, "csrc", "clue_engine.hip")
+    out = os.path.join(HERE, "libclue_hip.so")
+    tmp = out + ".tmp"
+    run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, src])
+    os.replace(tmp, out)
+    return out
+
+
+def build_oracle():
+    run(["make", "-C", os.path.join(ROOT, "oracle"), "-s"])
+
+
+def build_all():
+    build_oracle()
+    build_engine()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["all"]
+    for w in what:
+        {"all": build_all, "engine": build_engine, "oracle": build_oracle}[w]()

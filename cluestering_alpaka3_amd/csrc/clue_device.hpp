@@ -1,0 +1,223 @@
+// clue_device.hpp -- device arithmetic of the MI355X CLUE engine.
+//
+// Every helper reproduces the reference's float arithmetic operation by
+// operation (the library is compiled with -ffp-contract=off, so no multiply-add
+// is fused), which is what makes the engine's tile ids, densities,
+// nearest-highers and labels bit-identical to the serial backend.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cfloat>
+#include <climits>
+
+namespace clue_hip {
+
+constexpr int kMaxDim = 10;
+constexpr int kWave = 64;
+
+// Constant address space view of read-only device memory: a wave-uniform index
+// into it compiles to scalar (SMEM) loads that feed VALU operands directly.
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
+template <typename T>
+__device__ __forceinline__ cptr<T> as_const(const T* p) {
+  return (cptr<T>)(p);
+}
+
+// float -> int32 as static_cast<int> behaves on the reference's x86-64 host
+// (cvttss2si): truncation toward zero, INT_MIN for NaN or out-of-range values.
+__device__ __forceinline__ int32_t f2i(float v) {
+  return (v >= -2147483648.0f && v < 2147483648.0f) ? static_cast<int32_t>(v) : INT32_MIN;
+}
+
+// Grid bounds per dimension (CoordinateExtremes + tile sizes,
+// data_structures/internal/CoordinateExtremes.hpp:9-25, Tiles::m_tilesizes).
+struct DevGrid {
+  float mn[kMaxDim];
+  float mx[kMaxDim];
+  float ts[kMaxDim];
+};
+
+// Host-known grid shape, passed by value to every kernel.
+struct GridShape {
+  int32_t npd;
+  int32_t ntiles;
+  int32_t stride[kMaxDim];  // npd^(D-1-d): integer form of powf(npd, D-d-1), TilesView.hpp:56,67
+  uint8_t wrapped[kMaxDim];
+};
+
+// TilesView::normalizeCoordinate (TilesView.hpp:93-101).
+__device__ __forceinline__ float normalize_coord(float coord, float mn, float mx) {
+  const float range = mx - mn;
+  float rem = coord - static_cast<float>(f2i(coord / range)) * range;
+  if (rem >= mx)
+    rem -= range;
+  else if (rem < mn)
+    rem += range;
+  return rem;
+}
+
+// TilesView::getBin (TilesView.hpp:37-51).
+__device__ __forceinline__ int32_t tile_bin(float coord, float mn, float mx, float ts, int32_t npd,
+                                            bool wrapped) {
+  int32_t b = wrapped ? f2i((normalize_coord(coord, mn, mx) - mn) / ts) : f2i((coord - mn) / ts);
+  b = b < npd - 1 ? b : npd - 1;
+  b = b > 0 ? b : 0;
+  return b;
+}
+
+// ---------------------------------------------------------------------------
+// Engine exponential: the correctly rounded float exp, evaluated with IEEE
+// double operations only.  exp(x) = 2^(k/64) * exp(r) with k = rint(x*64/ln2),
+// r = x - k*ln2/64 (two-constant Cody-Waite split; hi has 33 significant bits so
+// k*hi is exact), exp(r)-1 by a degree-5 Taylor polynomial (|r| <= ln2/128,
+// truncation below 4e-17), 2^(j/64) from a table of correctly rounded doubles;
+// one final rounding double -> float.  The oracle evaluates the same sequence.
+// ---------------------------------------------------------------------------
+__device__ __constant__ double kExp2Table[64] = {
+    0x1.0000000000000p+0, 0x1.02c9a3e778061p+0, 0x1.059b0d3158574p+0, 0x1.0874518759bc8p+0,
+    0x1.0b5586cf9890fp+0, 0x1.0e3ec32d3d1a2p+0, 0x1.11301d0125b51p+0, 0x1.1429aaea92de0p+0,
+    0x1.172b83c7d517bp+0, 0x1.1a35beb6fcb75p+0, 0x1.1d4873168b9aap+0, 0x1.2063b88628cd6p+0,
+    0x1.2387a6e756238p+0, 0x1.26b4565e27cddp+0, 0x1.29e9df51fdee1p+0, 0x1.2d285a6e4030bp+0,
+    0x1.306fe0a31b715p+0, 0x1.33c08b26416ffp+0, 0x1.371a7373aa9cbp+0, 0x1.3a7db34e59ff7p+0,
+    0x1.3dea64c123422p+0, 0x1.4160a21f72e2ap+0, 0x1.44e086061892dp+0, 0x1.486a2b5c13cd0p+0,
+    0x1.4bfdad5362a27p+0, 0x1.4f9b2769d2ca7p+0, 0x1.5342b569d4f82p+0, 0x1.56f4736b527dap+0,
+    0x1.5ab07dd485429p+0, 0x1.5e76f15ad2148p+0, 0x1.6247eb03a5585p+0, 0x1.6623882552225p+0,
+    0x1.6a09e667f3bcdp+0, 0x1.6dfb23c651a2fp+0, 0x1.71f75e8ec5f74p+0, 0x1.75feb564267c9p+0,
+    0x1.7a11473eb0187p+0, 0x1.7e2f336cf4e62p+0, 0x1.82589994cce13p+0, 0x1.868d99b4492edp+0,
+    0x1.8ace5422aa0dbp+0, 0x1.8f1ae99157736p+0, 0x1.93737b0cdc5e5p+0, 0x1.97d829fde4e50p+0,
+    0x1.9c49182a3f090p+0, 0x1.a0c667b5de565p+0, 0x1.a5503b23e255dp+0, 0x1.a9e6b5579fdbfp+0,
+    0x1.ae89f995ad3adp+0, 0x1.b33a2b84f15fbp+0, 0x1.b7f76f2fb5e47p+0, 0x1.bcc1e904bc1d2p+0,
+    0x1.c199bdd85529cp+0, 0x1.c67f12e57d14bp+0, 0x1.cb720dcef9069p+0, 0x1.d072d4a07897cp+0,
+    0x1.d5818dcfba487p+0, 0x1.da9e603db3285p+0, 0x1.dfc97337b9b5fp+0, 0x1.e502ee78b3ff6p+0,
+    0x1.ea4afa2a490dap+0, 0x1.efa1bee615a27p+0, 0x1.f50765b6e4540p+0, 0x1.fa7c1819e90d8p+0};
+
+__device__ __forceinline__ float engine_expf(float x) {
+  if (x != x) return x;
+  if (x > 89.0f) return __builtin_inff();
+  if (x < -110.0f) return 0.0f;
+  const double xd = static_cast<double>(x);
+  const double kd = __builtin_rint(xd * 0x1.71547652b82fep+6);
+  const int k = static_cast<int>(kd);
+  double r = xd - kd * 0x1.62e42fefp-7;
+  r = r - kd * 0x1.473de6af278edp-40;
+  const double p =
+      r + (r * r) * (0.5 + r * (1.0 / 6.0 + r * (1.0 / 24.0 + r * (1.0 / 120.0))));
+  const int j = k & 63;
+  const int m = (k - j) / 64;
+  const double t = kExp2Table[j];
+  const double y = t + t * p;
+  return static_cast<float>(__builtin_ldexp(y, m));
+}
+
+// Convolutional kernels (core/detail/ConvolutionalKernel.hpp:16-65), j != i.
+enum { KFLAT = 0, KGAUSS = 1, KEXP = 2 };
+template <int KERNEL>
+__device__ __forceinline__ float kernel_value(float dist, float p0, float p1, float p2) {
+  if constexpr (KERNEL == KFLAT) {
+    return p0;
+  } else if constexpr (KERNEL == KGAUSS) {
+    const float a = dist - p0;
+    const float num = (-a) * a;        // -(d-mu)*(d-mu)
+    const float den = (2.f * p1) * p1; // 2*sigma*sigma
+    return p2 * engine_expf(num / den);
+  } else {
+    return p1 * engine_expf((-p0) * dist);
+  }
+}
+
+// Distance metrics (core/DistanceMetrics.hpp).  The squared-term sum is the
+// right fold of meta::accumulate (internal/meta/accumulate.hpp:14-15).
+enum { MEUCLID = 0, MWEUCLID = 1, MPERIODIC = 2, MMANHATTAN = 3, MCHEBYSHEV = 4, MWCHEBYSHEV = 5 };
+
+template <int D, int METRIC>
+__device__ __forceinline__ float metric_distance(const float* a, const float* b, const float* mp) {
+  if constexpr (METRIC == MEUCLID || METRIC == MWEUCLID || METRIC == MPERIODIC) {
+    float s = 0.f;
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) {
+      const float df = a[d] - b[d];
+      float t;
+      if constexpr (METRIC == MEUCLID) {
+        t = df * df;
+      } else if constexpr (METRIC == MWEUCLID) {
+        t = (mp[d] * df) * df;
+      } else {
+        const float diff = __builtin_fabsf(df);
+        const float per = mp[d] - diff;
+        const float pd = per < diff ? per : diff;  // std::min(diff, per)
+        t = pd * pd;
+      }
+      s = t + s;  // t_{D-1} + 0.f first, as the fold's init
+    }
+    return __builtin_sqrtf(s);
+  } else if constexpr (METRIC == MMANHATTAN) {
+    float s = 0.f;
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) s = __builtin_fabsf(a[d] - b[d]) + s;
+    return s;
+  } else {
+    float m = 0.f;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      float v = __builtin_fabsf(a[d] - b[d]);
+      if constexpr (METRIC == MWCHEBYSHEV) v = mp[d] * v;
+      m = (m < v) ? v : m;  // std::max(m, v)
+    }
+    return m;
+  }
+}
+
+// Squared Euclidean distance with the reference's right fold; the candidate
+// test d <= r becomes s2 <= thr with thr = max{s : sqrtf(s) <= r} (exact,
+// sqrtf is correctly rounded and monotone).
+template <int D>
+__device__ __forceinline__ float squared_distance(const float* a, const float* b) {
+  float s = 0.f;
+#pragma unroll
+  for (int d = D - 1; d >= 0; --d) {
+    const float df = a[d] - b[d];
+    const float t = df * df;
+    s = (d == D - 1) ? t : t + s;
+  }
+  return s;
+}
+
+// Largest float s with sqrtf(s) < delta (-1 when none): an update of the
+// nearest-higher (d < delta) is possible only for candidates with s2 <= it.
+__device__ __forceinline__ float sqrt_lt_bound(float delta) {
+  if (!(delta > 0.f)) return -1.f;
+  float t = delta * delta;
+  if (!(t <= FLT_MAX)) t = FLT_MAX;
+  while (t > 0.f && !(__builtin_sqrtf(t) < delta)) t = __int_as_float(__float_as_int(t) - 1);
+  if (!(__builtin_sqrtf(t) < delta)) return -1.f;
+  while (t < FLT_MAX) {
+    const float nt = __int_as_float(__float_as_int(t) + 1);
+    if (!(__builtin_sqrtf(nt) < delta)) break;
+    t = nt;
+  }
+  return t;
+}
+
+__device__ __forceinline__ int wave_min_i32(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const int o = __shfl_xor(v, off);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+__device__ __forceinline__ int wave_max_i32(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const int o = __shfl_xor(v, off);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+}  // namespace clue_hip

@@ -1,0 +1,1421 @@
+// clue_engine.hip -- MI355X (gfx950) CLUE engine: kernels, plan state and the C ABI.
+//
+// Pipeline of one clustering call (all on one HIP stream, no host round trip):
+//   bbox        per-dimension min/max, tile sizes      ComputeTiles.hpp:13-27
+//   bin         tile id per point + tile histogram     Tiles.hpp:76-89, detail/AssociationMap.hpp:13-48
+//   sort        stable LSD counting sort of (tile, id) AssociationMap.hpp:122-191 (serial order)
+//   offsets     exclusive scan of the histogram        AssociationMap.hpp:170-174
+//   gather      tile-ordered SoA copy of coords/weight
+//   density     K4, rho in the reference visit order   ClusteringKernels.hpp:18-107
+//   nearest     K5, (distance, visit order) argmin     ClusteringKernels.hpp:109-212
+//   classify    K6, seed flags                         ClusteringKernels.hpp:214-244
+//   seed rank   exclusive scan of the seed flags       (serial push_back order)
+//   labels      K7 by pointer jumping to the root      ClusteringKernels.hpp:246-281
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "clue_abi.h"
+#include "clue_device.hpp"
+
+namespace clue_hip {
+
+constexpr int kThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kThreads * kScanItems;  // 4096
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kThreads * kSortItems;  // 4096 keys per block, 1024 per wave
+constexpr int kBBoxBlocks = 256;
+// Stages timed by clue_plan_stage_times, in pipeline order.
+constexpr int kStages = 8;
+const char* const kStageNames[kStages] = {"bbox",    "bin",       "sort",     "offsets_gather",
+                                          "density", "nearest_higher", "classify_rank", "labels"};
+
+struct Coords {
+  const float* p[kMaxDim];
+};
+
+// ===========================================================================
+// Block-level helpers
+// ===========================================================================
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = __lane_id();
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const int t = __shfl_up(v, off);
+    if (lane >= off) v += t;
+  }
+  return v;
+}
+
+// Inclusive scan across a block of NT threads; `total` receives the block sum.
+template <int NT>
+__device__ __forceinline__ int block_incl_scan(int v, int* smem, int& total) {
+  constexpr int NW = NT / kWave;
+  const int lane = __lane_id();
+  const int w = threadIdx.x / kWave;
+  int incl = wave_incl_scan(v);
+  if (lane == kWave - 1) smem[w] = incl;
+  __syncthreads();
+  if (w == 0) {
+    int t = lane < NW ? smem[lane] : 0;
+    t = wave_incl_scan(t);
+    if (lane < NW) smem[lane] = t;
+  }
+  __syncthreads();
+  if (w > 0) incl += smem[w - 1];
+  total = smem[NW - 1];
+  __syncthreads();
+  return incl;
+}
+
+__global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ===========================================================================
+// Exclusive scan of int32 (3 phases).  out has n+1 entries; out[n] = total.
+// in and out may alias.
+// ===========================================================================
+__global__ __launch_bounds__(kThreads) void k_scan_reduce(const int32_t* in, int64_t n,
+                                                           int32_t* part) {
+  __shared__ int smem[kThreads / kWave];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
+  int sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t idx = base + k * kThreads + threadIdx.x;
+    if (idx < n) sum += in[idx];
+  }
+  int total;
+  block_incl_scan<kThreads>(sum, smem, total);
+  if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_partials(int32_t* part, int nb, int32_t* total_out) {
+  __shared__ int smem[1024 / kWave];
+  int carry = 0;
+  for (int base = 0; base < nb; base += 1024) {
+    const int idx = base + threadIdx.x;
+    const int v = idx < nb ? part[idx] : 0;
+    int total;
+    const int incl = block_incl_scan<1024>(v, smem, total);
+    if (idx < nb) part[idx] = carry + incl - v;
+    carry += total;
+  }
+  if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_down(const int32_t* in, int64_t n,
+                                                         const int32_t* part, int32_t* out) {
+  __shared__ int32_t tile[kScanTile];
+  __shared__ int smem[kThreads / kWave];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t idx = base + k * kThreads + threadIdx.x;
+    tile[k * kThreads + threadIdx.x] = idx < n ? in[idx] : 0;
+  }
+  __syncthreads();
+  int local = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) local += tile[threadIdx.x * kScanItems + k];
+  int total;
+  const int incl = block_incl_scan<kThreads>(local, smem, total);
+  int prefix = part[blockIdx.x] + incl - local;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int v = tile[threadIdx.x * kScanItems + k];
+    tile[threadIdx.x * kScanItems + k] = prefix;
+    prefix += v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t idx = base + k * kThreads + threadIdx.x;
+    if (idx < n) out[idx] = tile[k * kThreads + threadIdx.x];
+  }
+}
+
+// ===========================================================================
+// Bounding box (per-dimension min / max), then tile sizes.
+// ===========================================================================
+__global__ __launch_bounds__(kThreads) void k_bbox_partial(Coords c, int32_t n, int D,
+                                                            float* part) {
+  __shared__ float slo[kThreads / kWave], shi[kThreads / kWave];
+  const int lane = __lane_id();
+  const int w = threadIdx.x / kWave;
+  for (int d = 0; d < D; ++d) {
+    float lo = __builtin_inff(), hi = -__builtin_inff();
+    for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+      const float v = c.p[d][i];
+      lo = v < lo ? v : lo;
+      hi = hi < v ? v : hi;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const float olo = __shfl_xor(lo, off), ohi = __shfl_xor(hi, off);
+      lo = olo < lo ? olo : lo;
+      hi = hi < ohi ? ohi : hi;
+    }
+    if (lane == 0) {
+      slo[w] = lo;
+      shi[w] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int k = 1; k < kThreads / kWave; ++k) {
+        lo = slo[k] < lo ? slo[k] : lo;
+        hi = hi < shi[k] ? shi[k] : hi;
+      }
+      part[(blockIdx.x * D + d) * 2 + 0] = lo;
+      part[(blockIdx.x * D + d) * 2 + 1] = hi;
+    }
+    __syncthreads();
+  }
+}
+
+// ComputeTiles.hpp:21-26: tile size = (max - min) / nPerDim, one float division.
+__global__ void k_bbox_final(const float* part, int nparts, int D, int32_t npd, DevGrid* g) {
+  const int d = threadIdx.x;
+  if (d >= D) return;
+  float lo = __builtin_inff(), hi = -__builtin_inff();
+  for (int b = 0; b < nparts; ++b) {
+    const float plo = part[(b * D + d) * 2 + 0], phi = part[(b * D + d) * 2 + 1];
+    lo = plo < lo ? plo : lo;
+    hi = hi < phi ? phi : hi;
+  }
+  g->mn[d] = lo;
+  g->mx[d] = hi;
+  g->ts[d] = (hi - lo) / static_cast<float>(npd);
+}
+
+// ===========================================================================
+// Tile id per point + histogram (K1 + K2 fused).
+// ===========================================================================
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_bin(Coords c, int32_t n, const DevGrid* grid,
+                                                   GridShape gs, int32_t* key, int32_t* count) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  cptr<DevGrid> g = as_const(grid);
+  int32_t gb = 0;
+#pragma unroll
+  for (int d = 0; d < D; ++d)
+    gb += gs.stride[d] *
+          tile_bin(c.p[d][i], g->mn[d], g->mx[d], g->ts[d], gs.npd, gs.wrapped[d] != 0);
+  key[i] = gb;
+  atomicAdd(&count[gb], 1);
+}
+
+// ===========================================================================
+// Stable LSD counting sort of (tile id, point id), digits of `bits` <= 8 bits.
+// Block = 4 waves x 1024 consecutive keys; inside a wave, keys with equal
+// digits are ranked by lane order (ballot match), waves by index order, blocks
+// by the scanned digit-major histogram: the order of equal tile ids is the
+// input order, i.e. ascending point id after the first pass -- exactly the
+// serial fill order of KernelFillAssociator.
+// ===========================================================================
+__global__ __launch_bounds__(kThreads) void k_radix_hist(const int32_t* keys, int32_t n, int shift,
+                                                          int bits, int nblocks, int32_t* hist) {
+  __shared__ int32_t h[256];
+  const int radix = 1 << bits;
+  for (int t = threadIdx.x; t < 256; t += kThreads) h[t] = 0;
+  __syncthreads();
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kSortTile;
+#pragma unroll 4
+  for (int r = 0; r < kSortItems; ++r) {
+    const int64_t idx = base + r * kThreads + threadIdx.x;
+    if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & (radix - 1)], 1);
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < radix; t += kThreads) hist[t * nblocks + blockIdx.x] = h[t];
+}
+
+__global__ __launch_bounds__(kThreads) void k_radix_scatter(const int32_t* keys_in,
+                                                             const int32_t* vals_in, int32_t n,
+                                                             int shift, int bits, int nblocks,
+                                                             const int32_t* hist_scan,
+                                                             int32_t* keys_out, int32_t* vals_out) {
+  __shared__ int32_t wcnt[kThreads / kWave][256];
+  __shared__ int32_t gbase[256];
+  const int radix = 1 << bits;
+  const int lane = __lane_id();
+  const int w = threadIdx.x / kWave;
+  for (int t = threadIdx.x; t < 256; t += kThreads) {
+#pragma unroll
+    for (int ww = 0; ww < kThreads / kWave; ++ww) wcnt[ww][t] = 0;
+    gbase[t] = t < radix ? hist_scan[t * nblocks + blockIdx.x] : 0;
+  }
+  __syncthreads();
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kSortTile + w * (kSortItems * kWave);
+  int32_t kk[kSortItems], vv[kSortItems], loc[kSortItems];
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int64_t idx = base + r * kWave + lane;
+    const bool valid = idx < n;
+    const int32_t key = valid ? keys_in[idx] : 0;
+    const int32_t val = valid ? (vals_in ? vals_in[idx] : static_cast<int32_t>(idx)) : 0;
+    const int digit = (key >> shift) & (radix - 1);
+    uint64_t peers = __ballot(valid);
+    for (int b = 0; b < bits; ++b) {
+      const bool bit = (digit >> b) & 1;
+      const uint64_t m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    int before = 0;
+    if (valid) before = wcnt[w][digit];
+    const uint64_t lower = peers & lt_mask;
+    if (valid && lower == 0) wcnt[w][digit] = before + __popcll(peers);
+    kk[r] = key;
+    vv[r] = val;
+    loc[r] = valid ? before + __popcll(lower) : -1;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < radix; t += kThreads) {
+    int s = 0;
+#pragma unroll
+    for (int ww = 0; ww < kThreads / kWave; ++ww) {
+      const int cnt = wcnt[ww][t];
+      wcnt[ww][t] = s;
+      s += cnt;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    if (loc[r] < 0) continue;
+    const int digit = (kk[r] >> shift) & (radix - 1);
+    const int pos = gbase[digit] + wcnt[w][digit] + loc[r];
+    if (keys_out) keys_out[pos] = kk[r];
+    vals_out[pos] = vv[r];
+  }
+}
+
+// Tile-ordered SoA copy: xs[d][slot] = x[d][perm[slot]] (the coalesced tile array).
+__global__ __launch_bounds__(kThreads) void k_gather(Coords c, const float* w, const int32_t* perm,
+                                                      int32_t n, int D, int64_t ld, float* xs,
+                                                      float* ws, float* hw, float h) {
+  const int s = blockIdx.x * kThreads + threadIdx.x;
+  if (s >= n) return;
+  const int32_t id = perm[s];
+  for (int d = 0; d < D; ++d) xs[d * ld + s] = c.p[d][id];
+  const float wv = w[id];
+  ws[s] = wv;
+  hw[s] = h * wv;  // FlatKernel term k*w with k = h, precomputed once per point
+}
+
+// ===========================================================================
+// Density (K4) and nearest-higher (K5).
+// ===========================================================================
+struct PassArgs {
+  const float* xs[kMaxDim];  // tile-ordered coordinates
+  const float* ws;           // tile-ordered weights
+  const float* hw;           // tile-ordered h*w (flat kernel)
+  const float* rho_s;        // tile-ordered densities (nearest-higher input)
+  const int32_t* perm;       // slot -> point id
+  const int32_t* offsets;    // tile CSR offsets
+  const DevGrid* grid;
+  GridShape gs;
+  int32_t n;
+  float radius;              // dc (density) or dm (nearest-higher)
+  float thr;                 // max{s : sqrtf(s) <= radius}
+  float kp[3];
+  float mp[kMaxDim];
+  float* rho_s_out;          // density: tile order
+  float* rho_out;            // density: original order
+  int32_t* nh_out;           // nearest-higher: original order
+  float* delta_out;          // nearest-higher distance: original order
+  int32_t* seed_candidates;  // # points with nh == -1
+};
+
+template <int D>
+__device__ __forceinline__ void load_grid(const DevGrid* grid, float* mn, float* mx, float* ts) {
+  cptr<DevGrid> g = as_const(grid);
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    mn[d] = g->mn[d];
+    mx[d] = g->mx[d];
+    ts[d] = g->ts[d];
+  }
+}
+
+// Search box of a point: [bin(x - r), bin(x + r)] per dimension, wrapped
+// dimensions unrolled past npd (TilesView::searchBox, TilesView.hpp:72-82).
+template <int D>
+__device__ __forceinline__ void search_box(const float* q, float r, const float* mn,
+                                           const float* mx, const float* ts, const GridShape& gs,
+                                           int* lo, int* hi) {
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const bool wr = gs.wrapped[d] != 0;
+    int inf = tile_bin(q[d] - r, mn[d], mx[d], ts[d], gs.npd, wr);
+    int sup = tile_bin(q[d] + r, mn[d], mx[d], ts[d], gs.npd, wr);
+    if (wr && inf > sup) sup += gs.npd;
+    lo[d] = inf;
+    hi[d] = sup;
+  }
+}
+
+template <int D>
+__device__ __forceinline__ int tile_index(const int* b, const GridShape& gs) {
+  int g = 0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) g += gs.stride[d] * (gs.wrapped[d] ? b[d] % gs.npd : b[d]);
+  return g;
+}
+
+// Visitor over the tiles a lane needs, in the reference order (lexicographic,
+// dimension 0 outermost; within a tile, CSR order).  UNION: the wave walks the
+// union of its lanes' boxes with wave-uniform tile and candidate indices (the
+// candidates arrive through scalar loads), lanes outside a tile idle under the
+// exec mask.  Otherwise each lane walks its own box.
+template <int D, bool UNION, typename F>
+__device__ __forceinline__ void visit_box(bool valid, const int* lo, const int* hi,
+                                          const GridShape& gs, cptr<int32_t> offsets, F&& body) {
+  if constexpr (UNION) {
+    int ulo[D], uhi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      ulo[d] = uniform(wave_min_i32(valid ? lo[d] : INT_MAX));
+      uhi[d] = uniform(wave_max_i32(valid ? hi[d] : INT_MIN));
+      if (ulo[d] > uhi[d]) return;
+    }
+    int b[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) b[d] = ulo[d];
+    while (true) {
+      bool need = valid;
+#pragma unroll
+      for (int d = 0; d < D; ++d) need = need && b[d] >= lo[d] && b[d] <= hi[d];
+      if (__ballot(need) != 0) {
+        const int g = uniform(tile_index<D>(b, gs));
+        const int beg = offsets[g];
+        const int end = offsets[g + 1];
+        if (need) body(beg, end);
+      }
+      int d = D - 1;
+      for (; d >= 0; --d) {
+        if (++b[d] <= uhi[d]) break;
+        b[d] = ulo[d];
+      }
+      if (d < 0) break;
+    }
+  } else {
+    if (!valid) return;
+    int b[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) b[d] = lo[d];
+    while (true) {
+      const int g = tile_index<D>(b, gs);
+      body(offsets[g], offsets[g + 1]);
+      int d = D - 1;
+      for (; d >= 0; --d) {
+        if (++b[d] <= hi[d]) break;
+        b[d] = lo[d];
+      }
+      if (d < 0) break;
+    }
+  }
+}
+
+template <int D, int METRIC, int KERNEL, bool UNION>
+__global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
+  const int s = blockIdx.x * kThreads + threadIdx.x;
+  const bool valid = s < a.n;
+  const int sq = valid ? s : a.n - 1;
+  float q[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) q[d] = a.xs[d][sq];
+  const float w_self = a.ws[sq];  // the point's own term is 1 * w_i (kernel value 1 for j == i)
+  float mn[D], mx[D], ts[D];
+  load_grid<D>(a.grid, mn, mx, ts);
+  int lo[D], hi[D];
+  search_box<D>(q, a.radius, mn, mx, ts, a.gs, lo, hi);
+  cptr<float> X[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) X[d] = as_const(a.xs[d]);
+  cptr<float> W = as_const(a.ws);
+  cptr<float> HW = as_const(a.hw);
+  float rho = 0.f;
+  // One candidate: rho_i += ((float)(d <= dc) * k) * w_j in the visit order
+  // (ClusteringKernels.hpp:38-44); out-of-radius terms are +0 and skipped.
+  auto eval = [&](auto self_c, int j, const float* c, float wj, float hwj) {
+    constexpr bool SELF = decltype(self_c)::value;  // may this tile hold the lane's own point?
+    if constexpr (METRIC == MEUCLID) {
+      const float s2 = squared_distance<D>(q, c);
+      if constexpr (KERNEL == KFLAT) {
+        float term = hwj;
+        if constexpr (SELF) term = (j == s) ? w_self : hwj;
+        if (s2 <= a.thr) rho += term;
+      } else {
+        if (s2 <= a.thr) {
+          const float k =
+              (j == s) ? 1.f : kernel_value<KERNEL>(__builtin_sqrtf(s2), a.kp[0], a.kp[1], a.kp[2]);
+          rho += k * wj;
+        }
+      }
+    } else {
+      const float dist = metric_distance<D, METRIC>(q, c, a.mp);
+      if (dist <= a.radius) {
+        const float k = (j == s) ? 1.f : kernel_value<KERNEL>(dist, a.kp[0], a.kp[1], a.kp[2]);
+        rho += k * wj;
+      }
+    }
+  };
+  auto run = [&](auto self_c, int beg, int end) {
+    int j = beg;
+    if constexpr (UNION) {
+      for (; j + 4 <= end; j += 4) {
+        float c[4][D], wv[4], hv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+#pragma unroll
+          for (int d = 0; d < D; ++d) c[u][d] = X[d][j + u];
+          if constexpr (KERNEL == KFLAT) hv[u] = HW[j + u];
+          else wv[u] = W[j + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) eval(self_c, j + u, c[u], wv[u], hv[u]);
+      }
+    }
+    for (; j < end; ++j) {
+      float c[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) c[d] = X[d][j];
+      float wj = 0.f, hwj = 0.f;
+      if constexpr (KERNEL == KFLAT) hwj = HW[j];
+      else wj = W[j];
+      eval(self_c, j, c, wj, hwj);
+    }
+  };
+  visit_box<D, UNION>(valid, lo, hi, a.gs, as_const(a.offsets), [&](int beg, int end) {
+    // Only the tile holding a lane's own point needs the j == i test.
+    if (!UNION || __ballot(s >= beg && s < end) != 0)
+      run(std::true_type{}, beg, end);
+    else
+      run(std::false_type{}, beg, end);
+  });
+  if (valid) {
+    a.rho_s_out[s] = rho;
+    a.rho_out[a.perm[s]] = rho;
+  }
+}
+
+template <int D, int METRIC, bool UNION>
+__global__ __launch_bounds__(kThreads) void k_nearest_higher(const PassArgs a) {
+  const int s = blockIdx.x * kThreads + threadIdx.x;
+  const bool valid = s < a.n;
+  const int sq = valid ? s : a.n - 1;
+  float q[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) q[d] = a.xs[d][sq];
+  const float rho_i = a.rho_s[sq];
+  const int32_t id_i = a.perm[sq];
+  float mn[D], mx[D], ts[D];
+  load_grid<D>(a.grid, mn, mx, ts);
+  int lo[D], hi[D];
+  search_box<D>(q, a.radius, mn, mx, ts, a.gs, lo, hi);
+  cptr<float> X[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) X[d] = as_const(a.xs[d]);
+  cptr<float> R = as_const(a.rho_s);
+  cptr<int32_t> P = as_const(a.perm);
+  float delta = FLT_MAX;
+  int32_t nh = -1;
+  float bound = a.thr;  // updates need s2 <= min(thr(dm), max{s : sqrtf(s) < delta})
+  // One candidate (ClusteringKernels.hpp:129-144): j is "higher" if rho_j > rho_i,
+  // or equal, positive and with a larger id; the first strict minimum of the
+  // distance in the visit order wins.
+  auto eval = [&](const float* c, float rho_j, int32_t id_j) {
+    const int rb = __float_as_int(rho_j);
+    const bool pos = (rb > 0) && (rb <= 0x7f800000);  // rho_j > 0.f, on the uniform bits
+    const bool higher = (rho_j > rho_i) || ((rho_j == rho_i) && pos && (id_j > id_i));
+    if constexpr (METRIC == MEUCLID) {
+      const float s2 = squared_distance<D>(q, c);
+      if (higher && s2 <= bound) {
+        delta = __builtin_sqrtf(s2);
+        nh = id_j;
+        const float lim = sqrt_lt_bound(delta);
+        bound = lim < a.thr ? lim : a.thr;
+      }
+    } else {
+      const float dist = metric_distance<D, METRIC>(q, c, a.mp);
+      if (higher && dist <= a.radius && dist < delta) {
+        delta = dist;
+        nh = id_j;
+      }
+    }
+  };
+  visit_box<D, UNION>(valid, lo, hi, a.gs, as_const(a.offsets), [&](int beg, int end) {
+    int j = beg;
+    if constexpr (UNION) {
+      for (; j + 4 <= end; j += 4) {
+        float c[4][D], rv[4];
+        int32_t iv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+#pragma unroll
+          for (int d = 0; d < D; ++d) c[u][d] = X[d][j + u];
+          rv[u] = R[j + u];
+          iv[u] = P[j + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) eval(c[u], rv[u], iv[u]);
+      }
+    }
+    for (; j < end; ++j) {
+      float c[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) c[d] = X[d][j];
+      eval(c, R[j], P[j]);
+    }
+  });
+  if (valid) {
+    a.nh_out[id_i] = nh;
+    a.delta_out[id_i] = delta;
+  }
+  const uint64_t none = __ballot(valid && nh == -1);
+  if (__lane_id() == 0 && none) atomicAdd(a.seed_candidates, static_cast<int>(__popcll(none)));
+}
+
+// ===========================================================================
+// Seeds (K6), seed ranks and labels (K7).
+// ===========================================================================
+struct ClassifyArgs {
+  Coords c;
+  const float* rho;
+  const float* delta;
+  int32_t* nh;        // in: nearest-higher; out: -1 for seeds
+  int32_t* is_seed;
+  int32_t* flag;      // seed flags to be scanned into ranks
+  int32_t* root;      // parent pointer, roots point to themselves
+  int32_t n;
+  float seed_dc;
+  float rhoc;
+  float mp[kMaxDim];
+};
+
+template <int D, int METRIC>
+__global__ __launch_bounds__(kThreads) void k_classify(const ClassifyArgs a) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= a.n) return;
+  const int32_t nhv = a.nh[i];
+  float dist;
+  if (nhv == -1) {
+    // metric(point_i, PointsView[-1]) with the all-FLT_MAX sentinel (PointsCommon.hpp:75-81)
+    float x[D], m[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      x[d] = a.c.p[d][i];
+      m[d] = FLT_MAX;
+    }
+    dist = metric_distance<D, METRIC>(x, m, a.mp);
+  } else {
+    dist = a.delta[i];  // the nearest-higher kernel's metric(point_i, point_nh)
+  }
+  const bool seed = (dist > a.seed_dc) && (a.rho[i] >= a.rhoc);
+  a.is_seed[i] = seed ? 1 : 0;
+  a.flag[i] = seed ? 1 : 0;
+  const int32_t parent = seed ? -1 : nhv;
+  if (seed) a.nh[i] = -1;
+  a.root[i] = parent == -1 ? i : parent;
+}
+
+// counters: [0] seed candidates, [1] grow-only capacity carried across calls,
+// [2] capacity of this call, [3] seeds, [4] stored seeds (clusters), [5] dropped.
+__global__ void k_capacity(int32_t* counters, const int32_t* n_seeds_total, int unbounded) {
+  if (threadIdx.x != 0) return;
+  const int32_t sc = counters[0];
+  const int32_t cap = counters[1] > sc ? counters[1] : sc;  // SetupSeeds.hpp:14-18
+  counters[1] = cap;
+  const int32_t ns = *n_seeds_total;
+  const int32_t used = unbounded ? ns : cap;
+  counters[2] = used;
+  counters[3] = ns;
+  counters[4] = ns < used ? ns : used;
+  counters[5] = ns - counters[4];
+}
+
+// Pointer jumping: every hop follows the current parent array, which other
+// threads only ever shorten; `hops` per round bounds the work of a round.
+__global__ __launch_bounds__(kThreads) void k_labels(int32_t* root, int32_t n, int hops,
+                                                      const int32_t* is_seed, const int32_t* rank,
+                                                      const int32_t* counters, int32_t* cluster,
+                                                      int final_round) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  int32_t r = root[i];
+  for (int h = 0; h < hops; ++h) {
+    const int32_t rr = root[r];
+    if (rr == r) break;
+    r = rr;
+  }
+  root[i] = r;
+  if (final_round) {
+    // label = position of the root in the serial seed array, -1 if the root is
+    // not a stored seed (KernelAssignClusters, ClusteringKernels.hpp:252-279)
+    const int32_t cap = counters[2];
+    cluster[i] = (is_seed[r] && rank[r] < cap) ? rank[r] : -1;
+  }
+}
+
+// ===========================================================================
+// Self-test arithmetic (parity of the device math with the host's).
+// ===========================================================================
+__global__ void k_selftest(int op, const float* a, const float* b, float* out, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float x = a[i];
+  switch (op) {
+    case 0: out[i] = engine_expf(x); break;
+    case 1: out[i] = __builtin_sqrtf(x); break;
+    case 2: out[i] = x / b[i]; break;
+    case 3: out[i] = sqrt_lt_bound(x); break;
+    default: out[i] = 0.f;
+  }
+}
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+thread_local std::string g_error;
+
+clue_status fail(clue_status st, const std::string& msg) {
+  g_error = msg;
+  return st;
+}
+
+#define CLUE_HIP(call)                                                                     \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return fail(e_ == hipErrorOutOfMemory ? CLUE_ERR_OUT_OF_MEMORY : CLUE_ERR_RUNTIME,   \
+                  std::string(#call) + ": " + hipGetErrorString(e_));                      \
+  } while (0)
+
+#define CLUE_LAUNCH_CHECK(what)                                                            \
+  do {                                                                                     \
+    hipError_t e_ = hipGetLastError();                                                     \
+    if (e_ != hipSuccess)                                                                  \
+      return fail(CLUE_ERR_RUNTIME, std::string("launch ") + what + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+inline int64_t div_up(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t need) {
+    if (need <= bytes && p) return hipSuccess;
+    if (p) {
+      hipError_t e = hipFree(p);
+      if (e != hipSuccess) return e;
+      p = nullptr;
+      bytes = 0;
+    }
+    const size_t alloc = need < 256 ? 256 : need;
+    hipError_t e = hipMalloc(&p, alloc);
+    if (e == hipSuccess) bytes = alloc;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+}  // namespace clue_hip
+
+using namespace clue_hip;
+
+struct clue_plan_s {
+  int device = 0;
+  DevBuf key, key2, perm, perm2, count, offsets, hist, part, xs, ws, hw, rho_s, delta, flag, root,
+      nh_tmp, rho_tmp, seed_tmp, bbox_part, grid, counters;
+  int64_t ld = 0;  // leading dimension of xs
+  // last call
+  bool has_run = false;
+  hipStream_t last_stream = nullptr;
+  int32_t last_n = 0, last_d = 0, last_npd = 0, last_T = 0;
+  int32_t* last_perm = nullptr;
+  // optional per-stage HIP event timing of the last call
+  bool timing = false;
+  hipEvent_t ev[kStages + 1] = {};
+};
+
+namespace clue_hip {
+
+// SetupTiles.hpp:23-26: ntiles = ceil(N / (float)ppb); npd = ceil(pow(ntiles, 1./D));
+// ntiles = pow(npd, D) (double pow, as std::pow(int, double) resolves).
+void grid_shape(int32_t D, int32_t n, int32_t ppb, int32_t& npd, int32_t& T) {
+  int32_t nt = static_cast<int32_t>(std::ceil(static_cast<float>(n) / static_cast<float>(ppb)));
+  npd = static_cast<int32_t>(std::ceil(std::pow(static_cast<double>(nt), 1. / D)));
+  T = static_cast<int32_t>(std::pow(static_cast<double>(npd), static_cast<double>(D)));
+}
+
+// Largest float s with sqrtf(s) <= r (host sqrtf is correctly rounded, as the device's).
+float sqrt_le_bound(float r) {
+  if (!(r >= 0.f)) return -1.f;
+  float t = r * r;
+  if (!(t <= FLT_MAX)) t = FLT_MAX;
+  while (t > 0.f && !(std::sqrt(t) <= r)) t = std::nextafter(t, -1.f);
+  if (!(std::sqrt(t) <= r)) return -1.f;
+  while (t < FLT_MAX && std::sqrt(std::nextafter(t, FLT_MAX)) <= r) t = std::nextafter(t, FLT_MAX);
+  return t;
+}
+
+clue_status reserve(clue_plan_s* p, int32_t D, int32_t n, int32_t T) {
+  CLUE_HIP(hipSetDevice(p->device));
+  const size_t N = static_cast<size_t>(n);
+  const size_t nb = static_cast<size_t>(div_up(n, kSortTile));
+  const size_t hist_entries = 256 * nb + 1;
+  const size_t scan_max = std::max<size_t>({N + 1, hist_entries, static_cast<size_t>(T) + 1});
+  const int64_t ld = div_up(n, 64) * 64;
+  CLUE_HIP(p->key.ensure(N * 4));
+  CLUE_HIP(p->key2.ensure(N * 4));
+  CLUE_HIP(p->perm.ensure(N * 4));
+  CLUE_HIP(p->perm2.ensure(N * 4));
+  CLUE_HIP(p->count.ensure((static_cast<size_t>(T) + 1) * 4));
+  CLUE_HIP(p->offsets.ensure((static_cast<size_t>(T) + 1) * 4));
+  CLUE_HIP(p->hist.ensure(hist_entries * 4));
+  CLUE_HIP(p->part.ensure((div_up(scan_max, kScanTile) + 1) * 4));
+  CLUE_HIP(p->xs.ensure(static_cast<size_t>(ld) * D * 4));
+  p->ld = ld;
+  CLUE_HIP(p->ws.ensure(N * 4));
+  CLUE_HIP(p->hw.ensure(N * 4));
+  CLUE_HIP(p->rho_s.ensure(N * 4));
+  CLUE_HIP(p->delta.ensure(N * 4));
+  CLUE_HIP(p->flag.ensure((N + 1) * 4));
+  CLUE_HIP(p->root.ensure(N * 4));
+  CLUE_HIP(p->nh_tmp.ensure(N * 4));
+  CLUE_HIP(p->rho_tmp.ensure(N * 4));
+  CLUE_HIP(p->seed_tmp.ensure(N * 4));
+  CLUE_HIP(p->bbox_part.ensure(kBBoxBlocks * kMaxDim * 2 * 4));
+  if (!p->grid.p) CLUE_HIP(p->grid.ensure(sizeof(DevGrid)));
+  if (!p->counters.p) {
+    CLUE_HIP(p->counters.ensure(16 * 4));
+    CLUE_HIP(hipMemset(p->counters.p, 0, 16 * 4));
+  }
+  return CLUE_OK;
+}
+
+clue_status scan_exclusive(const int32_t* in, int32_t* out, int64_t n, int32_t* part,
+                           hipStream_t st) {
+  const int nb = static_cast<int>(div_up(n, kScanTile));
+  if (nb == 0) {
+    k_fill_i32<<<1, 64, 0, st>>>(out, 0, 1);
+    CLUE_LAUNCH_CHECK("fill");
+    return CLUE_OK;
+  }
+  k_scan_reduce<<<nb, kThreads, 0, st>>>(in, n, part);
+  CLUE_LAUNCH_CHECK("scan_reduce");
+  k_scan_partials<<<1, 1024, 0, st>>>(part, nb, out + n);
+  CLUE_LAUNCH_CHECK("scan_partials");
+  k_scan_down<<<nb, kThreads, 0, st>>>(in, n, part, out);
+  CLUE_LAUNCH_CHECK("scan_down");
+  return CLUE_OK;
+}
+
+template <int D>
+clue_status launch_bin(const Coords& c, int32_t n, clue_plan_s* p, const GridShape& gs,
+                       hipStream_t st) {
+  k_bin<D><<<div_up(n, kThreads), kThreads, 0, st>>>(c, n, p->grid.as<DevGrid>(), gs,
+                                                      p->key.as<int32_t>(), p->count.as<int32_t>());
+  CLUE_LAUNCH_CHECK("bin");
+  return CLUE_OK;
+}
+
+clue_status mark(clue_plan_s* p, int k, hipStream_t st);
+
+template <int D, int METRIC>
+clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, float dm,
+                          hipStream_t st, clue_plan_s* p) {
+  constexpr bool U = (D <= 3);
+  const int grid = static_cast<int>(div_up(base_args.n, kThreads));
+  PassArgs a = base_args;
+  a.radius = dc;
+  a.thr = sqrt_le_bound(dc);
+  clue_status rc = mark(p, 4, st);
+  if (rc != CLUE_OK) return rc;
+  switch (kernel_type) {
+    case KFLAT: k_density<D, METRIC, KFLAT, U><<<grid, kThreads, 0, st>>>(a); break;
+    case KGAUSS: k_density<D, METRIC, KGAUSS, U><<<grid, kThreads, 0, st>>>(a); break;
+    case KEXP: k_density<D, METRIC, KEXP, U><<<grid, kThreads, 0, st>>>(a); break;
+    default: return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown convolutional kernel type");
+  }
+  CLUE_LAUNCH_CHECK("density");
+  a.radius = dm;
+  a.thr = sqrt_le_bound(dm);
+  rc = mark(p, 5, st);
+  if (rc != CLUE_OK) return rc;
+  k_nearest_higher<D, METRIC, U><<<grid, kThreads, 0, st>>>(a);
+  CLUE_LAUNCH_CHECK("nearest_higher");
+  return CLUE_OK;
+}
+
+template <int D, int METRIC>
+clue_status launch_classify(const ClassifyArgs& a, hipStream_t st) {
+  k_classify<D, METRIC><<<div_up(a.n, kThreads), kThreads, 0, st>>>(a);
+  CLUE_LAUNCH_CHECK("classify");
+  return CLUE_OK;
+}
+
+template <int D>
+clue_status dispatch_metric_passes(int metric, const PassArgs& a, int kt, float dc, float dm,
+                                   hipStream_t st, clue_plan_s* p) {
+  switch (metric) {
+    case MEUCLID: return launch_passes<D, MEUCLID>(a, kt, dc, dm, st, p);
+    case MWEUCLID: return launch_passes<D, MWEUCLID>(a, kt, dc, dm, st, p);
+    case MPERIODIC: return launch_passes<D, MPERIODIC>(a, kt, dc, dm, st, p);
+    case MMANHATTAN: return launch_passes<D, MMANHATTAN>(a, kt, dc, dm, st, p);
+    case MCHEBYSHEV: return launch_passes<D, MCHEBYSHEV>(a, kt, dc, dm, st, p);
+    case MWCHEBYSHEV: return launch_passes<D, MWCHEBYSHEV>(a, kt, dc, dm, st, p);
+  }
+  return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown distance metric type");
+}
+
+template <int D>
+clue_status dispatch_metric_classify(int metric, const ClassifyArgs& a, hipStream_t st) {
+  switch (metric) {
+    case MEUCLID: return launch_classify<D, MEUCLID>(a, st);
+    case MWEUCLID: return launch_classify<D, MWEUCLID>(a, st);
+    case MPERIODIC: return launch_classify<D, MPERIODIC>(a, st);
+    case MMANHATTAN: return launch_classify<D, MMANHATTAN>(a, st);
+    case MCHEBYSHEV: return launch_classify<D, MCHEBYSHEV>(a, st);
+    case MWCHEBYSHEV: return launch_classify<D, MWCHEBYSHEV>(a, st);
+  }
+  return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown distance metric type");
+}
+
+template <typename F>
+clue_status dispatch_dim(int D, F&& f) {
+  switch (D) {
+    case 1: return f(std::integral_constant<int, 1>{});
+    case 2: return f(std::integral_constant<int, 2>{});
+    case 3: return f(std::integral_constant<int, 3>{});
+    case 4: return f(std::integral_constant<int, 4>{});
+    case 5: return f(std::integral_constant<int, 5>{});
+    case 6: return f(std::integral_constant<int, 6>{});
+    case 7: return f(std::integral_constant<int, 7>{});
+    case 8: return f(std::integral_constant<int, 8>{});
+    case 9: return f(std::integral_constant<int, 9>{});
+    case 10: return f(std::integral_constant<int, 10>{});
+  }
+  return fail(CLUE_ERR_INVALID_ARGUMENT, "number of dimensions must be in [1, 10]");
+}
+
+clue_status validate(const clue_params* prm, int32_t ndim, int32_t n, const float* const* coords,
+                     const float* weights, const clue_kernel_desc* kernel,
+                     const clue_metric_desc* metric, const int32_t* cluster_out) {
+  if (!prm) return fail(CLUE_ERR_INVALID_ARGUMENT, "params is NULL");
+  // detail/Clusterer.hpp:32-35
+  if (prm->dc <= 0.f || prm->rhoc < 0.f || prm->dm <= 0.f || prm->seed_dc <= 0.f ||
+      prm->points_per_tile <= 0)
+    return fail(CLUE_ERR_INVALID_ARGUMENT,
+                "Invalid clustering parameters. The parameters must be positive.");
+  if (ndim < 1 || ndim > CLUE_MAX_DIM)
+    return fail(CLUE_ERR_INVALID_ARGUMENT, "number of dimensions must be in [1, 10]");
+  // detail/PointsDevice.hpp:24-31
+  if (n <= 0)
+    return fail(CLUE_ERR_INVALID_ARGUMENT,
+                "Number of points passed to PointsDevice constructor must be positive.");
+  if (!coords || !weights || !cluster_out)
+    return fail(CLUE_ERR_INVALID_ARGUMENT, "coords, weights and cluster output are required");
+  for (int d = 0; d < ndim; ++d)
+    if (!coords[d]) return fail(CLUE_ERR_INVALID_ARGUMENT, "coordinate pointer is NULL");
+  if (kernel && (kernel->type < 0 || kernel->type > 2))
+    return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown convolutional kernel type");
+  if (metric && (metric->type < 0 || metric->type > 5))
+    return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown distance metric type");
+  if (prm->seed_mode != CLUE_SEEDS_REFERENCE && prm->seed_mode != CLUE_SEEDS_UNBOUNDED)
+    return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown seed mode");
+  return CLUE_OK;
+}
+
+clue_status mark(clue_plan_s* p, int k, hipStream_t st) {
+  if (!p->timing) return CLUE_OK;
+  if (!p->ev[k]) CLUE_HIP(hipEventCreate(&p->ev[k]));
+  CLUE_HIP(hipEventRecord(p->ev[k], st));
+  return CLUE_OK;
+}
+
+#define CLUE_MARK(k)                       \
+  do {                                     \
+    clue_status m_ = mark(p, (k), st);     \
+    if (m_ != CLUE_OK) return m_;          \
+  } while (0)
+
+clue_status make_clusters(clue_plan_s* p, hipStream_t st, const clue_params* prm, int32_t D,
+                          int32_t n, const float* const* coords, const float* weights,
+                          const clue_kernel_desc* kernel, const clue_metric_desc* metric,
+                          const uint8_t* wrapped, int32_t* cluster_out, int32_t* is_seed_out,
+                          float* rho_out, int32_t* nh_out) {
+  clue_status rc = validate(prm, D, n, coords, weights, kernel, metric, cluster_out);
+  if (rc != CLUE_OK) return rc;
+  const clue_kernel_desc kd = kernel ? *kernel : clue_kernel_desc{CLUE_KERNEL_FLAT, {0.5f, 0.f, 0.f}};
+  clue_metric_desc md{};
+  if (metric) md = *metric;
+
+  int32_t npd, T;
+  grid_shape(D, n, prm->points_per_tile, npd, T);
+  if (npd <= 0 || T <= 0) return fail(CLUE_ERR_INVALID_ARGUMENT, "degenerate tile grid");
+  rc = reserve(p, D, n, T);
+  if (rc != CLUE_OK) return rc;
+
+  GridShape gs{};
+  gs.npd = npd;
+  gs.ntiles = T;
+  {
+    int32_t s = 1;
+    for (int d = D - 1; d >= 0; --d) {
+      gs.stride[d] = s;
+      s *= npd;
+    }
+    for (int d = 0; d < D; ++d) gs.wrapped[d] = wrapped ? (wrapped[d] ? 1 : 0) : 0;
+  }
+  Coords c{};
+  for (int d = 0; d < D; ++d) c.p[d] = coords[d];
+
+  float* rho_buf = rho_out ? rho_out : p->rho_tmp.as<float>();
+  int32_t* nh_buf = nh_out ? nh_out : p->nh_tmp.as<int32_t>();
+  int32_t* seed_buf = is_seed_out ? is_seed_out : p->seed_tmp.as<int32_t>();
+
+  // ---- bounding box and tile sizes
+  CLUE_MARK(0);
+  const int bb = static_cast<int>(std::min<int64_t>(kBBoxBlocks, div_up(n, kThreads)));
+  k_bbox_partial<<<bb, kThreads, 0, st>>>(c, n, D, p->bbox_part.as<float>());
+  CLUE_LAUNCH_CHECK("bbox_partial");
+  k_bbox_final<<<1, 64, 0, st>>>(p->bbox_part.as<float>(), bb, D, npd, p->grid.as<DevGrid>());
+  CLUE_LAUNCH_CHECK("bbox_final");
+
+  // ---- tile ids + histogram
+  CLUE_MARK(1);
+  k_fill_i32<<<div_up(T + 1, kThreads), kThreads, 0, st>>>(p->count.as<int32_t>(), 0, T + 1);
+  CLUE_LAUNCH_CHECK("fill");
+  rc = dispatch_dim(D, [&](auto dc_) { return launch_bin<decltype(dc_)::value>(c, n, p, gs, st); });
+  if (rc != CLUE_OK) return rc;
+
+  // ---- stable counting sort by tile id
+  CLUE_MARK(2);
+  int total_bits = 0;
+  while ((1ll << total_bits) < T) ++total_bits;
+  const int passes = std::max(1, (total_bits + 7) / 8);
+  const int bpp = std::max(1, (total_bits + passes - 1) / passes);
+  const int nblocks = static_cast<int>(div_up(n, kSortTile));
+  int32_t* kin = p->key.as<int32_t>();
+  int32_t* vin = nullptr;
+  int32_t* kout = p->key2.as<int32_t>();
+  int32_t* vout = p->perm.as<int32_t>();
+  for (int pass = 0; pass < passes; ++pass) {
+    const int shift = pass * bpp;
+    const bool last = pass == passes - 1;
+    k_radix_hist<<<nblocks, kThreads, 0, st>>>(kin, n, shift, bpp, nblocks, p->hist.as<int32_t>());
+    CLUE_LAUNCH_CHECK("radix_hist");
+    rc = scan_exclusive(p->hist.as<int32_t>(), p->hist.as<int32_t>(),
+                        static_cast<int64_t>(1 << bpp) * nblocks, p->part.as<int32_t>(), st);
+    if (rc != CLUE_OK) return rc;
+    k_radix_scatter<<<nblocks, kThreads, 0, st>>>(kin, vin, n, shift, bpp, nblocks,
+                                                  p->hist.as<int32_t>(), last ? nullptr : kout,
+                                                  vout);
+    CLUE_LAUNCH_CHECK("radix_scatter");
+    // next pass reads what this one wrote
+    int32_t* next_vout = (vout == p->perm.as<int32_t>()) ? p->perm2.as<int32_t>() : p->perm.as<int32_t>();
+    int32_t* next_kout = (kout == p->key2.as<int32_t>()) ? p->key.as<int32_t>() : p->key2.as<int32_t>();
+    kin = kout;
+    vin = vout;
+    kout = next_kout;
+    vout = next_vout;
+  }
+  int32_t* perm = vin;  // final permutation: slot -> point id
+  p->last_perm = perm;
+
+  // ---- tile offsets
+  CLUE_MARK(3);
+  rc = scan_exclusive(p->count.as<int32_t>(), p->offsets.as<int32_t>(), T, p->part.as<int32_t>(), st);
+  if (rc != CLUE_OK) return rc;
+
+  // ---- tile-ordered SoA
+  const float h = kd.type == CLUE_KERNEL_FLAT ? kd.params[0] : 0.f;
+  k_gather<<<div_up(n, kThreads), kThreads, 0, st>>>(c, weights, perm, n, D, p->ld,
+                                                     p->xs.as<float>(), p->ws.as<float>(),
+                                                     p->hw.as<float>(), h);
+  CLUE_LAUNCH_CHECK("gather");
+
+  // ---- density + nearest-higher
+  k_fill_i32<<<1, 64, 0, st>>>(p->counters.as<int32_t>(), 0, 1);  // seed candidates
+  CLUE_LAUNCH_CHECK("fill");
+  PassArgs a{};
+  for (int d = 0; d < D; ++d) a.xs[d] = p->xs.as<float>() + d * p->ld;
+  a.ws = p->ws.as<float>();
+  a.hw = p->hw.as<float>();
+  a.rho_s = p->rho_s.as<float>();
+  a.perm = perm;
+  a.offsets = p->offsets.as<int32_t>();
+  a.grid = p->grid.as<DevGrid>();
+  a.gs = gs;
+  a.n = n;
+  for (int k = 0; k < 3; ++k) a.kp[k] = kd.params[k];
+  for (int d = 0; d < CLUE_MAX_DIM; ++d) a.mp[d] = md.params[d];
+  a.rho_s_out = p->rho_s.as<float>();
+  a.rho_out = rho_buf;
+  a.nh_out = nh_buf;
+  a.delta_out = p->delta.as<float>();
+  a.seed_candidates = p->counters.as<int32_t>();
+  rc = dispatch_dim(D, [&](auto dc_) {
+    return dispatch_metric_passes<decltype(dc_)::value>(md.type, a, kd.type, prm->dc, prm->dm, st, p);
+  });
+  if (rc != CLUE_OK) return rc;
+
+  // ---- seeds, ranks, labels
+  CLUE_MARK(6);
+  ClassifyArgs ca{};
+  ca.c = c;
+  ca.rho = rho_buf;
+  ca.delta = p->delta.as<float>();
+  ca.nh = nh_buf;
+  ca.is_seed = seed_buf;
+  ca.flag = p->flag.as<int32_t>();
+  ca.root = p->root.as<int32_t>();
+  ca.n = n;
+  ca.seed_dc = prm->seed_dc;
+  ca.rhoc = prm->rhoc;
+  for (int d = 0; d < CLUE_MAX_DIM; ++d) ca.mp[d] = md.params[d];
+  rc = dispatch_dim(D, [&](auto dc_) {
+    return dispatch_metric_classify<decltype(dc_)::value>(md.type, ca, st);
+  });
+  if (rc != CLUE_OK) return rc;
+  rc = scan_exclusive(p->flag.as<int32_t>(), p->flag.as<int32_t>(), n, p->part.as<int32_t>(), st);
+  if (rc != CLUE_OK) return rc;
+  k_capacity<<<1, 64, 0, st>>>(p->counters.as<int32_t>(), p->flag.as<int32_t>() + n,
+                               prm->seed_mode == CLUE_SEEDS_UNBOUNDED ? 1 : 0);
+  CLUE_LAUNCH_CHECK("capacity");
+  CLUE_MARK(7);
+  // Rounds with 4096 hops each cover any forest depth below 4096^rounds.
+  const int hops = 4096;
+  int rounds = 1;
+  for (int64_t span = hops; span < n; span *= hops) ++rounds;
+  for (int r = 0; r < rounds; ++r) {
+    k_labels<<<div_up(n, kThreads), kThreads, 0, st>>>(p->root.as<int32_t>(), n, hops, seed_buf,
+                                                        p->flag.as<int32_t>(),
+                                                        p->counters.as<int32_t>(), cluster_out,
+                                                        r == rounds - 1 ? 1 : 0);
+    CLUE_LAUNCH_CHECK("labels");
+  }
+
+  CLUE_MARK(8);
+  p->has_run = true;
+  p->last_stream = st;
+  p->last_n = n;
+  p->last_d = D;
+  p->last_npd = npd;
+  p->last_T = T;
+  return CLUE_OK;
+}
+
+}  // namespace clue_hip
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* clue_last_error_message(void) { return g_error.c_str(); }
+const char* clue_version(void) { return "clue-mi355x 0.1 (gfx950)"; }
+
+clue_status clue_device_count(int32_t* count) {
+  if (!count) return fail(CLUE_ERR_INVALID_ARGUMENT, "count is NULL");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) {
+    *count = 0;
+    return fail(CLUE_ERR_RUNTIME, std::string("No device available! ") + hipGetErrorString(e));
+  }
+  *count = c;
+  return CLUE_OK;
+}
+
+clue_status clue_set_device(int32_t device) {
+  CLUE_HIP(hipSetDevice(device));
+  return CLUE_OK;
+}
+
+clue_status clue_stream_create(int32_t device, void** stream) {
+  if (!stream) return fail(CLUE_ERR_INVALID_ARGUMENT, "stream is NULL");
+  CLUE_HIP(hipSetDevice(device));
+  hipStream_t s;
+  CLUE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return CLUE_OK;
+}
+
+clue_status clue_stream_destroy(void* stream) {
+  if (stream) CLUE_HIP(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+  return CLUE_OK;
+}
+
+clue_status clue_stream_synchronize(void* stream) {
+  CLUE_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  return CLUE_OK;
+}
+
+clue_status clue_malloc_device(int32_t device, size_t bytes, void** ptr) {
+  if (!ptr) return fail(CLUE_ERR_INVALID_ARGUMENT, "ptr is NULL");
+  CLUE_HIP(hipSetDevice(device));
+  CLUE_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+  return CLUE_OK;
+}
+
+clue_status clue_free_device(void* ptr) {
+  if (ptr) CLUE_HIP(hipFree(ptr));
+  return CLUE_OK;
+}
+
+clue_status clue_malloc_host(size_t bytes, void** ptr) {
+  if (!ptr) return fail(CLUE_ERR_INVALID_ARGUMENT, "ptr is NULL");
+  CLUE_HIP(hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault));
+  return CLUE_OK;
+}
+
+clue_status clue_free_host(void* ptr) {
+  if (ptr) CLUE_HIP(hipHostFree(ptr));
+  return CLUE_OK;
+}
+
+clue_status clue_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream) {
+  CLUE_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, static_cast<hipStream_t>(stream)));
+  return CLUE_OK;
+}
+
+clue_status clue_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
+  CLUE_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, static_cast<hipStream_t>(stream)));
+  return CLUE_OK;
+}
+
+clue_status clue_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream) {
+  CLUE_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, static_cast<hipStream_t>(stream)));
+  return CLUE_OK;
+}
+
+clue_status clue_memset_device(void* dst, int32_t value, size_t bytes, void* stream) {
+  CLUE_HIP(hipMemsetAsync(dst, value, bytes, static_cast<hipStream_t>(stream)));
+  return CLUE_OK;
+}
+
+clue_status clue_plan_create(int32_t device, clue_plan_t* plan) {
+  if (!plan) return fail(CLUE_ERR_INVALID_ARGUMENT, "plan is NULL");
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) return fail(CLUE_ERR_RUNTIME, "No device available!");
+  if (device < 0 || device >= count) return fail(CLUE_ERR_OUT_OF_RANGE, "device index out of range");
+  clue_plan_s* p = new (std::nothrow) clue_plan_s();
+  if (!p) return fail(CLUE_ERR_OUT_OF_MEMORY, "plan allocation failed");
+  p->device = device;
+  *plan = p;
+  return CLUE_OK;
+}
+
+clue_status clue_plan_destroy(clue_plan_t plan) {
+  if (!plan) return CLUE_OK;
+  (void)hipSetDevice(plan->device);
+  if (plan->has_run && plan->last_stream) (void)hipStreamSynchronize(plan->last_stream);
+  for (DevBuf* b : {&plan->key, &plan->key2, &plan->perm, &plan->perm2, &plan->count, &plan->offsets,
+                    &plan->hist, &plan->part, &plan->xs, &plan->ws, &plan->hw, &plan->rho_s,
+                    &plan->delta, &plan->flag, &plan->root, &plan->nh_tmp, &plan->rho_tmp,
+                    &plan->seed_tmp, &plan->bbox_part, &plan->grid, &plan->counters})
+    b->release();
+  for (hipEvent_t& e : plan->ev)
+    if (e) (void)hipEventDestroy(e);
+  delete plan;
+  return CLUE_OK;
+}
+
+clue_status clue_plan_reset_capacity(clue_plan_t plan) {
+  if (!plan) return fail(CLUE_ERR_INVALID_ARGUMENT, "plan is NULL");
+  if (plan->counters.p) {
+    CLUE_HIP(hipSetDevice(plan->device));
+    CLUE_HIP(hipDeviceSynchronize());
+    CLUE_HIP(hipMemset(plan->counters.p, 0, 16 * 4));
+  }
+  return CLUE_OK;
+}
+
+clue_status clue_plan_reserve(clue_plan_t plan, int32_t ndim, int32_t npoints) {
+  if (!plan) return fail(CLUE_ERR_INVALID_ARGUMENT, "plan is NULL");
+  if (ndim < 1 || ndim > CLUE_MAX_DIM || npoints <= 0)
+    return fail(CLUE_ERR_INVALID_ARGUMENT, "invalid reserve size");
+  int32_t npd, T;
+  grid_shape(ndim, npoints, 128, npd, T);
+  return reserve(plan, ndim, npoints, T);
+}
+
+clue_status clue_plan_device_bytes(clue_plan_t plan, size_t* bytes) {
+  if (!plan || !bytes) return fail(CLUE_ERR_INVALID_ARGUMENT, "NULL argument");
+  size_t s = 0;
+  for (DevBuf* b : {&plan->key, &plan->key2, &plan->perm, &plan->perm2, &plan->count, &plan->offsets,
+                    &plan->hist, &plan->part, &plan->xs, &plan->ws, &plan->hw, &plan->rho_s,
+                    &plan->delta, &plan->flag, &plan->root, &plan->nh_tmp, &plan->rho_tmp,
+                    &plan->seed_tmp, &plan->bbox_part, &plan->grid, &plan->counters})
+    s += b->bytes;
+  *bytes = s;
+  return CLUE_OK;
+}
+
+clue_status clue_make_clusters(clue_plan_t plan, void* stream, const clue_params* params,
+                               int32_t ndim, int32_t n, const float* const* coords_dev,
+                               const float* weights_dev, const clue_kernel_desc* kernel,
+                               const clue_metric_desc* metric, const uint8_t* wrapped,
+                               int32_t* cluster_out, int32_t* is_seed_out, float* rho_out,
+                               int32_t* nearest_higher_out) {
+  if (!plan) return fail(CLUE_ERR_INVALID_ARGUMENT, "plan is NULL");
+  CLUE_HIP(hipSetDevice(plan->device));
+  return make_clusters(plan, static_cast<hipStream_t>(stream), params, ndim, n, coords_dev,
+                       weights_dev, kernel, metric, wrapped, cluster_out, is_seed_out, rho_out,
+                       nearest_higher_out);
+}
+
+clue_status clue_main_run(float dc, float rhoc, float dm, float seed_dc, int32_t points_per_tile,
+                          const uint8_t* wrapped, const float* data, int32_t* results,
+                          const clue_kernel_desc* kernel, int32_t ndim, int32_t n,
+                          int32_t block_size, int32_t device) {
+  (void)block_size;  // the engine picks its own launch shapes
+  if (!data || !results) return fail(CLUE_ERR_INVALID_ARGUMENT, "data and results are required");
+  if (ndim < 1 || ndim > CLUE_MAX_DIM) return fail(CLUE_ERR_INVALID_ARGUMENT, "Ndim must be in [1, 10]");
+  if (n <= 0)
+    return fail(CLUE_ERR_INVALID_ARGUMENT,
+                "Number of points passed to PointsDevice constructor must be positive.");
+  clue_plan_t plan = nullptr;
+  clue_status rc = clue_plan_create(device, &plan);
+  if (rc != CLUE_OK) return rc;
+  void* stream = nullptr;
+  float* dbuf = nullptr;
+  int32_t* dout = nullptr;
+  clue_params prm{dc, rhoc, dm, seed_dc, points_per_tile, CLUE_SEEDS_REFERENCE};
+  const size_t nn = static_cast<size_t>(n);
+  rc = clue_stream_create(device, &stream);
+  if (rc == CLUE_OK) rc = clue_malloc_device(device, (ndim + 1) * nn * 4, reinterpret_cast<void**>(&dbuf));
+  if (rc == CLUE_OK) rc = clue_malloc_device(device, nn * 4, reinterpret_cast<void**>(&dout));
+  if (rc == CLUE_OK) rc = clue_memcpy_h2d(dbuf, data, (ndim + 1) * nn * 4, stream);
+  if (rc == CLUE_OK) {
+    const float* cp[CLUE_MAX_DIM];
+    for (int d = 0; d < ndim; ++d) cp[d] = dbuf + d * nn;
+    rc = clue_make_clusters(plan, stream, &prm, ndim, n, cp, dbuf + ndim * nn, kernel, nullptr,
+                            wrapped, dout, nullptr, nullptr, nullptr);
+  }
+  if (rc == CLUE_OK) rc = clue_memcpy_d2h(results, dout, nn * 4, stream);
+  if (rc == CLUE_OK) rc = clue_stream_synchronize(stream);
+  const std::string keep = g_error;
+  clue_plan_destroy(plan);
+  clue_free_device(dbuf);
+  clue_free_device(dout);
+  clue_stream_destroy(stream);
+  g_error = keep;
+  return rc;
+}
+
+clue_status clue_plan_last_stats(clue_plan_t plan, clue_stats* stats) {
+  if (!plan || !stats) return fail(CLUE_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!plan->has_run) return fail(CLUE_ERR_INVALID_ARGUMENT, "the plan has not run");
+  CLUE_HIP(hipSetDevice(plan->device));
+  CLUE_HIP(hipStreamSynchronize(plan->last_stream));
+  int32_t c[16];
+  CLUE_HIP(hipMemcpy(c, plan->counters.p, sizeof(c), hipMemcpyDeviceToHost));
+  stats->seed_candidates = c[0];
+  stats->seed_capacity = c[2];
+  stats->n_seeds = c[3];
+  stats->n_clusters = c[4];
+  stats->n_dropped_seeds = c[5];
+  return CLUE_OK;
+}
+
+clue_status clue_plan_last_grid(clue_plan_t plan, clue_grid_info* grid) {
+  if (!plan || !grid) return fail(CLUE_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!plan->has_run) return fail(CLUE_ERR_INVALID_ARGUMENT, "the plan has not run");
+  CLUE_HIP(hipSetDevice(plan->device));
+  CLUE_HIP(hipStreamSynchronize(plan->last_stream));
+  DevGrid g;
+  CLUE_HIP(hipMemcpy(&g, plan->grid.p, sizeof(g), hipMemcpyDeviceToHost));
+  std::memset(grid, 0, sizeof(*grid));
+  grid->ndim = plan->last_d;
+  grid->npoints = plan->last_n;
+  grid->npd = plan->last_npd;
+  grid->ntiles = plan->last_T;
+  for (int d = 0; d < plan->last_d; ++d) {
+    grid->minmax[2 * d] = g.mn[d];
+    grid->minmax[2 * d + 1] = g.mx[d];
+    grid->tile_size[d] = g.ts[d];
+  }
+  return CLUE_OK;
+}
+
+clue_status clue_plan_copy_tiles(clue_plan_t plan, int32_t* offsets, int32_t* indexes) {
+  if (!plan || !offsets || !indexes) return fail(CLUE_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!plan->has_run) return fail(CLUE_ERR_INVALID_ARGUMENT, "the plan has not run");
+  CLUE_HIP(hipSetDevice(plan->device));
+  CLUE_HIP(hipStreamSynchronize(plan->last_stream));
+  CLUE_HIP(hipMemcpy(offsets, plan->offsets.p, (static_cast<size_t>(plan->last_T) + 1) * 4,
+                     hipMemcpyDeviceToHost));
+  CLUE_HIP(hipMemcpy(indexes, plan->last_perm, static_cast<size_t>(plan->last_n) * 4,
+                     hipMemcpyDeviceToHost));
+  return CLUE_OK;
+}
+
+clue_status clue_plan_copy_delta(clue_plan_t plan, float* delta) {
+  if (!plan || !delta) return fail(CLUE_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!plan->has_run) return fail(CLUE_ERR_INVALID_ARGUMENT, "the plan has not run");
+  CLUE_HIP(hipSetDevice(plan->device));
+  CLUE_HIP(hipStreamSynchronize(plan->last_stream));
+  CLUE_HIP(hipMemcpy(delta, plan->delta.p, static_cast<size_t>(plan->last_n) * 4,
+                     hipMemcpyDeviceToHost));
+  return CLUE_OK;
+}
+
+clue_status clue_plan_set_timing(clue_plan_t plan, int32_t enable) {
+  if (!plan) return fail(CLUE_ERR_INVALID_ARGUMENT, "plan is NULL");
+  plan->timing = enable != 0;
+  return CLUE_OK;
+}
+
+clue_status clue_plan_stage_times(clue_plan_t plan, float* ms, int32_t capacity, int32_t* count) {
+  if (!plan || !count) return fail(CLUE_ERR_INVALID_ARGUMENT, "NULL argument");
+  *count = kStages;
+  if (!ms) return CLUE_OK;
+  if (!plan->has_run || !plan->timing) return fail(CLUE_ERR_INVALID_ARGUMENT, "no timed call");
+  CLUE_HIP(hipSetDevice(plan->device));
+  CLUE_HIP(hipEventSynchronize(plan->ev[kStages]));
+  for (int k = 0; k < kStages && k < capacity; ++k)
+    CLUE_HIP(hipEventElapsedTime(&ms[k], plan->ev[k], plan->ev[k + 1]));
+  return CLUE_OK;
+}
+
+const char* clue_stage_name(int32_t stage) {
+  return (stage >= 0 && stage < kStages) ? kStageNames[stage] : "";
+}
+
+clue_status clue_selftest_math(int32_t op, const float* a_dev, const float* b_dev, float* out_dev,
+                               int64_t n, void* stream) {
+  if (n <= 0) return CLUE_OK;
+  if (!a_dev || !out_dev || (op == 2 && !b_dev))
+    return fail(CLUE_ERR_INVALID_ARGUMENT, "NULL argument");
+  k_selftest<<<div_up(n, kThreads), kThreads, 0, static_cast<hipStream_t>(stream)>>>(op, a_dev, b_dev,
+                                                                                   out_dev, n);
+  CLUE_LAUNCH_CHECK("selftest");
+  return CLUE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,193 @@
+/*
+ * clue_abi.h -- C ABI of the MI355X CLUE engine (libclue_hip.so).
+ *
+ * Plain pointers and sizes only.  All device pointers are HIP device memory on
+ * the plan's GPU; streams are hipStream_t passed as void*.  Every call returns a
+ * clue_status; on failure clue_last_error_message() describes the error of the
+ * calling thread.  The C++20 drop-in headers (CLUEstering/CLUEstering.hpp)
+ * translate the status codes back into the reference's exception types
+ * (std::invalid_argument, std::out_of_range, std::runtime_error).
+ *
+ * Which reference interface each entry point replaces is cited per function,
+ * as file:line into CLUEstering 2.9.0 (alpaka3 port).
+ */
+#ifndef CLUE_ABI_H
+#define CLUE_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLUE_MAX_DIM 10
+
+typedef enum {
+  CLUE_OK = 0,
+  CLUE_ERR_INVALID_ARGUMENT = 1, /* std::invalid_argument in the reference */
+  CLUE_ERR_OUT_OF_RANGE = 2,     /* std::out_of_range */
+  CLUE_ERR_RUNTIME = 3,          /* std::runtime_error (HIP failure, no device) */
+  CLUE_ERR_OUT_OF_MEMORY = 4,
+  CLUE_ERR_UNSUPPORTED = 5
+} clue_status;
+
+/* Convolutional kernels: core/ConvolutionalKernel.hpp:14-85 (FlatKernel,
+ * GaussianKernel, ExponentialKernel).  params: flat {h} | gaussian {mean, std,
+ * amplitude} | exponential {avg, amplitude}.  The engine evaluates exp() as the
+ * correctly rounded float exponential (double-precision table method). */
+enum { CLUE_KERNEL_FLAT = 0, CLUE_KERNEL_GAUSSIAN = 1, CLUE_KERNEL_EXPONENTIAL = 2 };
+typedef struct {
+  int32_t type;
+  float params[3];
+} clue_kernel_desc;
+
+/* Distance metrics: core/DistanceMetrics.hpp:33-230.  params hold the per-
+ * dimension weights (weighted metrics) or periods (periodic Euclidean). */
+enum {
+  CLUE_METRIC_EUCLIDEAN = 0,
+  CLUE_METRIC_WEIGHTED_EUCLIDEAN = 1,
+  CLUE_METRIC_PERIODIC_EUCLIDEAN = 2,
+  CLUE_METRIC_MANHATTAN = 3,
+  CLUE_METRIC_CHEBYSHEV = 4,
+  CLUE_METRIC_WEIGHTED_CHEBYSHEV = 5
+};
+typedef struct {
+  int32_t type;
+  float params[CLUE_MAX_DIM];
+} clue_metric_desc;
+
+/* Seed storage.  REFERENCE reproduces SeedArrayView::push_back
+ * (data_structures/internal/SeedArray.hpp:33-41): the seed array holds
+ * max(previous capacity, #points with no nearest-higher) entries and seeds past
+ * it, in ascending point order, are dropped (their clusters become -1).
+ * UNBOUNDED keeps every seed. */
+enum { CLUE_SEEDS_REFERENCE = 0, CLUE_SEEDS_UNBOUNDED = 1 };
+
+/* Clusterer parameters: core/Clusterer.hpp:40-49 (dc, seed_dc, rhoc, dm, pPBin).
+ * The reference validation applies (detail/Clusterer.hpp:32-35). */
+typedef struct {
+  float dc;
+  float rhoc;
+  float dm;
+  float seed_dc;
+  int32_t points_per_tile;
+  int32_t seed_mode;
+} clue_params;
+
+/* Per-call counters, read back with clue_plan_last_stats (synchronises). */
+typedef struct {
+  int64_t seed_candidates; /* points with nh == -1 after the nearest-higher stage */
+  int64_t seed_capacity;   /* capacity used for this call */
+  int64_t n_seeds;         /* points flagged as seeds */
+  int64_t n_clusters;      /* stored seeds = clusters */
+  int64_t n_dropped_seeds; /* seeds lost to the capacity (reference mode) */
+} clue_stats;
+
+/* Grid of the last call: npd tiles per dimension, npd^ndim tiles; minmax in
+ * CoordinateExtremes order (min0,max0,min1,max1,...). */
+typedef struct {
+  int32_t ndim;
+  int32_t npoints;
+  int32_t npd;
+  int32_t ntiles;
+  float minmax[2 * CLUE_MAX_DIM];
+  float tile_size[CLUE_MAX_DIM];
+} clue_grid_info;
+
+typedef struct clue_plan_s* clue_plan_t;
+
+/* ---- errors ------------------------------------------------------------- */
+const char* clue_last_error_message(void);
+const char* clue_version(void);
+
+/* ---- devices and streams: internal/alpaka/devices.hpp:108-133 (DevicePool),
+ *      utils/get_queue.hpp:14-27 (get_queue), alpaka::onHost::wait ---------- */
+clue_status clue_device_count(int32_t* count);
+clue_status clue_set_device(int32_t device);
+clue_status clue_stream_create(int32_t device, void** stream);
+clue_status clue_stream_destroy(void* stream);
+clue_status clue_stream_synchronize(void* stream);
+
+/* ---- buffers: internal/alpaka/memory.hpp:34-67 (make_device_buffer /
+ *      make_host_buffer), PointsDevice.hpp:19-99 (copyToHost/copyToDevice) --- */
+clue_status clue_malloc_device(int32_t device, size_t bytes, void** ptr);
+clue_status clue_free_device(void* ptr);
+clue_status clue_malloc_host(size_t bytes, void** ptr); /* pinned */
+clue_status clue_free_host(void* ptr);
+clue_status clue_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+clue_status clue_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+clue_status clue_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
+clue_status clue_memset_device(void* dst, int32_t value, size_t bytes, void* stream);
+
+/* ---- persistent clustering state: the Clusterer's grow-only scratch
+ *      (optional<Tiles>, optional<SeedArray>, optional<Followers>,
+ *      core/Clusterer.hpp:40-49; SetupTiles.hpp:28-37, SetupSeeds.hpp:14-18).
+ *      A plan is bound to one device and is not reentrant. ------------------ */
+clue_status clue_plan_create(int32_t device, clue_plan_t* plan);
+clue_status clue_plan_destroy(clue_plan_t plan);
+/* Forget the grow-only seed capacity (a fresh Clusterer). */
+clue_status clue_plan_reset_capacity(clue_plan_t plan);
+/* Pre-size the scratch for up to npoints points in ndim dimensions. */
+clue_status clue_plan_reserve(clue_plan_t plan, int32_t ndim, int32_t npoints);
+/* Device bytes currently held by the plan. */
+clue_status clue_plan_device_bytes(clue_plan_t plan, size_t* bytes);
+
+/* ---- the clustering path: Clusterer::make_clusters(queue, PointsDevice&,
+ *      metric, kernel, block_size), core/detail/Clusterer.hpp:84-95 and
+ *      make_clusters_impl :134-217 (tile fill, K4 density, K5 nearest-higher,
+ *      K6 seeds, K7 labels).
+ *
+ * coords_dev: host array of ndim device pointers, coordinate d of point i at
+ *             coords_dev[d][i] (PointsDevice SoA, detail/PointsDevice.hpp:24-52).
+ * weights_dev: n device floats.
+ * wrapped:    host array of ndim flags (Clusterer::setWrappedCoordinates), or NULL.
+ * Outputs (device, n entries each, original point order): cluster is required;
+ * rho, nearest_higher (after seed promotion: seeds hold -1), is_seed may be NULL.
+ * Enqueued on `stream`; nothing is synchronised.  Results equal the reference's
+ * serial backend (labels in ascending-seed order, i.e. the serial labels). ---- */
+clue_status clue_make_clusters(clue_plan_t plan, void* stream, const clue_params* params,
+                               int32_t ndim, int32_t n, const float* const* coords_dev,
+                               const float* weights_dev, const clue_kernel_desc* kernel,
+                               const clue_metric_desc* metric, const uint8_t* wrapped,
+                               int32_t* cluster_out, int32_t* is_seed_out, float* rho_out,
+                               int32_t* nearest_higher_out);
+
+/* Host-buffer convenience: the reference's Python FFI entry
+ * mainRun(dc, rhoc, dm, seed_dc, pPBin, wrapped, data, results, kernel, Ndim,
+ * n_points, block_size, device_id), CLUEstering/BindingModules/serial/
+ * binding_cpu.cpp:35-52.  data: host float[(ndim+1)*n] SoA (coords then
+ * weights); results: host int32[n].  Copies in, clusters with a fresh plan on
+ * `device`, copies labels out, synchronises. */
+clue_status clue_main_run(float dc, float rhoc, float dm, float seed_dc, int32_t points_per_tile,
+                          const uint8_t* wrapped, const float* data, int32_t* results,
+                          const clue_kernel_desc* kernel, int32_t ndim, int32_t n,
+                          int32_t block_size, int32_t device);
+
+/* ---- introspection of the last call (synchronise the plan's last stream) -- */
+clue_status clue_plan_last_stats(clue_plan_t plan, clue_stats* stats);
+clue_status clue_plan_last_grid(clue_plan_t plan, clue_grid_info* grid);
+/* Tile CSR of the last call, host buffers: offsets[ntiles+1], indexes[npoints]
+ * (Tiles / AssociationMap offsets+indexes, AssociationMap.hpp:122-191). */
+clue_status clue_plan_copy_tiles(clue_plan_t plan, int32_t* offsets, int32_t* indexes);
+/* Nearest-higher distance of the last call in original order (FLT_MAX when the
+ * point has no nearest-higher), host buffer of npoints floats. */
+clue_status clue_plan_copy_delta(clue_plan_t plan, float* delta);
+
+/* ---- per-stage timing (HIP events recorded on the call's stream; no
+ *      reference counterpart -- the reference has no built-in profiling). ---- */
+clue_status clue_plan_set_timing(clue_plan_t plan, int32_t enable);
+/* Milliseconds per stage of the last timed call; *count receives the number of
+ * stages (ms may be NULL to query it).  Synchronises on the last event. */
+clue_status clue_plan_stage_times(clue_plan_t plan, float* ms, int32_t capacity, int32_t* count);
+const char* clue_stage_name(int32_t stage);
+
+/* ---- diagnostics used by the parity tests: the engine's device arithmetic
+ *      on device arrays (expf as the kernels use it; sqrtf; division). ------- */
+clue_status clue_selftest_math(int32_t op, const float* a_dev, const float* b_dev, float* out_dev,
+                               int64_t n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

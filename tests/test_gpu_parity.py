@@ -1,0 +1,221 @@
+"""Parity of the HIP engine (through the C ABI) with the oracle, the CPU
+restatement of the reference's serial backend.  Integer outputs (tile CSR,
+nearest-higher, seed flags, labels) must be bit-exact; densities are compared
+bit-exactly too, because the engine accumulates in the reference's visit order
+and evaluates exp() with the same correctly rounded definition as the oracle's
+ENGINE mode."""
+import numpy as np
+import pytest
+
+import cluestering_alpaka3_amd as clue
+import oracle
+from cluestering_alpaka3_amd import datasets
+
+pytestmark = pytest.mark.gpu
+
+
+def run_both(coords, weights, dc, rhoc, dm=None, seed_dc=None, ppb=128, kernel=("flat", 0.5),
+             metric="euclidean", metric_params=None, wrapped=None, seed_mode=0, plan=None):
+    print(f"case: ndim={np.atleast_2d(coords).shape[0]} n={np.atleast_2d(coords).shape[1]} dc={dc} "
+          f"rhoc={rhoc} dm={dm} kernel={kernel} metric={metric} wrapped={wrapped}", flush=True)
+    own = plan is None
+    plan = plan or clue.Plan(0)
+    try:
+        got = clue.cluster_points(coords, weights, dc, rhoc, dm, seed_dc, ppb, kernel, metric, metric_params,
+                                  wrapped, seed_mode, plan=plan)
+        tiles = plan.tiles()
+        grid = plan.grid()
+        stats = plan.stats()
+        delta = plan.delta()
+    finally:
+        if own:
+            plan.close()
+    p = oracle.make_params(dc, rhoc, dm, seed_dc, ppb, kernel, metric, metric_params, wrapped,
+                           seed_mode=seed_mode)
+    ref = oracle.make_clusters(coords, weights, p)
+    return got, ref, tiles, grid, stats, delta
+
+
+def assert_same(got, ref, tiles, grid, stats, delta):
+    assert grid["npd"] == ref.grid["npd"] and grid["ntiles"] == ref.grid["ntiles"]
+    np.testing.assert_array_equal(grid["minmax"], ref.grid["minmax"])
+    np.testing.assert_array_equal(grid["tile_size"], ref.grid["tile_size"])
+    np.testing.assert_array_equal(tiles[0], ref.tile_offsets)
+    np.testing.assert_array_equal(tiles[1], ref.tile_indexes)
+    np.testing.assert_array_equal(got.rho.view(np.int32), ref.rho.view(np.int32))
+    np.testing.assert_array_equal(delta.view(np.int32), ref.delta.view(np.int32))
+    np.testing.assert_array_equal(got.nearest_higher, ref.nh)
+    np.testing.assert_array_equal(got.is_seed, ref.is_seed)
+    np.testing.assert_array_equal(got.cluster, ref.cluster)
+    assert stats["seed_candidates"] == ref.stats["seed_candidates"]
+    assert stats["n_seeds"] == ref.stats["n_seeds"]
+    assert stats["n_clusters"] == ref.stats["n_stored_seeds"]
+    assert stats["n_dropped_seeds"] == ref.stats["n_dropped_seeds"]
+
+
+def test_device_math_matches_host():
+    # expf: dense sweep of the whole range the kernels can meet, plus specials
+    x = np.concatenate([np.linspace(-111.0, 90.0, 400001, dtype=np.float32),
+                        np.array([0.0, -0.0, 1e-30, -1e-30, 88.72, 88.73, -103.9, -104.0, np.inf, -np.inf],
+                                 np.float32)])
+    got = clue.selftest_math(0, x)
+    want = oracle.expf_engine(x)
+    np.testing.assert_array_equal(got.view(np.int32), want.view(np.int32))
+    # sqrtf and division are correctly rounded on both sides
+    rng = np.random.default_rng(1)
+    a = np.abs(rng.standard_normal(200000).astype(np.float32)) * np.float32(1e3)
+    a[:1000] = rng.random(1000).astype(np.float32) * np.float32(1e-38)  # subnormal range
+    np.testing.assert_array_equal(clue.selftest_math(1, a).view(np.int32), np.sqrt(a).view(np.int32))
+    b = (rng.random(200000).astype(np.float32) + np.float32(0.01)) * np.float32(7.3)
+    np.testing.assert_array_equal(clue.selftest_math(2, a, b).view(np.int32), (a / b).view(np.int32))
+    # sqrt_lt_bound: largest s with sqrtf(s) < delta
+    delta = np.abs(rng.standard_normal(20000).astype(np.float32)) * np.float32(10) + np.float32(1e-3)
+    bnd = clue.selftest_math(3, delta)
+    assert np.all(np.sqrt(bnd) < delta)
+    nxt = np.nextafter(bnd, np.float32(np.inf))
+    assert np.all(np.sqrt(nxt) >= delta)
+
+
+WORKLOAD_CASES = [("2d_16k_flat", 16384), ("2d_1m_gauss", 65536), ("3d_4m_exp", 65536),
+                  ("10d_256k_flat", 8192)]
+
+
+@pytest.mark.parametrize("name,n", WORKLOAD_CASES)
+def test_workload_parity(name, n):
+    ps = datasets.workload(name, n=n)
+    prm = datasets.WORKLOADS[name]
+    assert_same(*run_both(ps.coords, ps.weights, prm["dc"], prm["rhoc"], prm["dm"], kernel=prm["kernel"]))
+
+
+def test_config1_golden(config1_golden):
+    g = config1_golden
+    res = clue.cluster_points(g["coords"], g["weights"], 1.5, 5.0, 3.0, kernel=("flat", 0.5))
+    np.testing.assert_array_equal(res.rho.view(np.int32), g["rho"].view(np.int32))
+    np.testing.assert_array_equal(res.nearest_higher, g["nh"])
+    np.testing.assert_array_equal(res.is_seed, g["is_seed"])
+    np.testing.assert_array_equal(res.cluster, g["cluster"])
+
+
+# Parameters the reference's own tests use on its datasets
+# (tests/test_clustering.cpp:11-131, tests/test_*_dataset.py).
+REF_CASES = [
+    ("data_1024", 1.5, 10.0, 1.5), ("data_4096", 1.5, 10.0, 1.5), ("data_16384", 1.5, 10.0, 1.5),
+    ("data_32768", 1.5, 10.0, 1.5), ("data_32768", 1.3, 10.0, 1.3),
+    ("toyDetector_1000", 4.0, 2.5, 4.0), ("toyDetector_10000", 4.0, 2.5, 4.0),
+    ("sissa_1000", 20.0, 10.0, 20.0), ("sissa_4000", 20.0, 10.0, 20.0),
+    ("aniso_1000", 25.0, 5.0, 23.0), ("blob", 1.0, 5.0, 2.0), ("moons", 21.0, 10.0, 21.0),
+    ("square", 1.0, 2.0, 1.6), ("box", 1.0, 2.0, 1.6), ("data_dim_3", 1.5, 10.0, 1.5),
+    ("data_dim_10", 3.0, 10.0, 6.0),
+]
+
+
+@pytest.mark.parametrize("name,dc,rhoc,dm", REF_CASES)
+def test_reference_dataset_parity(ref_data, name, dc, rhoc, dm):
+    c, w = ref_data(name)
+    assert_same(*run_both(c, w, dc, rhoc, dm))
+
+
+def test_known_answer_twenty_clusters(ref_data):
+    # tests/test_host_points.cpp:437-450 and tests/test_device_points.cpp:252-268
+    c, w = ref_data("data_32768")
+    res = clue.cluster_points(c, w, 1.3, 10.0, 1.3)
+    assert int(res.cluster.max()) + 1 == 20
+
+
+@pytest.mark.parametrize("metric,mp", [("weighted_euclidean", [0.5, 2.0]), ("periodic_euclidean", [200.0, 0.0]),
+                                       ("manhattan", None), ("chebyshev", None),
+                                       ("weighted_chebyshev", [1.5, 0.75])])
+@pytest.mark.parametrize("kernel", [("flat", 0.5), ("gaussian", 0.5, 1.2, 2.0), ("exponential", 0.8, 1.5)])
+def test_metrics_and_kernels(ref_data, metric, mp, kernel):
+    c, w = ref_data("toyDetector_1000")
+    assert_same(*run_both(c, w, 4.0, 2.5, 4.0, kernel=kernel, metric=metric, metric_params=mp))
+
+
+@pytest.mark.parametrize("kernel", [("gaussian", 0.0, 1.0, 1.0), ("exponential", 1.0, 1.0)])
+def test_kernels_euclidean(kernel):
+    ps = datasets.workload("2d_1m_gauss", n=32768, seed=77)
+    assert_same(*run_both(ps.coords, ps.weights, 2.0, 8.0, 4.0, kernel=kernel))
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
+def test_every_dimension(ndim):
+    ps = datasets.blobs(3000, ndim, 6, 0.0, 40.0, 1.0, 0.1, ("uniform", 0.5, 1.5), 50 + ndim)
+    assert_same(*run_both(ps.coords, ps.weights, 1.5, 3.0, 2.5, kernel=("gaussian", 0.3, 1.0, 1.0)))
+
+
+def test_seed_capacity_truncation():
+    # dm > seed_dc: seeds can outnumber the points without a nearest-higher; the
+    # reference drops the highest-index seeds (SeedArray.hpp:33-41)
+    ps = datasets.blobs(20000, 2, 60, 0.0, 200.0, 1.0, 0.0, ("const", 1.0), 99)
+    got, ref, *rest = run_both(ps.coords, ps.weights, 0.6, 1.0, 5.0, seed_dc=0.6)
+    assert ref.stats["n_dropped_seeds"] > 0
+    assert_same(got, ref, *rest)
+    got2, ref2, *rest2 = run_both(ps.coords, ps.weights, 0.6, 1.0, 5.0, seed_dc=0.6, seed_mode=1)
+    assert ref2.stats["n_dropped_seeds"] == 0
+    assert_same(got2, ref2, *rest2)
+
+
+def test_grow_only_capacity_across_calls():
+    # SetupSeeds.hpp:14-18: a Clusterer keeps the largest capacity it has seen
+    big = datasets.blobs(20000, 2, 60, 0.0, 200.0, 1.0, 0.0, ("const", 1.0), 99)
+    small = datasets.blobs(20000, 2, 60, 0.0, 200.0, 1.0, 0.0, ("const", 1.0), 98)
+    plan = clue.Plan(0)
+    try:
+        clue.cluster_points(small.coords, small.weights, 0.6, 1.0, 5.0, 0.6, plan=plan)
+        cap_small = plan.stats()["seed_capacity"]
+        clue.cluster_points(big.coords, big.weights, 0.6, 1.0, 5.0, 0.6, plan=plan)
+        st = plan.stats()
+        p1 = oracle.make_params(0.6, 1.0, 5.0, 0.6)
+        r_big_fresh = oracle.make_clusters(big.coords, big.weights, p1)
+        p2 = oracle.make_params(0.6, 1.0, 5.0, 0.6, seed_capacity_floor=cap_small)
+        r_big = oracle.make_clusters(big.coords, big.weights, p2)
+        assert st["seed_capacity"] == max(cap_small, r_big_fresh.stats["seed_candidates"])
+        assert st["n_clusters"] == r_big.stats["n_stored_seeds"]
+    finally:
+        plan.close()
+
+
+def test_wrapped_coordinates(ref_data):
+    c, w = ref_data("opposite_angles")
+    assert_same(*run_both(c, w, 0.2, 5.0, 0.2, wrapped=[0, 1]))
+    ps = datasets.blobs(5000, 2, 8, 0.0, 50.0, 1.0, 0.1, ("const", 1.0), 4242)
+    assert_same(*run_both(ps.coords, ps.weights, 1.0, 3.0, 2.0, wrapped=[1, 1]))
+
+
+def test_edge_cases():
+    # one point; identical points (zero tile size); two points; tiny clusters
+    assert_same(*run_both(np.zeros((2, 1), np.float32), np.ones(1, np.float32), 1.0, 0.5))
+    assert_same(*run_both(np.full((3, 257), 2.5, np.float32), np.ones(257, np.float32), 1.0, 0.5))
+    assert_same(*run_both(np.array([[0.0, 0.5]], np.float32), np.array([1.0, 2.0], np.float32), 1.0, 0.5))
+    rng = np.random.default_rng(3)
+    x = np.round(rng.random((2, 3000)) * 20).astype(np.float32)  # many exact ties in rho and distance
+    assert_same(*run_both(x, np.ones(3000, np.float32), 1.0, 2.0, 2.0))
+    assert_same(*run_both(x, np.ones(3000, np.float32), 1.0, 2.0, 2.0, ppb=7))
+    assert_same(*run_both(x, np.ones(3000, np.float32), 1.0, 2.0, 2.0, ppb=100000))
+
+
+def test_invalid_arguments():
+    with pytest.raises(clue.InvalidArgument):
+        clue.cluster_points(np.zeros((2, 4), np.float32), np.ones(4, np.float32), -1.0, 1.0)
+    with pytest.raises(clue.InvalidArgument):
+        clue.cluster_points(np.zeros((2, 4), np.float32), np.ones(4, np.float32), 1.0, -1.0)
+    with pytest.raises(clue.InvalidArgument):
+        clue.cluster_points(np.zeros((11, 4), np.float32), np.ones(4, np.float32), 1.0, 1.0)
+
+
+def test_main_run_matches(ref_data):
+    c, w = ref_data("toyDetector_1000")
+    n = c.shape[1]
+    data = np.concatenate([c.reshape(-1), w]).astype(np.float32)
+    results = np.empty(n, np.int32)
+    clue.main_run(4.0, 2.5, 4.0, 4.0, 128, [0, 0], data, results, ("flat", 0.5), 2, n, 256, 0)
+    ref = oracle.make_clusters(c, w, oracle.make_params(4.0, 2.5, 4.0))
+    np.testing.assert_array_equal(results, ref.cluster)
+
+
+def test_deterministic_repeat():
+    ps = datasets.workload("2d_1m_gauss", n=100000, seed=5)
+    a = clue.cluster_points(ps.coords, ps.weights, 2.0, 8.0, 4.0, kernel=("gaussian", 0.0, 1.0, 1.0))
+    b = clue.cluster_points(ps.coords, ps.weights, 2.0, 8.0, 4.0, kernel=("gaussian", 0.0, 1.0, 1.0))
+    for x, y in zip((a.rho, a.nearest_higher, a.is_seed, a.cluster), (b.rho, b.nearest_higher, b.is_seed, b.cluster)):
+        np.testing.assert_array_equal(x, y)

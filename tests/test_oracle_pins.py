@@ -1,0 +1,420 @@
+"""Pins of the oracle (CPU, no GPU).
+
+The oracle restates the reference's serial backend.  These tests check it
+against what does not depend on the oracle's own code: the reference's known
+answers and quality thresholds on its own datasets, closed forms, FP64 and
+exact-rational recomputations, and O(N^2) brute force written independently in
+numpy.  test_planted_mistakes_are_caught turns on each planted mistake inside
+the oracle and asserts that at least one pin fails.
+"""
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle
+from cluestering_alpaka3_amd import datasets
+
+FLT_MAX = np.float32(3.4028234663852886e38)
+
+
+# ---------------------------------------------------------------------------
+# independent restatements (numpy float32 is IEEE single, no FMA)
+# ---------------------------------------------------------------------------
+def np_grid(coords, ppb=128):
+    D, n = coords.shape
+    nt = int(np.ceil(np.float32(n) / np.float32(ppb)))
+    npd = int(np.ceil(nt ** (1.0 / D)))
+    T = npd ** D
+    mn = coords.min(axis=1)
+    mx = coords.max(axis=1)
+    ts = ((mx - mn) / np.float32(npd)).astype(np.float32)
+    return npd, T, mn, mx, ts
+
+
+def np_bins(c, mn, ts, npd):
+    with np.errstate(divide="ignore", invalid="ignore"):
+        v = (np.float32(c) - np.float32(mn)) / np.float32(ts)
+    v = np.asarray(v, np.float32)
+    ok = (v >= -2147483648.0) & (v < 2147483648.0)
+    b = np.where(ok, np.trunc(np.where(ok, v, 0)).astype(np.int64), -(2 ** 31))
+    return np.clip(b, 0, npd - 1)
+
+
+def np_tiles(coords, ppb=128):
+    npd, T, mn, mx, ts = np_grid(coords, ppb)
+    D, n = coords.shape
+    g = np.zeros(n, np.int64)
+    for d in range(D):
+        g = g * npd + np_bins(coords[d], mn[d], ts[d], npd)
+    order = np.argsort(g, kind="stable")
+    offsets = np.zeros(T + 1, np.int64)
+    np.add.at(offsets, g + 1, 1)
+    return np.cumsum(offsets), order, g
+
+
+def np_distance2(coords, i, js):
+    """Right fold of float32 squares, as meta::accumulate sums them."""
+    D = coords.shape[0]
+    s = np.zeros(len(js), np.float32)
+    for d in range(D - 1, -1, -1):
+        df = (coords[d, i] - coords[d, js]).astype(np.float32)
+        s = (df * df + s).astype(np.float32)
+    return s
+
+
+def np_nearest_higher(coords, rho, dm, ppb=128):
+    """O(N^2) nearest-higher: candidates inside the dm search box, key (d, visit slot)."""
+    D, n = coords.shape
+    npd, T, mn, mx, ts = np_grid(coords, ppb)
+    off, order, g = np_tiles(coords, ppb)
+    slot = np.empty(n, np.int64)
+    slot[order] = np.arange(n)
+    bins = np.stack([np_bins(coords[d], mn[d], ts[d], npd) for d in range(D)])
+    nh = np.full(n, -1, np.int32)
+    dist_out = np.full(n, FLT_MAX, np.float32)
+    ids = np.arange(n)
+    for i in range(n):
+        inbox = np.ones(n, bool)
+        for d in range(D):
+            lo = np_bins(np.float32(coords[d, i] - np.float32(dm)), mn[d], ts[d], npd)
+            hi = np_bins(np.float32(coords[d, i] + np.float32(dm)), mn[d], ts[d], npd)
+            inbox &= (bins[d] >= lo) & (bins[d] <= hi)
+        higher = (rho > rho[i]) | ((rho == rho[i]) & (rho > 0) & (ids > i))
+        dist = np.sqrt(np_distance2(coords, i, ids)).astype(np.float32)
+        cand = np.nonzero(inbox & higher & (dist <= np.float32(dm)))[0]
+        if len(cand):
+            dmin = dist[cand].min()
+            best = cand[dist[cand] == dmin]
+            j = best[np.argmin(slot[best])]
+            nh[i] = j
+            dist_out[i] = dist[j]
+    return nh, dist_out
+
+
+def np_density_fp64(coords, w, dc, kernel):
+    D, n = coords.shape
+    c = coords.astype(np.float64)
+    rho = np.zeros(n)
+    for i in range(n):
+        d = np.sqrt(((c - c[:, i:i + 1]) ** 2).sum(axis=0))
+        inside = d <= dc
+        if kernel[0] == "flat":
+            k = np.full(n, kernel[1])
+        elif kernel[0] == "gaussian":
+            k = kernel[3] * np.exp(-((d - kernel[1]) ** 2) / (2 * kernel[2] ** 2))
+        else:
+            k = kernel[2] * np.exp(-kernel[1] * d)
+        k[i] = 1.0
+        rho[i] = (k * w * inside).sum()
+    return rho
+
+
+def np_labels(nh_final, is_seed, delta, cap_floor=0, unbounded=False):
+    """Closed form of KernelAssignClusters: label = rank of the root among the
+    stored seeds (ascending id, capacity = #points without nearest-higher)."""
+    n = len(nh_final)
+    cap = max(int((delta == FLT_MAX).sum()), cap_floor)
+    seeds = np.nonzero(is_seed)[0]
+    stored = seeds if unbounded else seeds[:cap]
+    rank = np.full(n, -1)
+    rank[stored] = np.arange(len(stored))
+    root = np.arange(n)
+    parent = np.where(nh_final >= 0, nh_final, np.arange(n))
+    for _ in range(64):
+        root = parent[root]
+    assert np.all(parent[root] == root)
+    return rank[root].astype(np.int32)
+
+
+# ---------------------------------------------------------------------------
+# exact float32 arithmetic by rationals
+# ---------------------------------------------------------------------------
+def rnd32(x: Fraction) -> Fraction:
+    """Round a rational to the nearest float32 (ties to even); finite range only."""
+    if x == 0:
+        return Fraction(0)
+    s = -1 if x < 0 else 1
+    a = abs(x)
+    e = a.numerator.bit_length() - a.denominator.bit_length()
+    if Fraction(2) ** e > a:
+        e -= 1
+    e = max(e, -126)
+    ulp = Fraction(2) ** (e - 23)
+    q = a / ulp
+    fl = q.numerator // q.denominator
+    rem = q - fl
+    if rem > Fraction(1, 2) or (rem == Fraction(1, 2) and fl % 2 == 1):
+        fl += 1
+    return s * fl * ulp
+
+
+def exact_right_fold_distance(a, b):
+    t = [rnd32(rnd32(Fraction(float(x)) - Fraction(float(y))) ** 2) for x, y in zip(a, b)]
+    s = Fraction(0)
+    for v in reversed(t):
+        s = rnd32(v + s)
+    # correctly rounded float32 sqrt = float32(double sqrt) (double rounding is innocuous for sqrt)
+    return np.float32(np.sqrt(np.float64(s.numerator) / np.float64(s.denominator)))
+
+
+# ---------------------------------------------------------------------------
+# pins (each raises AssertionError on failure)
+# ---------------------------------------------------------------------------
+def pin_grid_closed_form():
+    # SetupTiles.hpp:23-26 with pPBin = 128
+    for D, n, npd in ((2, 16384, 12), (2, 1048576, 91), (3, 4194304, 32), (10, 262144, 3), (1, 1000, 8)):
+        c = np.zeros((D, n), np.float32)
+        c[:, 1] = 1.0
+        g = oracle.grid(c)
+        assert g["npd"] == npd and g["ntiles"] == npd ** D, (D, n, g["npd"])
+
+
+def pin_tiles(coords):
+    p = oracle.make_params(1.0, 1.0)
+    r = oracle.make_clusters(coords, np.ones(coords.shape[1], np.float32), p)
+    off, order, _ = np_tiles(coords)
+    np.testing.assert_array_equal(r.tile_offsets, off)
+    np.testing.assert_array_equal(r.tile_indexes, order)
+
+
+def pin_tiles_sets():
+    pin_tiles(datasets.workload("2d_16k_flat").coords)
+    pin_tiles(datasets.blobs(5000, 3, 7, -5.0, 5.0, 0.7, 0.2, ("const", 1.0), 11).coords)
+
+
+def pin_known_answer(ref_data):
+    # tests/test_host_points.cpp:437-450: data_32768, dc=1.3, rhoc=10, dm=1.3 -> 20 clusters
+    c, w = ref_data("data_32768")
+    r = oracle.make_clusters(c, w, oracle.make_params(1.3, 10.0, 1.3))
+    assert int(r.cluster.max()) + 1 == 20
+
+
+def lattice(n_side=24, spacing=1.0, jitter_seed=None):
+    g = np.stack(np.meshgrid(np.arange(n_side), np.arange(n_side), indexing="ij")).reshape(2, -1)
+    c = (g * spacing).astype(np.float32)
+    keep = np.random.default_rng(5).random(c.shape[1]) < 0.8  # holes give varied densities
+    return np.ascontiguousarray(c[:, keep])
+
+
+def pin_density_fp64():
+    rng = np.random.default_rng(7)
+    pts = datasets.blobs(1500, 2, 5, 0.0, 30.0, 1.2, 0.1, ("uniform", 0.5, 1.5), 21)
+    lat = lattice()
+    wl = rng.uniform(0.5, 1.5, lat.shape[1]).astype(np.float32)
+    cases = [(pts.coords, pts.weights, 1.5, ("flat", 0.5)),
+             (pts.coords, pts.weights, 1.5, ("gaussian", 0.5, 1.3, 2.0)),
+             (pts.coords, pts.weights, 1.5, ("exponential", 0.7, 1.7)),
+             (lat, wl, 1.0, ("flat", 0.5)),  # neighbours at exactly dc
+             (lat, wl, 1.0, ("gaussian", 0.0, 1.0, 3.0))]
+    for c, w, dc, kern in cases:
+        r = oracle.make_clusters(c, w, oracle.make_params(dc, 1.0, dc, kernel=kern))
+        want = np_density_fp64(c, w.astype(np.float64), dc, kern)
+        rel = np.abs(r.rho.astype(np.float64) - want) / np.abs(want)
+        assert rel.max() < 1e-6, (kern, rel.max())
+
+
+def pin_flat_unit_weights_exact():
+    # Flat kernel, unit weights: rho = 1 + h * #(neighbours within dc), exactly
+    c = lattice()
+    n = c.shape[1]
+    r = oracle.make_clusters(c, np.ones(n, np.float32), oracle.make_params(1.0, 1.0, 1.0, kernel=("flat", 0.5)))
+    d2 = ((c[:, :, None].astype(np.int64) - c[:, None, :].astype(np.int64)) ** 2).sum(axis=0)
+    cnt = (d2 <= 1).sum(axis=1) - 1
+    np.testing.assert_array_equal(r.rho, (1.0 + 0.5 * cnt).astype(np.float32))
+
+
+def pin_nearest_higher_bruteforce():
+    rng = np.random.default_rng(9)
+    # integer coordinates and unit weights: many exact ties in rho and distance
+    c1 = np.round(rng.random((2, 1200)) * 25).astype(np.float32)
+    c2 = datasets.blobs(1200, 3, 4, 0.0, 12.0, 1.0, 0.2, ("const", 1.0), 31).coords
+    for c, dc, dm in ((c1, 1.0, 2.5), (c2, 0.8, 1.9)):
+        w = np.ones(c.shape[1], np.float32)
+        r = oracle.make_clusters(c, w, oracle.make_params(dc, 2.0, dm))
+        nh, dist = np_nearest_higher(c, r.rho, dm)
+        pre = r.delta  # nearest-higher distance before seed promotion
+        np.testing.assert_array_equal(dist, pre)
+        seeds = r.is_seed.astype(bool)
+        np.testing.assert_array_equal(np.where(seeds, -1, nh), r.nh)
+
+
+def pin_labels_closed_form():
+    for (c, w, dc, rhoc, dm, sdc) in (
+        (datasets.workload("2d_16k_flat").coords, np.ones(16384, np.float32), 1.5, 5.0, 3.0, 1.5),
+        (datasets.blobs(20000, 2, 60, 0.0, 200.0, 1.0, 0.0, ("const", 1.0), 99).coords,
+         np.ones(20000, np.float32), 0.6, 1.0, 5.0, 0.6),
+    ):
+        r = oracle.make_clusters(c, w, oracle.make_params(dc, rhoc, dm, sdc))
+        # seed predicate (ClusteringKernels.hpp:232-233) recomputed from rho and delta
+        seed = (r.delta > np.float32(sdc)) & (r.rho >= np.float32(rhoc))
+        np.testing.assert_array_equal(seed.astype(np.int32), r.is_seed)
+        np.testing.assert_array_equal(np_labels(r.nh, r.is_seed, r.delta), r.cluster)
+
+
+def pin_rhoc_boundary():
+    # points whose density equals rhoc exactly must be seeds (rho >= rhoc)
+    c = lattice()
+    n = c.shape[1]
+    r = oracle.make_clusters(c, np.ones(n, np.float32),
+                             oracle.make_params(1.0, 3.0, 1.0, seed_dc=0.5, kernel=("flat", 0.5)))
+    at = r.rho == np.float32(3.0)
+    assert at.any()
+    assert np.all(r.is_seed[at] == 1)  # every point has delta = inf or 1 > 0.5
+
+
+def pin_distance_fold_order():
+    rng = np.random.default_rng(13)
+    p = oracle.make_params(1.0, 1.0)
+    found = 0
+    for _ in range(4000):
+        a = (rng.random(3) * 10).astype(np.float32)
+        b = (rng.random(3) * 10).astype(np.float32)
+        t = [np.float32(np.float32(x - y) ** 2) for x, y in zip(a, b)]
+        left = np.float32(np.sqrt(np.float32(np.float32(t[0] + t[1]) + t[2])))
+        right = np.float32(np.sqrt(np.float32(t[0] + np.float32(t[1] + t[2]))))
+        if left != right:
+            found += 1
+            assert oracle.distance(a, b, p) == exact_right_fold_distance(a, b)
+    assert found > 0
+
+
+def pin_metric_closed_forms():
+    # tests/test_distance_metrics.cpp
+    a, b = np.array([1, 2], np.float32), np.array([4, 6], np.float32)
+    assert oracle.distance(a, b, oracle.make_params(1, 1)) == 5.0
+    assert oracle.distance(a, b, oracle.make_params(1, 1, metric="weighted_euclidean",
+                                                    metric_params=[1, 2])) == np.sqrt(np.float32(41))
+    a, b = np.array([-1, 3], np.float32), np.array([0, 6], np.float32)
+    assert oracle.distance(a, b, oracle.make_params(1, 1, metric="manhattan")) == 4.0
+    a, b = np.array([-1, 4], np.float32), np.array([0, 6], np.float32)
+    assert oracle.distance(a, b, oracle.make_params(1, 1, metric="chebyshev")) == 2.0
+    assert oracle.distance(a, b, oracle.make_params(1, 1, metric="weighted_chebyshev",
+                                                    metric_params=[1, 2])) == 4.0
+    # periodic: |0.5 - 9.5| = 9 with period 10 -> 1
+    assert oracle.distance(np.array([0.5], np.float32), np.array([9.5], np.float32),
+                           oracle.make_params(1, 1, metric="periodic_euclidean", metric_params=[10])) == 1.0
+
+
+PINS = [pin_grid_closed_form, pin_tiles_sets, pin_density_fp64, pin_flat_unit_weights_exact,
+        pin_nearest_higher_bruteforce, pin_labels_closed_form, pin_rhoc_boundary, pin_distance_fold_order,
+        pin_metric_closed_forms]
+
+
+# ---------------------------------------------------------------------------
+# tests
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("pin", PINS, ids=lambda f: f.__name__)
+def test_pin(pin):
+    pin()
+
+
+def test_known_answer_twenty_clusters(ref_data):
+    pin_known_answer(ref_data)
+
+
+@pytest.mark.parametrize("mutation", list(range(1, 11)))
+def test_planted_mistakes_are_caught(mutation):
+    failures = []
+    with oracle.mutation(mutation):
+        for pin in PINS:
+            try:
+                pin()
+            except AssertionError:
+                failures.append(pin.__name__)
+    assert failures, f"planted mistake {mutation} passed every pin"
+
+
+def test_engine_exp_is_correctly_rounded():
+    x = np.linspace(-104.0, 88.7, 200001, dtype=np.float32)
+    x = np.concatenate([x, (np.random.default_rng(2).standard_normal(20000) * 3).astype(np.float32)])
+    np.testing.assert_array_equal(oracle.expf_engine(x), oracle.expf_longdouble(x))
+    # and it differs from the host libm only where libm is not correctly rounded
+    lm = oracle.expf_libm(x)
+    diff = oracle.expf_engine(x) != lm
+    assert diff.mean() < 1e-3
+
+
+def test_serial_and_openmp_agree():
+    ps = datasets.workload("2d_1m_gauss", n=40000, seed=3)
+    p = oracle.make_params(2.0, 8.0, 4.0, kernel=("gaussian", 0.0, 1.0, 1.0))
+    a = oracle.make_clusters(ps.coords, ps.weights, p)
+    b = oracle.make_clusters_omp(ps.coords, ps.weights, p, 4)
+    for x, y in ((a.rho, b.rho), (a.nh, b.nh), (a.is_seed, b.is_seed), (a.cluster, b.cluster)):
+        np.testing.assert_array_equal(x, y)
+
+
+def test_square_equals_box_on_two_dimensions(ref_data):
+    # tests/test_partial_dimensional_clustering.py:120-137
+    sq, wsq = ref_data("square")
+    bx, wbx = ref_data("box")
+    p = oracle.make_params(1.0, 2.0, 1.6)
+    a = oracle.make_clusters(sq, wsq, p)
+    b = oracle.make_clusters(np.ascontiguousarray(bx[:2]), wbx, p)
+    np.testing.assert_array_equal(a.cluster, b.cluster)
+    np.testing.assert_array_equal(a.is_seed, b.is_seed)
+
+
+def reference_silhouette(coords, labels):
+    """utils/detail/scores.hpp:20-99: mean over clustered points in clusters of
+    size >= 2 of (b - a) / max(a, b)."""
+    c = coords.T.astype(np.float64)
+    ok = labels >= 0
+    ids = np.unique(labels[ok])
+    members = {k: np.nonzero(labels == k)[0] for k in ids}
+    scores = []
+    for i in np.nonzero(ok)[0]:
+        own = members[labels[i]]
+        if len(own) < 2:
+            continue
+        d = np.sqrt(((c - c[i]) ** 2).sum(axis=1))
+        a = d[own].sum() / (len(own) - 1)
+        b = min(d[m].mean() for k, m in members.items() if k != labels[i]) if len(members) > 1 else np.inf
+        scores.append((b - a) / max(a, b))
+    return float(np.mean(scores))
+
+
+@pytest.mark.parametrize("name,dc,rhoc,dm,threshold", [
+    ("data_1024", 1.5, 10.0, 1.5, 0.9), ("data_2048", 1.5, 10.0, 1.5, 0.9), ("data_4096", 1.5, 10.0, 1.5, 0.9),
+    ("sissa_1000", 20.0, 10.0, 20.0, 0.5), ("sissa_4000", 20.0, 10.0, 20.0, 0.45),
+    ("toyDetector_1000", 4.0, 2.5, 4.0, 0.8),
+])
+def test_reference_quality_thresholds(ref_data, name, dc, rhoc, dm, threshold):
+    # tests/test_clustering.cpp:11-114
+    c, w = ref_data(name)
+    r = oracle.make_clusters(c, w, oracle.make_params(dc, rhoc, dm))
+    assert reference_silhouette(c, r.cluster) >= threshold
+
+
+def test_blob_3d_quality(ref_data):
+    # tests/test_clustering.cpp:116-131 (silhouette >= 0.8), on a 3000-point subsample of the scores
+    c, w = ref_data("blob")
+    r = oracle.make_clusters(c, w, oracle.make_params(1.0, 5.0, 2.0))
+    from sklearn.metrics import silhouette_score
+
+    ok = r.cluster >= 0
+    assert silhouette_score(c.T[ok], r.cluster[ok], sample_size=3000, random_state=0) >= 0.8
+
+
+@pytest.mark.parametrize("name,truth,dc,rhoc,dm", [
+    ("data_1024", "data_1024_truth", 1.5, 10.0, 1.5), ("data_32768", "data_32768_truth", 1.5, 10.0, 1.5),
+    ("toyDetector_1000", "toy_det_1000_truth", 4.0, 2.5, 4.0), ("sissa_1000", "sissa_1000_truth", 20.0, 10.0, 20.0),
+])
+def test_truth_files_cluster_counts(ref_data, ref_truth, name, truth, dc, rhoc, dm):
+    """data/truth_files hold labels of an unrecorded code version and parameters;
+    with the parameters the reference's tests use on these datasets the number
+    of clusters matches (tests/check_result.py:5-25, first criterion).  Their
+    per-cluster sizes differ by a few boundary points and are not pinned."""
+    c, w = ref_data(name)
+    r = oracle.make_clusters(c, w, oracle.make_params(dc, rhoc, dm))
+    t = ref_truth[truth + "__cluster_ids"]
+    assert len(np.unique(r.cluster)) == len(np.unique(t))
+    assert abs(int((r.cluster >= 0).sum()) - int((t >= 0).sum())) <= 0.05 * len(t)
+
+
+def test_config1_golden_regression(config1_golden):
+    g = config1_golden
+    r = oracle.make_clusters(g["coords"], g["weights"], oracle.make_params(1.5, 5.0, 3.0))
+    np.testing.assert_array_equal(r.rho, g["rho"])
+    np.testing.assert_array_equal(r.cluster, g["cluster"])
+    np.testing.assert_array_equal(r.nh, g["nh"])
