@@ -43,6 +43,18 @@ def build_engine():
     return out
 
 
+def build_cpp_tests():
+    """The drop-in C++ API test program (g++ only needs clue_abi.h and links the engine)."""
+    src = os.path.join(ROOT, "tests", "cpp", "test_api.cpp")
+    outdir = os.path.join(ROOT, "tests", "cpp", "bin")
+    os.makedirs(outdir, exist_ok=True)
+    out = os.path.join(outdir, "test_api")
+    run(["g++", "-std=c++20", "-O2", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"), src,
+         "-L", HERE, "-lclue_hip", "-Wl,-rpath,$ORIGIN/../../../cluestering_alpaka3_amd", "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_oracle():
     run(["make", "-C", os.path.join(ROOT, "oracle"), "-s"])
 
@@ -50,9 +62,10 @@ def build_oracle():
 def build_all():
     build_oracle()
     build_engine()
+    build_cpp_tests()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "engine": build_engine, "oracle": build_oracle}[w]()
+        {"all": build_all, "engine": build_engine, "oracle": build_oracle, "cpp": build_cpp_tests}[w]()

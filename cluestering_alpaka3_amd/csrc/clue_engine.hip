@@ -448,6 +448,9 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
   cptr<float> W = as_const(a.ws);
   cptr<float> HW = as_const(a.hw);
   float rho = 0.f;
+  // The Euclidean flat-kernel loop reads the precomputed h*w_j; every other
+  // combination reads w_j and evaluates the kernel.
+  constexpr bool kUsesHW = (METRIC == MEUCLID) && (KERNEL == KFLAT);
   // One candidate: rho_i += ((float)(d <= dc) * k) * w_j in the visit order
   // (ClusteringKernels.hpp:38-44); out-of-radius terms are +0 and skipped.
   auto eval = [&](auto self_c, int j, const float* c, float wj, float hwj) {
@@ -482,7 +485,7 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
         for (int u = 0; u < 4; ++u) {
 #pragma unroll
           for (int d = 0; d < D; ++d) c[u][d] = X[d][j + u];
-          if constexpr (KERNEL == KFLAT) hv[u] = HW[j + u];
+          if constexpr (kUsesHW) hv[u] = HW[j + u];
           else wv[u] = W[j + u];
         }
 #pragma unroll
@@ -494,7 +497,7 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
 #pragma unroll
       for (int d = 0; d < D; ++d) c[d] = X[d][j];
       float wj = 0.f, hwj = 0.f;
-      if constexpr (KERNEL == KFLAT) hwj = HW[j];
+      if constexpr (kUsesHW) hwj = HW[j];
       else wj = W[j];
       eval(self_c, j, c, wj, hwj);
     }
