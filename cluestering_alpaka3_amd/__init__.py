@@ -67,6 +67,14 @@ class ClueStats(ctypes.Structure):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
 
 
+class ClueTuning(ctypes.Structure):
+    _fields_ = [("nh_first_radius", ctypes.c_float), ("density_bitmask", ctypes.c_int32),
+                ("bitmask_min_points", ctypes.c_int32), ("morton_bits", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 class ClueGridInfo(ctypes.Structure):
     _fields_ = [("ndim", ctypes.c_int32), ("npoints", ctypes.c_int32), ("npd", ctypes.c_int32),
                 ("ntiles", ctypes.c_int32), ("minmax", ctypes.c_float * (2 * MAX_DIM)),
@@ -107,6 +115,8 @@ _SIGNATURES = {
     "clue_plan_last_grid": ([_vp, _P(ClueGridInfo)], ctypes.c_int),
     "clue_plan_copy_tiles": ([_vp, _P(ctypes.c_int32), _P(ctypes.c_int32)], ctypes.c_int),
     "clue_plan_copy_delta": ([_vp, _P(ctypes.c_float)], ctypes.c_int),
+    "clue_plan_set_tuning": ([_vp, _P(ClueTuning)], ctypes.c_int),
+    "clue_plan_get_tuning": ([_vp, _P(ClueTuning)], ctypes.c_int),
     "clue_plan_set_timing": ([_vp, ctypes.c_int32], ctypes.c_int),
     "clue_plan_stage_times": ([_vp, _P(ctypes.c_float), ctypes.c_int32, _P(ctypes.c_int32)], ctypes.c_int),
     "clue_stage_name": ([ctypes.c_int32], ctypes.c_char_p),
@@ -311,6 +321,18 @@ class Plan:
         check(lib().clue_plan_copy_delta(self.handle, out.ctypes.data_as(_P(ctypes.c_float))))
         return out
 
+    def get_tuning(self):
+        t = ClueTuning()
+        check(lib().clue_plan_get_tuning(self.handle, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = ClueTuning()
+        check(lib().clue_plan_get_tuning(self.handle, ctypes.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, v)
+        check(lib().clue_plan_set_tuning(self.handle, ctypes.byref(t)))
+
     def set_timing(self, enable=True):
         check(lib().clue_plan_set_timing(self.handle, 1 if enable else 0))
 
@@ -354,6 +376,8 @@ def cluster_points(coords, weights, dc, rhoc, dm=None, seed_dc=None, ppb=128, ke
     if c.ndim == 1:
         c = c[None, :]
     ndim, n = c.shape
+    if not 1 <= ndim <= MAX_DIM:
+        raise InvalidArgument(1, "number of dimensions must be in [1, 10]")
     w = np.ascontiguousarray(weights, dtype=np.float32)
     own_plan = plan is None
     plan = plan or Plan(device)

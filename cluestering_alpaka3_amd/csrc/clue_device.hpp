@@ -70,11 +70,14 @@ __device__ __forceinline__ int32_t tile_bin(float coord, float mn, float mx, flo
 
 // ---------------------------------------------------------------------------
 // Engine exponential: the correctly rounded float exp, evaluated with IEEE
-// double operations only.  exp(x) = 2^(k/64) * exp(r) with k = rint(x*64/ln2),
+// double operations only (__builtin_fma is one correctly rounded fused
+// multiply-add).  exp(x) = 2^(k/64) * exp(r) with k = rint(x*64/ln2),
 // r = x - k*ln2/64 (two-constant Cody-Waite split; hi has 33 significant bits so
-// k*hi is exact), exp(r)-1 by a degree-5 Taylor polynomial (|r| <= ln2/128,
-// truncation below 4e-17), 2^(j/64) from a table of correctly rounded doubles;
-// one final rounding double -> float.  The oracle evaluates the same sequence.
+// k*hi is exact), exp(r)-1 by a degree-5 Taylor polynomial in Horner form
+// (|r| <= ln2/128, truncation below 4e-17), 2^(j/64) from a table of correctly
+// rounded doubles; one final rounding double -> float.  The oracle evaluates
+// the same sequence.  Written without branches: out-of-range and NaN inputs
+// are resolved by selects after a clamped evaluation.
 // ---------------------------------------------------------------------------
 __device__ __constant__ double kExp2Table[64] = {
     0x1.0000000000000p+0, 0x1.02c9a3e778061p+0, 0x1.059b0d3158574p+0, 0x1.0874518759bc8p+0,
@@ -95,34 +98,45 @@ __device__ __constant__ double kExp2Table[64] = {
     0x1.ea4afa2a490dap+0, 0x1.efa1bee615a27p+0, 0x1.f50765b6e4540p+0, 0x1.fa7c1819e90d8p+0};
 
 __device__ __forceinline__ float engine_expf(float x) {
-  if (x != x) return x;
-  if (x > 89.0f) return __builtin_inff();
-  if (x < -110.0f) return 0.0f;
-  const double xd = static_cast<double>(x);
+  const float xc = x > 89.0f ? 89.0f : (x < -110.0f ? -110.0f : x);
+  const double xd = static_cast<double>(xc == xc ? xc : 0.0f);
   const double kd = __builtin_rint(xd * 0x1.71547652b82fep+6);
   const int k = static_cast<int>(kd);
-  double r = xd - kd * 0x1.62e42fefp-7;
-  r = r - kd * 0x1.473de6af278edp-40;
-  const double p =
-      r + (r * r) * (0.5 + r * (1.0 / 6.0 + r * (1.0 / 24.0 + r * (1.0 / 120.0))));
+  double r = __builtin_fma(-kd, 0x1.62e42fefp-7, xd);
+  r = __builtin_fma(-kd, 0x1.473de6af278edp-40, r);
+  double p = __builtin_fma(r, 1.0 / 120.0, 1.0 / 24.0);
+  p = __builtin_fma(r, p, 1.0 / 6.0);
+  p = __builtin_fma(r, p, 0.5);
+  p = __builtin_fma(r * r, p, r);
   const int j = k & 63;
   const int m = (k - j) / 64;
   const double t = kExp2Table[j];
-  const double y = t + t * p;
-  return static_cast<float>(__builtin_ldexp(y, m));
+  float y = static_cast<float>(__builtin_ldexp(__builtin_fma(t, p, t), m));
+  y = x > 89.0f ? __builtin_inff() : y;
+  y = x < -110.0f ? 0.0f : y;
+  return x != x ? x : y;
+}
+
+// Gaussian kernel argument -(a*a) / den as the correctly rounded float quotient,
+// computed as (float)((double)num * (1/den)) with 1/den rounded to double on the
+// host: a quotient of two floats lies at least ~2^-48 (relative) from any float
+// rounding boundary, and the double product is within 2^-52 of it, so the
+// final rounding lands on the same float as the IEEE division.
+__device__ __forceinline__ float quotient_via_rcp(float num, double rcp_den) {
+  return static_cast<float>(static_cast<double>(num) * rcp_den);
 }
 
 // Convolutional kernels (core/detail/ConvolutionalKernel.hpp:16-65), j != i.
 enum { KFLAT = 0, KGAUSS = 1, KEXP = 2 };
+// rcp_den: 1 / ((2*sigma)*sigma) rounded to double (Gaussian only).
 template <int KERNEL>
-__device__ __forceinline__ float kernel_value(float dist, float p0, float p1, float p2) {
+__device__ __forceinline__ float kernel_value(float dist, float p0, float p1, float p2, double rcp_den) {
   if constexpr (KERNEL == KFLAT) {
     return p0;
   } else if constexpr (KERNEL == KGAUSS) {
     const float a = dist - p0;
-    const float num = (-a) * a;        // -(d-mu)*(d-mu)
-    const float den = (2.f * p1) * p1; // 2*sigma*sigma
-    return p2 * engine_expf(num / den);
+    const float num = (-a) * a;  // -(d-mu)*(d-mu), divided by (2*sigma)*sigma
+    return p2 * engine_expf(quotient_via_rcp(num, rcp_den));
   } else {
     return p1 * engine_expf((-p0) * dist);
   }

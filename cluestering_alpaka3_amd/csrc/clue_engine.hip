@@ -204,17 +204,29 @@ __global__ void k_bbox_final(const float* part, int nparts, int D, int32_t npd, 
 // ===========================================================================
 template <int D>
 __global__ __launch_bounds__(kThreads) void k_bin(Coords c, int32_t n, const DevGrid* grid,
-                                                   GridShape gs, int32_t* key, int32_t* count) {
+                                                   GridShape gs, int32_t* key, int32_t* count,
+                                                   int32_t* key_nh, int sub_bits) {
   const int i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
   cptr<DevGrid> g = as_const(grid);
   int32_t gb = 0;
+  uint32_t code = 0;
 #pragma unroll
-  for (int d = 0; d < D; ++d)
-    gb += gs.stride[d] *
-          tile_bin(c.p[d][i], g->mn[d], g->mx[d], g->ts[d], gs.npd, gs.wrapped[d] != 0);
+  for (int d = 0; d < D; ++d) {
+    const float x = c.p[d][i];
+    const int32_t b = tile_bin(x, g->mn[d], g->mx[d], g->ts[d], gs.npd, gs.wrapped[d] != 0);
+    gb += gs.stride[d] * b;
+    if (key_nh) {
+      // position inside the tile on a 2^sub_bits grid, interleaved into a Morton code
+      const float f = (x - g->mn[d]) / g->ts[d] - static_cast<float>(b);
+      int32_t sub = f2i(f * static_cast<float>(1 << sub_bits));
+      sub = sub < 0 ? 0 : (sub > (1 << sub_bits) - 1 ? (1 << sub_bits) - 1 : sub);
+      for (int bit = 0; bit < sub_bits; ++bit) code |= ((sub >> bit) & 1u) << (bit * D + d);
+    }
+  }
   key[i] = gb;
   atomicAdd(&count[gb], 1);
+  if (key_nh) key_nh[i] = static_cast<int32_t>((static_cast<uint32_t>(gb) << (sub_bits * D)) | code);
 }
 
 // ===========================================================================
@@ -315,6 +327,82 @@ __global__ __launch_bounds__(kThreads) void k_gather(Coords c, const float* w, c
   hw[s] = h * wv;  // FlatKernel term k*w with k = h, precomputed once per point
 }
 
+// Morton-ordered SoA copy (tile-major, Morton code inside the tile): the query
+// order of both neighbour passes and the candidate layout of the nearest-higher
+// search.  Also the weight and the point's slot in the tile-ordered layout.
+__global__ __launch_bounds__(kThreads) void k_gather_morton(Coords c, const float* w,
+                                                             const int32_t* pn,
+                                                             const int32_t* inv_ref, int32_t n,
+                                                             int D, int64_t ld, float* xn,
+                                                             float* wn, int32_t* sn) {
+  const int p = blockIdx.x * kThreads + threadIdx.x;
+  if (p >= n) return;
+  const int32_t id = pn[p];
+  for (int d = 0; d < D; ++d) xn[d * ld + p] = c.p[d][id];
+  wn[p] = w[id];
+  sn[p] = inv_ref[id];
+}
+
+__global__ void k_inverse(const int32_t* perm, int32_t n, int32_t* inv) {
+  const int s = blockIdx.x * kThreads + threadIdx.x;
+  if (s < n) inv[perm[s]] = s;
+}
+
+constexpr int kBlock = 32;       // points per block of the Morton layout
+constexpr int kMaskWords = 256;  // per-wave LDS bitmask: tiles up to 8192 points
+
+__global__ void k_block_count(const int32_t* offsets, int32_t T, int32_t* cnt) {
+  const int t = blockIdx.x * kThreads + threadIdx.x;
+  if (t < T) cnt[t] = (offsets[t + 1] - offsets[t] + kBlock - 1) / kBlock;
+}
+
+// Bounding box of each block of 32 consecutive Morton-ordered points of a tile.
+__global__ __launch_bounds__(kThreads) void k_block_boxes(const float* xn, int64_t ld, int D,
+                                                           const int32_t* offsets,
+                                                           const int32_t* blk_off, int32_t T,
+                                                           float* bmin, float* bmax,
+                                                           int64_t ldb) {
+  const int b = blockIdx.x * kThreads + threadIdx.x;
+  const int nblk = blk_off[T];
+  if (b >= nblk) return;
+  int lo = 0, hi = T;  // last t with blk_off[t] <= b
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) / 2;
+    if (blk_off[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  const int t = lo;
+  const int start = offsets[t] + (b - blk_off[t]) * kBlock;
+  const int end = min(start + kBlock, offsets[t + 1]);
+  for (int d = 0; d < D; ++d) {
+    float mn = __builtin_inff(), mx = -__builtin_inff();
+    for (int j = start; j < end; ++j) {
+      const float v = xn[d * ld + j];
+      mn = v < mn ? v : mn;
+      mx = mx < v ? v : mx;
+    }
+    bmin[d * ldb + b] = mn;
+    bmax[d * ldb + b] = mx;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_tile_boxes(const float* bmin, const float* bmax,
+                                                          int64_t ldb, int D,
+                                                          const int32_t* blk_off, int32_t T,
+                                                          float* tmin, float* tmax) {
+  const int t = blockIdx.x * kThreads + threadIdx.x;
+  if (t >= T) return;
+  for (int d = 0; d < D; ++d) {
+    float mn = __builtin_inff(), mx = -__builtin_inff();
+    for (int b = blk_off[t]; b < blk_off[t + 1]; ++b) {
+      mn = bmin[d * ldb + b] < mn ? bmin[d * ldb + b] : mn;
+      mx = mx < bmax[d * ldb + b] ? bmax[d * ldb + b] : mx;
+    }
+    tmin[d * T + t] = mn;
+    tmax[d * T + t] = mx;
+  }
+}
+
 // ===========================================================================
 // Density (K4) and nearest-higher (K5).
 // ===========================================================================
@@ -331,7 +419,27 @@ struct PassArgs {
   float radius;              // dc (density) or dm (nearest-higher)
   float thr;                 // max{s : sqrtf(s) <= radius}
   float kp[3];
+  double kden_rcp;           // Gaussian: 1 / ((2*sigma)*sigma) rounded to double
   float mp[kMaxDim];
+  // Morton-ordered layout (queries of both passes; candidates of the block search)
+  const float* xn[kMaxDim];
+  const float* wn;
+  const int32_t* sn;         // Morton position -> tile-order slot
+  const int32_t* pn;         // Morton position -> point id
+  const int32_t* kn;         // sorted Morton keys (tile id << mshift | code)
+  int32_t mshift;
+  const int32_t* blk_off;    // first block of each tile
+  const float* bmin[kMaxDim];
+  const float* bmax[kMaxDim];
+  const float* tmin[kMaxDim];
+  const float* tmax[kMaxDim];
+  int32_t* pending;          // nearest-higher: Morton positions left for the full search
+  int32_t* pending_count;
+  int32_t* pending_flag;     // per Morton position: 1 if unresolved after the first phase
+  float thr1;                // first-phase search radius threshold
+  int32_t mask_min_points;   // density: tiles at least this large use the bitmask walk (0: never)
+  float* rn_out;             // density: Morton order
+  const float* rn;           // nearest-higher input: Morton order
   float* rho_s_out;          // density: tile order
   float* rho_out;            // density: original order
   int32_t* nh_out;           // nearest-higher: original order
@@ -378,8 +486,9 @@ __device__ __forceinline__ int tile_index(const int* b, const GridShape& gs) {
 // Visitor over the tiles a lane needs, in the reference order (lexicographic,
 // dimension 0 outermost; within a tile, CSR order).  UNION: the wave walks the
 // union of its lanes' boxes with wave-uniform tile and candidate indices (the
-// candidates arrive through scalar loads), lanes outside a tile idle under the
-// exec mask.  Otherwise each lane walks its own box.
+// candidates arrive through scalar loads); body(tile, begin, end, need) runs
+// with the whole wave and `need` tells a lane whether the tile is in its box.
+// Otherwise each lane walks its own box.
 template <int D, bool UNION, typename F>
 __device__ __forceinline__ void visit_box(bool valid, const int* lo, const int* hi,
                                           const GridShape& gs, cptr<int32_t> offsets, F&& body) {
@@ -400,9 +509,7 @@ __device__ __forceinline__ void visit_box(bool valid, const int* lo, const int* 
       for (int d = 0; d < D; ++d) need = need && b[d] >= lo[d] && b[d] <= hi[d];
       if (__ballot(need) != 0) {
         const int g = uniform(tile_index<D>(b, gs));
-        const int beg = offsets[g];
-        const int end = offsets[g + 1];
-        if (need) body(beg, end);
+        body(g, offsets[g], offsets[g + 1], need);
       }
       int d = D - 1;
       for (; d >= 0; --d) {
@@ -418,7 +525,7 @@ __device__ __forceinline__ void visit_box(bool valid, const int* lo, const int* 
     for (int d = 0; d < D; ++d) b[d] = lo[d];
     while (true) {
       const int g = tile_index<D>(b, gs);
-      body(offsets[g], offsets[g + 1]);
+      body(g, offsets[g], offsets[g + 1], true);
       int d = D - 1;
       for (; d >= 0; --d) {
         if (++b[d] <= hi[d]) break;
@@ -429,15 +536,47 @@ __device__ __forceinline__ void visit_box(bool valid, const int* lo, const int* 
   }
 }
 
-template <int D, int METRIC, int KERNEL, bool UNION>
-__global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
-  const int s = blockIdx.x * kThreads + threadIdx.x;
-  const bool valid = s < a.n;
-  const int sq = valid ? s : a.n - 1;
-  float q[D];
+__device__ __forceinline__ float wave_max_f32(float v) {
 #pragma unroll
-  for (int d = 0; d < D; ++d) q[d] = a.xs[d][sq];
-  const float w_self = a.ws[sq];  // the point's own term is 1 * w_i (kernel value 1 for j == i)
+  for (int off = 32; off > 0; off >>= 1) {
+    const float o = __shfl_xor(v, off);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+__device__ __forceinline__ float wave_min_f32(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const float o = __shfl_xor(v, off);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+__device__ __forceinline__ float readlane_f32(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// QNH: the lanes take their queries in Morton order (spatially coherent waves,
+// so the lanes of a wave share most of their search box); otherwise in tile order.
+template <int D, int METRIC, int KERNEL, bool UNION, bool QNH>
+__global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
+  const int pos = blockIdx.x * kThreads + threadIdx.x;
+  const bool valid = pos < a.n;
+  const int pq = valid ? pos : a.n - 1;
+  float q[D];
+  float w_self;  // the point's own term is 1 * w_i (kernel value 1 for j == i)
+  int s;         // the query's slot in the tile-ordered candidate layout
+  if constexpr (QNH) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) q[d] = a.xn[d][pq];
+    w_self = a.wn[pq];
+    s = valid ? a.sn[pq] : -1;
+  } else {
+#pragma unroll
+    for (int d = 0; d < D; ++d) q[d] = a.xs[d][pq];
+    w_self = a.ws[pq];
+    s = valid ? pq : -1;
+  }
   float mn[D], mx[D], ts[D];
   load_grid<D>(a.grid, mn, mx, ts);
   int lo[D], hi[D];
@@ -464,14 +603,14 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
       } else {
         if (s2 <= a.thr) {
           const float k =
-              (j == s) ? 1.f : kernel_value<KERNEL>(__builtin_sqrtf(s2), a.kp[0], a.kp[1], a.kp[2]);
+              (j == s) ? 1.f : kernel_value<KERNEL>(__builtin_sqrtf(s2), a.kp[0], a.kp[1], a.kp[2], a.kden_rcp);
           rho += k * wj;
         }
       }
     } else {
       const float dist = metric_distance<D, METRIC>(q, c, a.mp);
       if (dist <= a.radius) {
-        const float k = (j == s) ? 1.f : kernel_value<KERNEL>(dist, a.kp[0], a.kp[1], a.kp[2]);
+        const float k = (j == s) ? 1.f : kernel_value<KERNEL>(dist, a.kp[0], a.kp[1], a.kp[2], a.kden_rcp);
         rho += k * wj;
       }
     }
@@ -502,16 +641,96 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
       eval(self_c, j, c, wj, hwj);
     }
   };
-  visit_box<D, UNION>(valid, lo, hi, a.gs, as_const(a.offsets), [&](int beg, int end) {
+  // Bitmask walk (Morton queries, Euclidean, no wrapping): the blocks of the
+  // Morton layout that can hold a point within dc of some lane mark their
+  // points' tile-order slots in a per-wave LDS bitmask; walking the set bits in
+  // ascending order visits exactly the candidates that can contribute, in the
+  // reference order.  Points of unmarked blocks are beyond dc for every lane
+  // (box lower bounds are exact, see lower_bound2), so their terms are +0.
+  constexpr bool kMasked = QNH && UNION && METRIC == MEUCLID;
+  __shared__ uint32_t s_mask[kMasked ? kThreads / kWave : 1][kMasked ? kMaskWords : 1];
+  const int lane = __lane_id();
+  const int wv = threadIdx.x / kWave;
+  float qlo[D], qhi[D];
+  if constexpr (kMasked) {
+    for (int w = lane; w < kMaskWords; w += kWave) s_mask[wv][w] = 0u;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      qlo[d] = wave_min_f32(valid ? q[d] : __builtin_inff());
+      qhi[d] = wave_max_f32(valid ? q[d] : -__builtin_inff());
+    }
+  }
+  auto run_masked = [&](auto self_c, int t, int beg, int end, bool need) {
+    uint32_t* mask = s_mask[wv];
+    const int k0 = uniform(a.blk_off[t]), k1 = uniform(a.blk_off[t + 1]);
+    for (int kb = k0; kb < k1; kb += kWave) {
+      const int k = kb + lane;
+      const bool inb = k < k1;
+      float lbw = 0.f;
+#pragma unroll
+      for (int d = D - 1; d >= 0; --d) {
+        const float bl = inb ? a.bmin[d][k] : __builtin_inff();
+        const float bh = inb ? a.bmax[d][k] : -__builtin_inff();
+        float g = bl - qhi[d];
+        const float g2 = qlo[d] - bh;
+        g = g2 > g ? g2 : g;
+        g = g > 0.f ? g : 0.f;
+        const float tt = g * g;
+        lbw = (d == D - 1) ? tt : tt + lbw;
+      }
+      uint64_t m = __ballot(inb && lbw <= a.thr);
+      while (m) {
+        const int kl = __builtin_ctzll(m);
+        m &= m - 1;
+        const int start = beg + (kb + kl - k0) * kBlock;
+        const int cnt = min(kBlock, end - start);
+        if (lane < cnt) {
+          const int bit = a.sn[start + lane] - beg;
+          atomicOr(&mask[bit >> 5], 1u << (bit & 31));
+        }
+      }
+    }
+    const int nwords = (end - beg + 31) >> 5;
+    for (int w = 0; w < nwords; ++w) {
+      uint32_t word = static_cast<uint32_t>(uniform(static_cast<int>(mask[w])));
+      if (word == 0u) continue;
+      if (lane == 0) mask[w] = 0u;
+      while (word) {
+        const int j = beg + w * 32 + __builtin_ctz(word);
+        word &= word - 1u;
+        float c[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) c[d] = X[d][j];
+        float wj = 0.f, hwj = 0.f;
+        if constexpr (kUsesHW) hwj = HW[j];
+        else wj = W[j];
+        if (need) eval(self_c, j, c, wj, hwj);
+      }
+    }
+  };
+  visit_box<D, UNION>(valid, lo, hi, a.gs, as_const(a.offsets), [&](int t, int beg, int end, bool need) {
     // Only the tile holding a lane's own point needs the j == i test.
-    if (!UNION || __ballot(s >= beg && s < end) != 0)
-      run(std::true_type{}, beg, end);
-    else
-      run(std::false_type{}, beg, end);
+    const bool self = !UNION || __ballot(s >= beg && s < end) != 0;
+    if constexpr (kMasked) {
+      if (a.mask_min_points > 0 && end - beg >= a.mask_min_points && end - beg <= kMaskWords * 32) {
+        if (self) run_masked(std::true_type{}, t, beg, end, need);
+        else run_masked(std::false_type{}, t, beg, end, need);
+        return;
+      }
+    }
+    if (need) {
+      if (self) run(std::true_type{}, beg, end);
+      else run(std::false_type{}, beg, end);
+    }
   });
   if (valid) {
     a.rho_s_out[s] = rho;
-    a.rho_out[a.perm[s]] = rho;
+    if constexpr (QNH) {
+      a.rn_out[pos] = rho;
+      a.rho_out[a.pn[pos]] = rho;
+    } else {
+      a.rho_out[a.perm[s]] = rho;
+    }
   }
 }
 
@@ -560,7 +779,8 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher(const PassArgs a) {
       }
     }
   };
-  visit_box<D, UNION>(valid, lo, hi, a.gs, as_const(a.offsets), [&](int beg, int end) {
+  visit_box<D, UNION>(valid, lo, hi, a.gs, as_const(a.offsets), [&](int, int beg, int end, bool need) {
+    if (!need) return;
     int j = beg;
     if constexpr (UNION) {
       for (; j + 4 <= end; j += 4) {
@@ -590,6 +810,250 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher(const PassArgs a) {
   }
   const uint64_t none = __ballot(valid && nh == -1);
   if (__lane_id() == 0 && none) atomicAdd(a.seed_candidates, static_cast<int>(__popcll(none)));
+}
+
+// Squared distance from q to the nearest point of a box, with the fold order of
+// squared_distance.  Rounding is monotone, so for every point c of the box
+// lower_bound2 <= squared_distance(q, c) exactly: a box whose bound exceeds a
+// lane's threshold holds no candidate that lane can accept.
+template <int D>
+__device__ __forceinline__ float lower_bound2(const float* q, const float* lo, const float* hi) {
+  float s = 0.f;
+#pragma unroll
+  for (int d = D - 1; d >= 0; --d) {
+    float g = lo[d] - q[d];
+    const float g2 = q[d] - hi[d];
+    g = g2 > g ? g2 : g;
+    g = g > 0.f ? g : 0.f;
+    const float t = g * g;
+    s = (d == D - 1) ? t : t + s;
+  }
+  return s;
+}
+
+// Largest float s with sqrtf(s) <= delta.
+__device__ __forceinline__ float sqrt_le_bound_dev(float delta) {
+  if (!(delta >= 0.f)) return -1.f;
+  float t = delta * delta;
+  if (!(t <= FLT_MAX)) t = FLT_MAX;
+  while (t > 0.f && !(__builtin_sqrtf(t) <= delta)) t = __int_as_float(__float_as_int(t) - 1);
+  if (!(__builtin_sqrtf(t) <= delta)) return -1.f;
+  while (t < FLT_MAX) {
+    const float nt = __int_as_float(__float_as_int(t) + 1);
+    if (!(__builtin_sqrtf(nt) <= delta)) break;
+    t = nt;
+  }
+  return t;
+}
+
+// Nearest-higher (K5) as a block search over the Morton-ordered layout
+// (Euclidean metric, no wrapped dimension).  The result is the serial one: the
+// candidate minimising (distance, visit position) where the visit position is
+// (tile id, point id) -- ascending tile ids and ascending ids inside a tile are
+// the order of for_recursion_nearest_higher (ClusteringKernels.hpp:109-166) on a
+// grid without wrapping -- so candidates may be examined in any order, and a
+// block whose bounding box cannot beat a lane's current distance is skipped.
+//
+// PHASE 1 limits the search to the radius thr1 (the wave's own 64 points first,
+// then the blocks of its lanes' search boxes); a lane that finds a higher point
+// within it is final (every candidate at or below its distance was examined).
+// The others are queued for PHASE 2, the full search within dm.
+template <int D, int PHASE>
+__global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassArgs a) {
+  const int gid = blockIdx.x * kThreads + threadIdx.x;
+  int pos;
+  bool valid;
+  if constexpr (PHASE == 1) {
+    valid = gid < a.n;
+    pos = valid ? gid : a.n - 1;
+  } else {
+    const int npend = uniform(*a.pending_count);
+    if (uniform(gid & ~(kWave - 1)) >= npend) return;  // wave-uniform exit
+    valid = gid < npend;
+    pos = a.pending[valid ? gid : npend - 1];
+  }
+  float q[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) q[d] = a.xn[d][pos];
+  const float rho_i = a.rn[pos];
+  const int32_t id_i = a.pn[pos];
+  float mn[D], mx[D], ts[D];
+  load_grid<D>(a.grid, mn, mx, ts);
+  int lo[D], hi[D];
+  search_box<D>(q, a.radius, mn, mx, ts, a.gs, lo, hi);
+  cptr<float> X[D], TMIN[D], TMAX[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    X[d] = as_const(a.xn[d]);
+    TMIN[d] = as_const(a.tmin[d]);
+    TMAX[d] = as_const(a.tmax[d]);
+  }
+  cptr<float> R = as_const(a.rn);
+  cptr<int32_t> P = as_const(a.pn);
+  cptr<int32_t> OFF = as_const(a.offsets);
+  cptr<int32_t> BOFF = as_const(a.blk_off);
+  float delta = FLT_MAX;
+  int32_t nh = -1;
+  int64_t best = INT64_MAX;
+  // candidates need s2 <= bound = min(search radius, max{s : sqrtf(s) <= delta})
+  const float cap = (PHASE == 1 && a.thr1 < a.thr) ? a.thr1 : a.thr;
+  float bound = cap;
+  // the wave's query box, for the block prefilter
+  float qlo[D], qhi[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    qlo[d] = wave_min_f32(valid ? q[d] : __builtin_inff());
+    qhi[d] = wave_max_f32(valid ? q[d] : -__builtin_inff());
+  }
+
+  auto in_box = [&](int t) {
+    bool in = valid;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int b = (t / a.gs.stride[d]) % a.gs.npd;
+      in = in && b >= lo[d] && b <= hi[d];
+    }
+    return in;
+  };
+  auto scan = [&](int t, int beg, int end) {
+    for (int j = beg; j < end; ++j) {
+      float c[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) c[d] = X[d][j];
+      const float rho_j = R[j];
+      const int32_t id_j = P[j];
+      const int rb = __float_as_int(rho_j);
+      const bool pos_rho = (rb > 0) && (rb <= 0x7f800000);
+      const bool higher = (rho_j > rho_i) || ((rho_j == rho_i) && pos_rho && (id_j > id_i));
+      const float s2 = squared_distance<D>(q, c);
+      if (higher && s2 <= bound) {
+        const float dist = __builtin_sqrtf(s2);
+        const int64_t key = (static_cast<int64_t>(t) << 32) | static_cast<uint32_t>(id_j);
+        if (dist < delta || key < best) {
+          delta = dist;
+          nh = id_j;
+          best = key;
+          const float lim = sqrt_le_bound_dev(dist);
+          bound = lim < cap ? lim : cap;
+        }
+      }
+    }
+  };
+
+  if constexpr (PHASE == 1) {
+    // the wave's own points: same tile, neighbouring Morton codes
+    const int base = uniform(gid & ~(kWave - 1));
+    if (base < a.n) {
+      const int t0 = uniform(as_const(a.kn)[base] >> a.mshift);
+      const int beg = uniform(max(base, OFF[t0]));
+      const int end = uniform(min(min(base + kWave, OFF[t0 + 1]), a.n));
+      const bool need0 = in_box(t0);
+      if (__ballot(need0) != 0 && need0) scan(t0, beg, end);
+    }
+  }
+
+  int ulo[D], uhi[D];
+  bool any = true;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    ulo[d] = uniform(wave_min_i32(valid ? lo[d] : INT_MAX));
+    uhi[d] = uniform(wave_max_i32(valid ? hi[d] : INT_MIN));
+    any = any && ulo[d] <= uhi[d];
+  }
+  if (any) {
+    int b[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) b[d] = ulo[d];
+    const int lane = __lane_id();
+    while (true) {
+      bool need = valid;
+#pragma unroll
+      for (int d = 0; d < D; ++d) need = need && b[d] >= lo[d] && b[d] <= hi[d];
+      if (__ballot(need) != 0) {
+        const int t = uniform(tile_index<D>(b, a.gs));
+        float tl[D], th[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          tl[d] = TMIN[d][t];
+          th[d] = TMAX[d][t];
+        }
+        need = need && lower_bound2<D>(q, tl, th) <= bound;
+        const float maxb = wave_max_f32(need ? bound : -1.f);
+        if (maxb >= 0.f) {
+          const int k0 = BOFF[t], k1 = BOFF[t + 1];
+          const int off0 = OFF[t], off1 = OFF[t + 1];
+          for (int kb = k0; kb < k1; kb += kWave) {
+            // one block per lane against the wave's query box
+            const int k = kb + lane;
+            const bool inb = k < k1;
+            float bl[D], bh[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+              bl[d] = inb ? a.bmin[d][k] : __builtin_inff();
+              bh[d] = inb ? a.bmax[d][k] : -__builtin_inff();
+            }
+            float lbw = 0.f;
+#pragma unroll
+            for (int d = D - 1; d >= 0; --d) {
+              float g = bl[d] - qhi[d];
+              const float g2 = qlo[d] - bh[d];
+              g = g2 > g ? g2 : g;
+              g = g > 0.f ? g : 0.f;
+              const float tt = g * g;
+              lbw = (d == D - 1) ? tt : tt + lbw;
+            }
+            uint64_t m = __ballot(inb && lbw <= maxb);
+            while (m) {
+              const int kl = __builtin_ctzll(m);
+              m &= m - 1;
+              float ul[D], uh[D];
+#pragma unroll
+              for (int d = 0; d < D; ++d) {
+                ul[d] = readlane_f32(bl[d], kl);
+                uh[d] = readlane_f32(bh[d], kl);
+              }
+              const bool nk = need && lower_bound2<D>(q, ul, uh) <= bound;
+              if (__ballot(nk) != 0) {
+                const int start = off0 + (kb + kl - k0) * kBlock;
+                const int end = min(start + kBlock, off1);
+                if (nk) scan(t, start, end);
+              }
+            }
+          }
+        }
+      }
+      int d = D - 1;
+      for (; d >= 0; --d) {
+        if (++b[d] <= uhi[d]) break;
+        b[d] = ulo[d];
+      }
+      if (d < 0) break;
+    }
+  }
+
+  if constexpr (PHASE == 1) {
+    // unresolved positions are compacted in Morton order (flag -> scan -> scatter)
+    // so that the full search runs on spatially coherent waves
+    if (valid) a.pending_flag[pos] = nh == -1 ? 1 : 0;
+    if (valid && nh != -1) {
+      a.nh_out[id_i] = nh;
+      a.delta_out[id_i] = delta;
+    }
+  } else {
+    if (valid) {
+      a.nh_out[id_i] = nh;
+      a.delta_out[id_i] = delta;
+    }
+    const uint64_t none = __ballot(valid && nh == -1);
+    if (__lane_id() == 0 && none) atomicAdd(a.seed_candidates, static_cast<int>(__popcll(none)));
+  }
+}
+
+// pending[rank] = position for every flagged position; idx is the exclusive scan
+// of the flags (idx[n] = number flagged).
+__global__ void k_compact(const int32_t* idx, int32_t n, int32_t* out) {
+  const int p = blockIdx.x * kThreads + threadIdx.x;
+  if (p < n && idx[p + 1] != idx[p]) out[idx[p]] = p;
 }
 
 // ===========================================================================
@@ -685,6 +1149,7 @@ __global__ void k_selftest(int op, const float* a, const float* b, float* out, i
     case 1: out[i] = __builtin_sqrtf(x); break;
     case 2: out[i] = x / b[i]; break;
     case 3: out[i] = sqrt_lt_bound(x); break;
+    case 4: out[i] = quotient_via_rcp(x, 1.0 / static_cast<double>(b[i])); break;
     default: out[i] = 0.f;
   }
 }
@@ -751,6 +1216,11 @@ struct clue_plan_s {
   int device = 0;
   DevBuf key, key2, perm, perm2, count, offsets, hist, part, xs, ws, hw, rho_s, delta, flag, root,
       nh_tmp, rho_tmp, seed_tmp, bbox_part, grid, counters;
+  // Morton-ordered layout and the block boxes of the nearest-higher search
+  DevBuf kn, kn2, pn, pn2, inv, xn, wn, sn, rn, bcnt, blk_off, bbox_lo, bbox_hi, tbox_lo, tbox_hi;
+  DevBuf pending, pending_flag;
+  clue_tuning tuning{0.25f, 1, 128, -1};
+  int64_t ldb = 0;  // leading dimension of the block boxes
   int64_t ld = 0;  // leading dimension of xs
   // last call
   bool has_run = false;
@@ -810,6 +1280,25 @@ clue_status reserve(clue_plan_s* p, int32_t D, int32_t n, int32_t T) {
   CLUE_HIP(p->rho_tmp.ensure(N * 4));
   CLUE_HIP(p->seed_tmp.ensure(N * 4));
   CLUE_HIP(p->bbox_part.ensure(kBBoxBlocks * kMaxDim * 2 * 4));
+  CLUE_HIP(p->kn.ensure(N * 4));
+  CLUE_HIP(p->kn2.ensure(N * 4));
+  CLUE_HIP(p->pn.ensure(N * 4));
+  CLUE_HIP(p->pn2.ensure(N * 4));
+  CLUE_HIP(p->inv.ensure(N * 4));
+  CLUE_HIP(p->xn.ensure(static_cast<size_t>(ld) * D * 4));
+  CLUE_HIP(p->wn.ensure(N * 4));
+  CLUE_HIP(p->sn.ensure(N * 4));
+  CLUE_HIP(p->rn.ensure(N * 4));
+  CLUE_HIP(p->bcnt.ensure((static_cast<size_t>(T) + 1) * 4));
+  CLUE_HIP(p->blk_off.ensure((static_cast<size_t>(T) + 1) * 4));
+  const int64_t ldb = div_up(n, kBlock) + T + 1;
+  CLUE_HIP(p->bbox_lo.ensure(static_cast<size_t>(ldb) * D * 4));
+  CLUE_HIP(p->bbox_hi.ensure(static_cast<size_t>(ldb) * D * 4));
+  p->ldb = ldb;
+  CLUE_HIP(p->tbox_lo.ensure(static_cast<size_t>(T) * D * 4));
+  CLUE_HIP(p->tbox_hi.ensure(static_cast<size_t>(T) * D * 4));
+  CLUE_HIP(p->pending.ensure((N + 64) * 4));
+  CLUE_HIP(p->pending_flag.ensure((N + 1) * 4));
   if (!p->grid.p) CLUE_HIP(p->grid.ensure(sizeof(DevGrid)));
   if (!p->counters.p) {
     CLUE_HIP(p->counters.ensure(16 * 4));
@@ -837,9 +1326,10 @@ clue_status scan_exclusive(const int32_t* in, int32_t* out, int64_t n, int32_t* 
 
 template <int D>
 clue_status launch_bin(const Coords& c, int32_t n, clue_plan_s* p, const GridShape& gs,
-                       hipStream_t st) {
-  k_bin<D><<<div_up(n, kThreads), kThreads, 0, st>>>(c, n, p->grid.as<DevGrid>(), gs,
-                                                      p->key.as<int32_t>(), p->count.as<int32_t>());
+                       int sub_bits, hipStream_t st) {
+  k_bin<D><<<div_up(n, kThreads), kThreads, 0, st>>>(
+      c, n, p->grid.as<DevGrid>(), gs, p->key.as<int32_t>(), p->count.as<int32_t>(),
+      sub_bits >= 0 ? p->kn.as<int32_t>() : nullptr, sub_bits < 0 ? 0 : sub_bits);
   CLUE_LAUNCH_CHECK("bin");
   return CLUE_OK;
 }
@@ -848,7 +1338,7 @@ clue_status mark(clue_plan_s* p, int k, hipStream_t st);
 
 template <int D, int METRIC>
 clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, float dm,
-                          hipStream_t st, clue_plan_s* p) {
+                          hipStream_t st, clue_plan_s* p, bool blocks) {
   constexpr bool U = (D <= 3);
   const int grid = static_cast<int>(div_up(base_args.n, kThreads));
   PassArgs a = base_args;
@@ -856,17 +1346,41 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
   a.thr = sqrt_le_bound(dc);
   clue_status rc = mark(p, 4, st);
   if (rc != CLUE_OK) return rc;
-  switch (kernel_type) {
-    case KFLAT: k_density<D, METRIC, KFLAT, U><<<grid, kThreads, 0, st>>>(a); break;
-    case KGAUSS: k_density<D, METRIC, KGAUSS, U><<<grid, kThreads, 0, st>>>(a); break;
-    case KEXP: k_density<D, METRIC, KEXP, U><<<grid, kThreads, 0, st>>>(a); break;
-    default: return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown convolutional kernel type");
+  if (blocks) {
+    switch (kernel_type) {
+      case KFLAT: k_density<D, METRIC, KFLAT, U, true><<<grid, kThreads, 0, st>>>(a); break;
+      case KGAUSS: k_density<D, METRIC, KGAUSS, U, true><<<grid, kThreads, 0, st>>>(a); break;
+      case KEXP: k_density<D, METRIC, KEXP, U, true><<<grid, kThreads, 0, st>>>(a); break;
+      default: return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown convolutional kernel type");
+    }
+  } else {
+    switch (kernel_type) {
+      case KFLAT: k_density<D, METRIC, KFLAT, U, false><<<grid, kThreads, 0, st>>>(a); break;
+      case KGAUSS: k_density<D, METRIC, KGAUSS, U, false><<<grid, kThreads, 0, st>>>(a); break;
+      case KEXP: k_density<D, METRIC, KEXP, U, false><<<grid, kThreads, 0, st>>>(a); break;
+      default: return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown convolutional kernel type");
+    }
   }
   CLUE_LAUNCH_CHECK("density");
   a.radius = dm;
   a.thr = sqrt_le_bound(dm);
   rc = mark(p, 5, st);
   if (rc != CLUE_OK) return rc;
+  if constexpr (METRIC == MEUCLID && D <= 3) {
+    if (blocks) {
+      a.thr1 = sqrt_le_bound(dm * p->tuning.nh_first_radius);
+      k_nearest_higher_blocks<D, 1><<<grid, kThreads, 0, st>>>(a);
+      CLUE_LAUNCH_CHECK("nearest_higher_blocks");
+      rc = scan_exclusive(a.pending_flag, a.pending_flag, a.n, p->part.as<int32_t>(), st);
+      if (rc != CLUE_OK) return rc;
+      k_compact<<<grid, kThreads, 0, st>>>(a.pending_flag, a.n, a.pending);
+      CLUE_LAUNCH_CHECK("compact");
+      a.pending_count = a.pending_flag + a.n;
+      k_nearest_higher_blocks<D, 2><<<grid, kThreads, 0, st>>>(a);
+      CLUE_LAUNCH_CHECK("nearest_higher_blocks_full");
+      return CLUE_OK;
+    }
+  }
   k_nearest_higher<D, METRIC, U><<<grid, kThreads, 0, st>>>(a);
   CLUE_LAUNCH_CHECK("nearest_higher");
   return CLUE_OK;
@@ -881,14 +1395,14 @@ clue_status launch_classify(const ClassifyArgs& a, hipStream_t st) {
 
 template <int D>
 clue_status dispatch_metric_passes(int metric, const PassArgs& a, int kt, float dc, float dm,
-                                   hipStream_t st, clue_plan_s* p) {
+                                   hipStream_t st, clue_plan_s* p, bool blocks) {
   switch (metric) {
-    case MEUCLID: return launch_passes<D, MEUCLID>(a, kt, dc, dm, st, p);
-    case MWEUCLID: return launch_passes<D, MWEUCLID>(a, kt, dc, dm, st, p);
-    case MPERIODIC: return launch_passes<D, MPERIODIC>(a, kt, dc, dm, st, p);
-    case MMANHATTAN: return launch_passes<D, MMANHATTAN>(a, kt, dc, dm, st, p);
-    case MCHEBYSHEV: return launch_passes<D, MCHEBYSHEV>(a, kt, dc, dm, st, p);
-    case MWCHEBYSHEV: return launch_passes<D, MWCHEBYSHEV>(a, kt, dc, dm, st, p);
+    case MEUCLID: return launch_passes<D, MEUCLID>(a, kt, dc, dm, st, p, blocks);
+    case MWEUCLID: return launch_passes<D, MWEUCLID>(a, kt, dc, dm, st, p, false);
+    case MPERIODIC: return launch_passes<D, MPERIODIC>(a, kt, dc, dm, st, p, false);
+    case MMANHATTAN: return launch_passes<D, MMANHATTAN>(a, kt, dc, dm, st, p, false);
+    case MCHEBYSHEV: return launch_passes<D, MCHEBYSHEV>(a, kt, dc, dm, st, p, false);
+    case MWCHEBYSHEV: return launch_passes<D, MWCHEBYSHEV>(a, kt, dc, dm, st, p, false);
   }
   return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown distance metric type");
 }
@@ -951,6 +1465,48 @@ clue_status validate(const clue_params* prm, int32_t ndim, int32_t n, const floa
   return CLUE_OK;
 }
 
+// Stable LSD counting sort of n int32 keys (< 2^total_bits) with their values
+// (identity when vals is NULL), ping-ponging through (k0,k1) and (v0,v1).
+clue_status radix_sort(clue_plan_s* p, hipStream_t st, int32_t n, int total_bits, int32_t* k0,
+                       int32_t* k1, int32_t* v0, int32_t* v1, bool keep_keys,
+                       int32_t** keys_sorted, int32_t** vals_sorted) {
+  const int passes = std::max(1, (total_bits + 7) / 8);
+  const int bpp = std::max(1, (total_bits + passes - 1) / passes);
+  const int nblocks = static_cast<int>(div_up(n, kSortTile));
+  int32_t* kin = k0;
+  int32_t* vin = nullptr;
+  int32_t* kout = k1;
+  int32_t* vout = v0;
+  for (int pass = 0; pass < passes; ++pass) {
+    const int shift = pass * bpp;
+    const bool last = pass == passes - 1;
+    k_radix_hist<<<nblocks, kThreads, 0, st>>>(kin, n, shift, bpp, nblocks, p->hist.as<int32_t>());
+    CLUE_LAUNCH_CHECK("radix_hist");
+    clue_status rc = scan_exclusive(p->hist.as<int32_t>(), p->hist.as<int32_t>(),
+                                    static_cast<int64_t>(1 << bpp) * nblocks, p->part.as<int32_t>(), st);
+    if (rc != CLUE_OK) return rc;
+    k_radix_scatter<<<nblocks, kThreads, 0, st>>>(kin, vin, n, shift, bpp, nblocks,
+                                                  p->hist.as<int32_t>(),
+                                                  (last && !keep_keys) ? nullptr : kout, vout);
+    CLUE_LAUNCH_CHECK("radix_scatter");
+    int32_t* next_kout = (kout == k1) ? k0 : k1;
+    int32_t* next_vout = (vout == v0) ? v1 : v0;
+    kin = kout;
+    vin = vout;
+    kout = next_kout;
+    vout = next_vout;
+  }
+  *keys_sorted = kin;
+  *vals_sorted = vin;
+  return CLUE_OK;
+}
+
+int bits_for(int64_t count) {
+  int b = 0;
+  while ((int64_t{1} << b) < count) ++b;
+  return b;
+}
+
 clue_status mark(clue_plan_s* p, int k, hipStream_t st) {
   if (!p->timing) return CLUE_OK;
   if (!p->ev[k]) CLUE_HIP(hipEventCreate(&p->ev[k]));
@@ -1007,58 +1563,73 @@ clue_status make_clusters(clue_plan_s* p, hipStream_t st, const clue_params* prm
   k_bbox_final<<<1, 64, 0, st>>>(p->bbox_part.as<float>(), bb, D, npd, p->grid.as<DevGrid>());
   CLUE_LAUNCH_CHECK("bbox_final");
 
-  // ---- tile ids + histogram
+  // The block search of the nearest-higher pass needs the Euclidean metric on a
+  // grid without wrapping; other cases run the tile walk for both passes.
+  bool wrapped_any = false;
+  for (int d = 0; d < D; ++d) wrapped_any = wrapped_any || gs.wrapped[d];
+  const int tbits = bits_for(T);
+  int sub_bits = std::min(D == 1 ? 10 : (D == 2 ? 5 : 4), (31 - tbits) / D);
+  if (p->tuning.morton_bits >= 0) sub_bits = std::min(p->tuning.morton_bits, (31 - tbits) / D);
+  const bool blocks = md.type == MEUCLID && !wrapped_any && D <= 3 && sub_bits >= 1;
+  if (!blocks) sub_bits = -1;
+
+  // ---- tile ids + histogram (+ Morton keys)
   CLUE_MARK(1);
   k_fill_i32<<<div_up(T + 1, kThreads), kThreads, 0, st>>>(p->count.as<int32_t>(), 0, T + 1);
   CLUE_LAUNCH_CHECK("fill");
-  rc = dispatch_dim(D, [&](auto dc_) { return launch_bin<decltype(dc_)::value>(c, n, p, gs, st); });
+  rc = dispatch_dim(D, [&](auto dc_) {
+    return launch_bin<decltype(dc_)::value>(c, n, p, gs, sub_bits, st);
+  });
   if (rc != CLUE_OK) return rc;
 
-  // ---- stable counting sort by tile id
+  // ---- stable counting sort by tile id: the reference's CSR order
   CLUE_MARK(2);
-  int total_bits = 0;
-  while ((1ll << total_bits) < T) ++total_bits;
-  const int passes = std::max(1, (total_bits + 7) / 8);
-  const int bpp = std::max(1, (total_bits + passes - 1) / passes);
-  const int nblocks = static_cast<int>(div_up(n, kSortTile));
-  int32_t* kin = p->key.as<int32_t>();
-  int32_t* vin = nullptr;
-  int32_t* kout = p->key2.as<int32_t>();
-  int32_t* vout = p->perm.as<int32_t>();
-  for (int pass = 0; pass < passes; ++pass) {
-    const int shift = pass * bpp;
-    const bool last = pass == passes - 1;
-    k_radix_hist<<<nblocks, kThreads, 0, st>>>(kin, n, shift, bpp, nblocks, p->hist.as<int32_t>());
-    CLUE_LAUNCH_CHECK("radix_hist");
-    rc = scan_exclusive(p->hist.as<int32_t>(), p->hist.as<int32_t>(),
-                        static_cast<int64_t>(1 << bpp) * nblocks, p->part.as<int32_t>(), st);
-    if (rc != CLUE_OK) return rc;
-    k_radix_scatter<<<nblocks, kThreads, 0, st>>>(kin, vin, n, shift, bpp, nblocks,
-                                                  p->hist.as<int32_t>(), last ? nullptr : kout,
-                                                  vout);
-    CLUE_LAUNCH_CHECK("radix_scatter");
-    // next pass reads what this one wrote
-    int32_t* next_vout = (vout == p->perm.as<int32_t>()) ? p->perm2.as<int32_t>() : p->perm.as<int32_t>();
-    int32_t* next_kout = (kout == p->key2.as<int32_t>()) ? p->key.as<int32_t>() : p->key2.as<int32_t>();
-    kin = kout;
-    vin = vout;
-    kout = next_kout;
-    vout = next_vout;
-  }
-  int32_t* perm = vin;  // final permutation: slot -> point id
+  int32_t* sorted_keys = nullptr;
+  int32_t* perm = nullptr;  // slot -> point id
+  rc = radix_sort(p, st, n, tbits, p->key.as<int32_t>(), p->key2.as<int32_t>(), p->perm.as<int32_t>(),
+                  p->perm2.as<int32_t>(), false, &sorted_keys, &perm);
+  if (rc != CLUE_OK) return rc;
   p->last_perm = perm;
+  // Morton layout: sort by (tile id, Morton code inside the tile)
+  int32_t* mkeys = nullptr;
+  int32_t* pn = nullptr;
+  if (blocks) {
+    rc = radix_sort(p, st, n, tbits + sub_bits * D, p->kn.as<int32_t>(), p->kn2.as<int32_t>(),
+                    p->pn.as<int32_t>(), p->pn2.as<int32_t>(), true, &mkeys, &pn);
+    if (rc != CLUE_OK) return rc;
+  }
 
   // ---- tile offsets
   CLUE_MARK(3);
   rc = scan_exclusive(p->count.as<int32_t>(), p->offsets.as<int32_t>(), T, p->part.as<int32_t>(), st);
   if (rc != CLUE_OK) return rc;
 
-  // ---- tile-ordered SoA
+  // ---- tile-ordered SoA (+ Morton-ordered SoA and block boxes)
   const float h = kd.type == CLUE_KERNEL_FLAT ? kd.params[0] : 0.f;
   k_gather<<<div_up(n, kThreads), kThreads, 0, st>>>(c, weights, perm, n, D, p->ld,
                                                      p->xs.as<float>(), p->ws.as<float>(),
                                                      p->hw.as<float>(), h);
   CLUE_LAUNCH_CHECK("gather");
+  if (blocks) {
+    k_inverse<<<div_up(n, kThreads), kThreads, 0, st>>>(perm, n, p->inv.as<int32_t>());
+    CLUE_LAUNCH_CHECK("inverse");
+    k_gather_morton<<<div_up(n, kThreads), kThreads, 0, st>>>(c, weights, pn, p->inv.as<int32_t>(), n, D,
+                                                              p->ld, p->xn.as<float>(), p->wn.as<float>(),
+                                                              p->sn.as<int32_t>());
+    CLUE_LAUNCH_CHECK("gather_morton");
+    k_block_count<<<div_up(T, kThreads), kThreads, 0, st>>>(p->offsets.as<int32_t>(), T, p->bcnt.as<int32_t>());
+    CLUE_LAUNCH_CHECK("block_count");
+    rc = scan_exclusive(p->bcnt.as<int32_t>(), p->blk_off.as<int32_t>(), T, p->part.as<int32_t>(), st);
+    if (rc != CLUE_OK) return rc;
+    k_block_boxes<<<div_up(div_up(n, kBlock) + T, kThreads), kThreads, 0, st>>>(
+        p->xn.as<float>(), p->ld, D, p->offsets.as<int32_t>(), p->blk_off.as<int32_t>(), T,
+        p->bbox_lo.as<float>(), p->bbox_hi.as<float>(), p->ldb);
+    CLUE_LAUNCH_CHECK("block_boxes");
+    k_tile_boxes<<<div_up(T, kThreads), kThreads, 0, st>>>(p->bbox_lo.as<float>(), p->bbox_hi.as<float>(),
+                                                           p->ldb, D, p->blk_off.as<int32_t>(), T,
+                                                           p->tbox_lo.as<float>(), p->tbox_hi.as<float>());
+    CLUE_LAUNCH_CHECK("tile_boxes");
+  }
 
   // ---- density + nearest-higher
   k_fill_i32<<<1, 64, 0, st>>>(p->counters.as<int32_t>(), 0, 1);  // seed candidates
@@ -1074,14 +1645,36 @@ clue_status make_clusters(clue_plan_s* p, hipStream_t st, const clue_params* prm
   a.gs = gs;
   a.n = n;
   for (int k = 0; k < 3; ++k) a.kp[k] = kd.params[k];
+  a.kden_rcp = 1.0 / static_cast<double>((2.f * kd.params[1]) * kd.params[1]);
   for (int d = 0; d < CLUE_MAX_DIM; ++d) a.mp[d] = md.params[d];
+  if (blocks) {
+    for (int d = 0; d < D; ++d) {
+      a.xn[d] = p->xn.as<float>() + d * p->ld;
+      a.bmin[d] = p->bbox_lo.as<float>() + d * p->ldb;
+      a.bmax[d] = p->bbox_hi.as<float>() + d * p->ldb;
+      a.tmin[d] = p->tbox_lo.as<float>() + static_cast<int64_t>(d) * T;
+      a.tmax[d] = p->tbox_hi.as<float>() + static_cast<int64_t>(d) * T;
+    }
+    a.wn = p->wn.as<float>();
+    a.sn = p->sn.as<int32_t>();
+    a.pn = pn;
+    a.kn = mkeys;
+    a.mshift = sub_bits * D;
+    a.blk_off = p->blk_off.as<int32_t>();
+    a.rn_out = p->rn.as<float>();
+    a.rn = p->rn.as<float>();
+    a.pending = p->pending.as<int32_t>();
+    a.pending_flag = p->pending_flag.as<int32_t>();
+    a.mask_min_points = p->tuning.density_bitmask ? std::max(p->tuning.bitmask_min_points, 1) : 0;
+  }
   a.rho_s_out = p->rho_s.as<float>();
   a.rho_out = rho_buf;
   a.nh_out = nh_buf;
   a.delta_out = p->delta.as<float>();
   a.seed_candidates = p->counters.as<int32_t>();
   rc = dispatch_dim(D, [&](auto dc_) {
-    return dispatch_metric_passes<decltype(dc_)::value>(md.type, a, kd.type, prm->dc, prm->dm, st, p);
+    return dispatch_metric_passes<decltype(dc_)::value>(md.type, a, kd.type, prm->dc, prm->dm, st, p,
+                                                        blocks);
   });
   if (rc != CLUE_OK) return rc;
 
@@ -1240,7 +1833,10 @@ clue_status clue_plan_destroy(clue_plan_t plan) {
   for (DevBuf* b : {&plan->key, &plan->key2, &plan->perm, &plan->perm2, &plan->count, &plan->offsets,
                     &plan->hist, &plan->part, &plan->xs, &plan->ws, &plan->hw, &plan->rho_s,
                     &plan->delta, &plan->flag, &plan->root, &plan->nh_tmp, &plan->rho_tmp,
-                    &plan->seed_tmp, &plan->bbox_part, &plan->grid, &plan->counters})
+                    &plan->seed_tmp, &plan->bbox_part, &plan->grid, &plan->counters, &plan->kn,
+                    &plan->kn2, &plan->pn, &plan->pn2, &plan->inv, &plan->xn, &plan->wn, &plan->sn,
+                    &plan->rn, &plan->bcnt, &plan->blk_off, &plan->bbox_lo, &plan->bbox_hi,
+                    &plan->tbox_lo, &plan->tbox_hi, &plan->pending, &plan->pending_flag})
     b->release();
   for (hipEvent_t& e : plan->ev)
     if (e) (void)hipEventDestroy(e);
@@ -1273,7 +1869,10 @@ clue_status clue_plan_device_bytes(clue_plan_t plan, size_t* bytes) {
   for (DevBuf* b : {&plan->key, &plan->key2, &plan->perm, &plan->perm2, &plan->count, &plan->offsets,
                     &plan->hist, &plan->part, &plan->xs, &plan->ws, &plan->hw, &plan->rho_s,
                     &plan->delta, &plan->flag, &plan->root, &plan->nh_tmp, &plan->rho_tmp,
-                    &plan->seed_tmp, &plan->bbox_part, &plan->grid, &plan->counters})
+                    &plan->seed_tmp, &plan->bbox_part, &plan->grid, &plan->counters, &plan->kn,
+                    &plan->kn2, &plan->pn, &plan->pn2, &plan->inv, &plan->xn, &plan->wn, &plan->sn,
+                    &plan->rn, &plan->bcnt, &plan->blk_off, &plan->bbox_lo, &plan->bbox_hi,
+                    &plan->tbox_lo, &plan->tbox_hi, &plan->pending, &plan->pending_flag})
     s += b->bytes;
   *bytes = s;
   return CLUE_OK;
@@ -1388,6 +1987,20 @@ clue_status clue_plan_copy_delta(clue_plan_t plan, float* delta) {
   return CLUE_OK;
 }
 
+clue_status clue_plan_set_tuning(clue_plan_t plan, const clue_tuning* tuning) {
+  if (!plan || !tuning) return fail(CLUE_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!(tuning->nh_first_radius > 0.f) || tuning->bitmask_min_points < 0 || tuning->morton_bits > 10)
+    return fail(CLUE_ERR_INVALID_ARGUMENT, "invalid tuning values");
+  plan->tuning = *tuning;
+  return CLUE_OK;
+}
+
+clue_status clue_plan_get_tuning(clue_plan_t plan, clue_tuning* tuning) {
+  if (!plan || !tuning) return fail(CLUE_ERR_INVALID_ARGUMENT, "NULL argument");
+  *tuning = plan->tuning;
+  return CLUE_OK;
+}
+
 clue_status clue_plan_set_timing(clue_plan_t plan, int32_t enable) {
   if (!plan) return fail(CLUE_ERR_INVALID_ARGUMENT, "plan is NULL");
   plan->timing = enable != 0;
@@ -1413,7 +2026,7 @@ const char* clue_stage_name(int32_t stage) {
 clue_status clue_selftest_math(int32_t op, const float* a_dev, const float* b_dev, float* out_dev,
                                int64_t n, void* stream) {
   if (n <= 0) return CLUE_OK;
-  if (!a_dev || !out_dev || (op == 2 && !b_dev))
+  if (!a_dev || !out_dev || ((op == 2 || op == 4) && !b_dev))
     return fail(CLUE_ERR_INVALID_ARGUMENT, "NULL argument");
   k_selftest<<<div_up(n, kThreads), kThreads, 0, static_cast<hipStream_t>(stream)>>>(op, a_dev, b_dev,
                                                                                    out_dev, n);

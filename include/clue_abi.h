@@ -95,6 +95,20 @@ typedef struct {
   float tile_size[CLUE_MAX_DIM];
 } clue_grid_info;
 
+/* Launch tuning of a plan.  Every value leaves the results unchanged (they only
+ * steer work order and pruning); the autotuner picks them by timing.
+ *  nh_first_radius     first nearest-higher phase radius, as a fraction of dm
+ *  density_bitmask     1: large tiles walk only the candidates near the wave
+ *                      through an LDS bitmask (Euclidean, no wrapping)
+ *  bitmask_min_points  smallest tile that takes the bitmask walk
+ *  morton_bits         bits per dimension of the in-tile Morton order, -1 auto */
+typedef struct {
+  float nh_first_radius;
+  int32_t density_bitmask;
+  int32_t bitmask_min_points;
+  int32_t morton_bits;
+} clue_tuning;
+
 typedef struct clue_plan_s* clue_plan_t;
 
 /* ---- errors ------------------------------------------------------------- */
@@ -132,6 +146,10 @@ clue_status clue_plan_reset_capacity(clue_plan_t plan);
 clue_status clue_plan_reserve(clue_plan_t plan, int32_t ndim, int32_t npoints);
 /* Device bytes currently held by the plan. */
 clue_status clue_plan_device_bytes(clue_plan_t plan, size_t* bytes);
+/* Launch tuning (no reference counterpart: the reference exposes only the
+ * alpaka block size, which the engine does not need). */
+clue_status clue_plan_set_tuning(clue_plan_t plan, const clue_tuning* tuning);
+clue_status clue_plan_get_tuning(clue_plan_t plan, clue_tuning* tuning);
 
 /* ---- the clustering path: Clusterer::make_clusters(queue, PointsDevice&,
  *      metric, kernel, block_size), core/detail/Clusterer.hpp:84-95 and

@@ -54,12 +54,14 @@ inline int32_t f2i(float v) {
 }
 
 // ----------------------------------------------------------------------------
-// Engine exponential.  exp(x) = 2^(k/64) * exp(r), k = rint(x*64/ln2),
-// r = x - k*ln2/64 (two-constant Cody-Waite split), exp(r)-1 by a degree-5
-// Taylor polynomial (|r| <= ln2/128, truncation < 4e-17), 2^(j/64) from a
-// 64-entry table rounded to double; the double result is rounded once to
-// float.  Only IEEE-754 double operations are used, so the GPU engine
-// evaluates the identical sequence to identical bits.
+// Engine exponential: the correctly rounded float exp, evaluated with IEEE
+// double operations only (each std::fma is one correctly rounded fused
+// multiply-add).  exp(x) = 2^(k/64) * exp(r), k = rint(x*64/ln2),
+// r = x - k*ln2/64 (two-constant Cody-Waite split; hi has 33 significant bits so
+// k*hi is exact), exp(r) - 1 by a degree-5 Taylor polynomial in Horner form
+// (|r| <= ln2/128, truncation < 4e-17), 2^(j/64) from a 64-entry table rounded
+// to double, one final rounding double -> float.  The GPU engine evaluates the
+// identical sequence, so both produce identical bits.
 // ----------------------------------------------------------------------------
 double g_exp_table[64];
 bool g_exp_table_ready = false;
@@ -71,11 +73,9 @@ void init_exp_table() {
   g_exp_table_ready = true;
 }
 
-// ln2/64 split: hi has 33 significant bits so k*hi is exact for |k| < 2^20;
-// lo = ln2/64 - hi rounded to double.
 constexpr double kInvLn2x64 = 0x1.71547652b82fep+6;  // 64/ln2 rounded to double
-constexpr double kLn2by64Hi = 0x1.62e42fefp-7;
-constexpr double kLn2by64Lo = 0x1.473de6af278edp-40;
+constexpr double kLn2by64Hi = 0x1.62e42fefp-7;       // 33 significant bits
+constexpr double kLn2by64Lo = 0x1.473de6af278edp-40; // ln2/64 - hi, rounded
 
 inline float expf_engine(float x) {
   if (x != x) return x;
@@ -84,15 +84,16 @@ inline float expf_engine(float x) {
   const double xd = static_cast<double>(x);
   const double kd = std::rint(xd * kInvLn2x64);
   const int k = static_cast<int>(kd);
-  double r = xd - kd * kLn2by64Hi;
-  r = r - kd * kLn2by64Lo;
-  const double p =
-      r + (r * r) * (0.5 + r * (1.0 / 6.0 + r * (1.0 / 24.0 + r * (1.0 / 120.0))));
+  double r = std::fma(-kd, kLn2by64Hi, xd);
+  r = std::fma(-kd, kLn2by64Lo, r);
+  double p = std::fma(r, 1.0 / 120.0, 1.0 / 24.0);
+  p = std::fma(r, p, 1.0 / 6.0);
+  p = std::fma(r, p, 0.5);
+  p = std::fma(r * r, p, r);
   const int j = k & 63;
   const int m = (k - j) / 64;
   const double t = g_exp_table[j];
-  const double y = t + t * p;
-  return static_cast<float>(std::ldexp(y, m));
+  return static_cast<float>(std::ldexp(std::fma(t, p, t), m));
 }
 
 inline float expf_sel(float x, int mode) {

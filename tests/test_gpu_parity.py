@@ -15,11 +15,13 @@ pytestmark = pytest.mark.gpu
 
 
 def run_both(coords, weights, dc, rhoc, dm=None, seed_dc=None, ppb=128, kernel=("flat", 0.5),
-             metric="euclidean", metric_params=None, wrapped=None, seed_mode=0, plan=None):
+             metric="euclidean", metric_params=None, wrapped=None, seed_mode=0, plan=None, tuning=None):
     print(f"case: ndim={np.atleast_2d(coords).shape[0]} n={np.atleast_2d(coords).shape[1]} dc={dc} "
           f"rhoc={rhoc} dm={dm} kernel={kernel} metric={metric} wrapped={wrapped}", flush=True)
     own = plan is None
     plan = plan or clue.Plan(0)
+    if tuning:
+        plan.set_tuning(**tuning)
     try:
         got = clue.cluster_points(coords, weights, dc, rhoc, dm, seed_dc, ppb, kernel, metric, metric_params,
                                   wrapped, seed_mode, plan=plan)
@@ -68,6 +70,14 @@ def test_device_math_matches_host():
     np.testing.assert_array_equal(clue.selftest_math(1, a).view(np.int32), np.sqrt(a).view(np.int32))
     b = (rng.random(200000).astype(np.float32) + np.float32(0.01)) * np.float32(7.3)
     np.testing.assert_array_equal(clue.selftest_math(2, a, b).view(np.int32), (a / b).view(np.int32))
+    # the Gaussian argument's quotient through a double reciprocal equals the IEEE float division
+    num = -(rng.standard_normal(300000).astype(np.float32) ** 2) * np.float32(3.0)
+    num[:2000] = -(rng.random(2000).astype(np.float32) * np.float32(1e-30))
+    num[2000:4000] = -np.ldexp(rng.random(2000), -130).astype(np.float32)
+    den = (rng.random(300000).astype(np.float32) * np.float32(20) + np.float32(1e-3))
+    den[4000:6000] = np.float32(2.0)
+    den[6000:8000] = (rng.random(2000).astype(np.float32) * np.float32(1e-38))
+    np.testing.assert_array_equal(clue.selftest_math(4, num, den).view(np.int32), (num / den).view(np.int32))
     # sqrt_lt_bound: largest s with sqrtf(s) < delta
     delta = np.abs(rng.standard_normal(20000).astype(np.float32)) * np.float32(10) + np.float32(1e-3)
     bnd = clue.selftest_math(3, delta)
@@ -85,6 +95,20 @@ def test_workload_parity(name, n):
     ps = datasets.workload(name, n=n)
     prm = datasets.WORKLOADS[name]
     assert_same(*run_both(ps.coords, ps.weights, prm["dc"], prm["rhoc"], prm["dm"], kernel=prm["kernel"]))
+
+
+TUNINGS = [dict(nh_first_radius=0.05, density_bitmask=1, bitmask_min_points=1, morton_bits=5),
+           dict(nh_first_radius=1.0, density_bitmask=0, bitmask_min_points=0, morton_bits=1),
+           dict(nh_first_radius=0.5, density_bitmask=1, bitmask_min_points=64, morton_bits=-1)]
+
+
+@pytest.mark.parametrize("tuning", TUNINGS)
+@pytest.mark.parametrize("name,n", [("2d_1m_gauss", 50000), ("3d_4m_exp", 50000)])
+def test_tunings_do_not_change_results(name, n, tuning):
+    ps = datasets.workload(name, n=n, seed=11)
+    prm = datasets.WORKLOADS[name]
+    assert_same(*run_both(ps.coords, ps.weights, prm["dc"], prm["rhoc"], prm["dm"], kernel=prm["kernel"],
+                          tuning=tuning))
 
 
 def test_config1_golden(config1_golden):
