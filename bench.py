@@ -35,6 +35,9 @@ def parse():
     ap.add_argument("--workload", default="2d_1m_gauss")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--calibration-copy-mib", type=int, default=64,
+                    help="size of one device-to-device copy made before timing; its rocprofv3 FETCH_SIZE / "
+                         "WRITE_SIZE calibrate the HBM byte counters (0 disables)")
     ap.add_argument("--window-fraction", type=float, default=0.25,
                     help="side of the serial CPU sample window, as a fraction of the box side")
     return ap.parse_args()
@@ -112,6 +115,15 @@ def main():
     kd = clue.kernel_desc(prm["kernel"])
     md = clue.metric_desc("euclidean")
     cptrs = [dcoords.offset(d * N) for d in range(D)]
+
+    if args.calibration_copy_mib > 0:
+        nbytes = args.calibration_copy_mib << 20
+        src = clue.DeviceArray((nbytes // 4,), np.float32)
+        dst = clue.DeviceArray((nbytes // 4,), np.float32)
+        clue.check(clue.lib().clue_memcpy_d2d(dst.ptr, src.ptr, nbytes, stream.handle))
+        stream.synchronize()
+        src.free()
+        dst.free()
 
     def step():
         plan.make_clusters_device(stream, params, D, N, cptrs, dw.ptr, kd, md, None, out["cluster"].ptr,

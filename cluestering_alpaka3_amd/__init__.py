@@ -108,6 +108,9 @@ _SIGNATURES = {
     "clue_make_clusters": ([_vp, _vp, _P(ClueParams), ctypes.c_int32, ctypes.c_int32, _P(_vp), _vp,
                             _P(ClueKernelDesc), _P(ClueMetricDesc), _P(ctypes.c_uint8), _vp, _vp,
                             _vp, _vp], ctypes.c_int),
+    "clue_make_clusters_batched": ([_vp, _vp, _P(ClueParams), ctypes.c_int32, ctypes.c_int32,
+                                    _P(ctypes.c_int64), _P(_vp), _vp, _P(ClueKernelDesc),
+                                    _P(ClueMetricDesc), _P(ctypes.c_uint8), _vp, _vp, _vp, _vp], ctypes.c_int),
     "clue_main_run": ([ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int32,
                        _P(ctypes.c_uint8), _P(ctypes.c_float), _P(ctypes.c_int32), _P(ClueKernelDesc),
                        ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32], ctypes.c_int),
@@ -292,6 +295,26 @@ class Plan:
                                        ndim, n, cp, weight_ptr, ctypes.byref(kernel), ctypes.byref(metric),
                                        wr, cluster_ptr, is_seed_ptr, rho_ptr, nh_ptr))
 
+    def make_clusters_batched_device(self, stream, params, ndim, event_offsets, coord_ptrs, weight_ptr,
+                                     kernel, metric, wrapped, cluster_ptr, is_seed_ptr=None, rho_ptr=None,
+                                     nh_ptr=None):
+        """Enqueue one batched clustering: event e is points [event_offsets[e],
+        event_offsets[e+1]) of the device arrays; every event is clustered as a
+        fresh Clusterer would cluster it alone (labels and nearest-higher ids local
+        to the event).  No synchronisation."""
+        off = np.ascontiguousarray(event_offsets, dtype=np.int64)
+        if off.ndim != 1 or off.size < 2:
+            raise InvalidArgument(1, "event_offsets needs n_events + 1 entries")
+        cp = (_vp * MAX_DIM)(*([int(p) for p in coord_ptrs] + [0] * (MAX_DIM - len(coord_ptrs))))
+        wr = None
+        if wrapped is not None:
+            wr = (ctypes.c_uint8 * MAX_DIM)(*([int(v) for v in wrapped] + [0] * (MAX_DIM - len(wrapped))))
+        check(lib().clue_make_clusters_batched(self.handle, stream.handle if stream else None,
+                                               ctypes.byref(params), ndim, off.size - 1,
+                                               off.ctypes.data_as(_P(ctypes.c_int64)), cp, weight_ptr,
+                                               ctypes.byref(kernel), ctypes.byref(metric), wr, cluster_ptr,
+                                               is_seed_ptr, rho_ptr, nh_ptr))
+
     def stats(self):
         s = ClueStats()
         check(lib().clue_plan_last_stats(self.handle, ctypes.byref(s)))
@@ -394,6 +417,48 @@ def cluster_points(coords, weights, dc, rhoc, dm=None, seed_dc=None, ppb=128, ke
         plan.make_clusters_device(stream, params, ndim, n, [dc_arr.offset(d * n) for d in range(ndim)], dw.ptr,
                                   kernel_desc(kernel), metric_desc(metric, metric_params), wrapped,
                                   out_c.ptr, out_s.ptr, out_r.ptr, out_nh.ptr)
+        stream.synchronize()
+        return Result(out_c.download(stream), out_s.download(stream), out_r.download(stream),
+                      out_nh.download(stream))
+    finally:
+        if own_stream:
+            stream.close()
+        if own_plan:
+            plan.close()
+
+
+def cluster_events(coords, weights, event_offsets, dc, rhoc, dm=None, seed_dc=None, ppb=128,
+                   kernel=("flat", 0.5), metric="euclidean", metric_params=None, wrapped=None,
+                   seed_mode=SEEDS_REFERENCE, device=0, plan=None, stream=None):
+    """Cluster a batch of independent events in one call.  coords: (ndim, n) float32
+    SoA of all events back to back; event e is columns [event_offsets[e],
+    event_offsets[e+1]).  Returns a Result in batch order whose labels and
+    nearest-higher ids are local to each event."""
+    c = np.ascontiguousarray(coords, dtype=np.float32)
+    if c.ndim == 1:
+        c = c[None, :]
+    ndim, n = c.shape
+    if not 1 <= ndim <= MAX_DIM:
+        raise InvalidArgument(1, "number of dimensions must be in [1, 10]")
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    off = np.ascontiguousarray(event_offsets, dtype=np.int64)
+    if off.size < 2 or off[0] != 0 or off[-1] != n:
+        raise InvalidArgument(1, "event_offsets must run from 0 to the number of points")
+    own_plan = plan is None
+    plan = plan or Plan(device)
+    own_stream = stream is None
+    stream = stream or Stream(device)
+    try:
+        dc_arr = DeviceArray.from_host(c, device)
+        dw = DeviceArray.from_host(w, device)
+        out_c = DeviceArray((n,), np.int32, device)
+        out_s = DeviceArray((n,), np.int32, device)
+        out_r = DeviceArray((n,), np.float32, device)
+        out_nh = DeviceArray((n,), np.int32, device)
+        params = make_params(dc, rhoc, dm, seed_dc, ppb, seed_mode)
+        plan.make_clusters_batched_device(stream, params, ndim, off, [dc_arr.offset(d * n) for d in range(ndim)],
+                                          dw.ptr, kernel_desc(kernel), metric_desc(metric, metric_params),
+                                          wrapped, out_c.ptr, out_s.ptr, out_r.ptr, out_nh.ptr)
         stream.synchronize()
         return Result(out_c.download(stream), out_s.download(stream), out_r.download(stream),
                       out_nh.download(stream))

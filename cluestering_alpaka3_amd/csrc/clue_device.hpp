@@ -32,20 +32,49 @@ __device__ __forceinline__ int32_t f2i(float v) {
   return (v >= -2147483648.0f && v < 2147483648.0f) ? static_cast<int32_t>(v) : INT32_MIN;
 }
 
-// Grid bounds per dimension (CoordinateExtremes + tile sizes,
-// data_structures/internal/CoordinateExtremes.hpp:9-25, Tiles::m_tilesizes).
-struct DevGrid {
+// One event of a batch: an independent clustering problem with its own tile
+// grid.  The host fills the shape (SetupTiles.hpp:23-26 evaluated with the
+// host's double pow, as the reference does); the bounding-box kernel fills the
+// extremes and tile sizes (CoordinateExtremes, ComputeTiles.hpp:21-26).
+struct EventDesc {
+  int32_t begin;        // first point of the event in the batch arrays
+  int32_t n;            // points
+  int32_t npd;          // tiles per dimension
+  int32_t tile_base;    // first global tile id of the event
+  int32_t ntiles;       // npd^D
+  int32_t chunk_begin;  // chunks of the event (bounding-box partials)
+  int32_t chunk_end;
+  int32_t pad;
+  int32_t stride[kMaxDim];  // npd^(D-1-d): integer form of powf(npd, D-d-1), TilesView.hpp:56,67
   float mn[kMaxDim];
   float mx[kMaxDim];
   float ts[kMaxDim];
 };
 
-// Host-known grid shape, passed by value to every kernel.
-struct GridShape {
+// At most kChunk consecutive points of one event: the unit of the per-point
+// kernels that need their event's grid uniformly.
+struct Chunk {
+  int32_t event;
+  int32_t begin;
+  int32_t end;
+  int32_t pad;
+};
+
+// Wrapped (periodic) coordinates, Clusterer::setWrappedCoordinates.
+struct Wrap {
+  uint8_t w[kMaxDim];
+};
+
+// The grid of one event as the kernels use it (wave-uniform values).
+template <int D>
+struct EvGrid {
   int32_t npd;
+  int32_t tile_base;
   int32_t ntiles;
-  int32_t stride[kMaxDim];  // npd^(D-1-d): integer form of powf(npd, D-d-1), TilesView.hpp:56,67
-  uint8_t wrapped[kMaxDim];
+  int32_t begin;
+  int32_t stride[D];
+  float mn[D], mx[D], ts[D];
+  bool wrapped[D];
 };
 
 // TilesView::normalizeCoordinate (TilesView.hpp:93-101).

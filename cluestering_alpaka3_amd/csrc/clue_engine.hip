@@ -147,16 +147,19 @@ __global__ __launch_bounds__(kThreads) void k_scan_down(const int32_t* in, int64
 }
 
 // ===========================================================================
-// Bounding box (per-dimension min / max), then tile sizes.
+// Bounding box of every event (per-dimension min / max), then its tile sizes.
 // ===========================================================================
-__global__ __launch_bounds__(kThreads) void k_bbox_partial(Coords c, int32_t n, int D,
+constexpr int kChunk = 4096;  // points per chunk
+
+__global__ __launch_bounds__(kThreads) void k_bbox_partial(Coords c, const Chunk* chunks, int D,
                                                             float* part) {
   __shared__ float slo[kThreads / kWave], shi[kThreads / kWave];
   const int lane = __lane_id();
   const int w = threadIdx.x / kWave;
+  const Chunk ch = chunks[blockIdx.x];
   for (int d = 0; d < D; ++d) {
     float lo = __builtin_inff(), hi = -__builtin_inff();
-    for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    for (int i = ch.begin + threadIdx.x; i < ch.end; i += kThreads) {
       const float v = c.p[d][i];
       lo = v < lo ? v : lo;
       hi = hi < v ? v : hi;
@@ -185,48 +188,81 @@ __global__ __launch_bounds__(kThreads) void k_bbox_partial(Coords c, int32_t n, 
 }
 
 // ComputeTiles.hpp:21-26: tile size = (max - min) / nPerDim, one float division.
-__global__ void k_bbox_final(const float* part, int nparts, int D, int32_t npd, DevGrid* g) {
-  const int d = threadIdx.x;
-  if (d >= D) return;
+// One wave per event and dimension.
+__global__ void k_bbox_final(const float* part, int D, int n_events, EventDesc* ev) {
+  const int e = blockIdx.x;
+  const int d = threadIdx.x / kWave;
+  const int lane = __lane_id();
+  if (e >= n_events || d >= D) return;
   float lo = __builtin_inff(), hi = -__builtin_inff();
-  for (int b = 0; b < nparts; ++b) {
+  for (int b = ev[e].chunk_begin + lane; b < ev[e].chunk_end; b += kWave) {
     const float plo = part[(b * D + d) * 2 + 0], phi = part[(b * D + d) * 2 + 1];
     lo = plo < lo ? plo : lo;
     hi = hi < phi ? phi : hi;
   }
-  g->mn[d] = lo;
-  g->mx[d] = hi;
-  g->ts[d] = (hi - lo) / static_cast<float>(npd);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const float olo = __shfl_xor(lo, off), ohi = __shfl_xor(hi, off);
+    lo = olo < lo ? olo : lo;
+    hi = hi < ohi ? ohi : hi;
+  }
+  if (lane == 0) {
+    ev[e].mn[d] = lo;
+    ev[e].mx[d] = hi;
+    ev[e].ts[d] = (hi - lo) / static_cast<float>(ev[e].npd);
+  }
+}
+
+template <int D>
+__device__ __forceinline__ EvGrid<D> load_event(const EventDesc* events, int e, const Wrap& wrap) {
+  cptr<EventDesc> ev = as_const(events) + e;
+  EvGrid<D> g;
+  g.npd = ev->npd;
+  g.tile_base = ev->tile_base;
+  g.ntiles = ev->ntiles;
+  g.begin = ev->begin;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    g.stride[d] = ev->stride[d];
+    g.mn[d] = ev->mn[d];
+    g.mx[d] = ev->mx[d];
+    g.ts[d] = ev->ts[d];
+    g.wrapped[d] = wrap.w[d] != 0;
+  }
+  return g;
 }
 
 // ===========================================================================
 // Tile id per point + histogram (K1 + K2 fused).
 // ===========================================================================
 template <int D>
-__global__ __launch_bounds__(kThreads) void k_bin(Coords c, int32_t n, const DevGrid* grid,
-                                                   GridShape gs, int32_t* key, int32_t* count,
-                                                   int32_t* key_nh, int sub_bits) {
-  const int i = blockIdx.x * kThreads + threadIdx.x;
-  if (i >= n) return;
-  cptr<DevGrid> g = as_const(grid);
-  int32_t gb = 0;
-  uint32_t code = 0;
+__global__ __launch_bounds__(kThreads) void k_bin(Coords c, const Chunk* chunks,
+                                                   const EventDesc* events, Wrap wrap, int32_t* key,
+                                                   int32_t* count, int32_t* key_nh, int sub_bits,
+                                                   int32_t* pev) {
+  const Chunk ch = chunks[blockIdx.x];
+  const EvGrid<D> g = load_event<D>(events, ch.event, wrap);
+  for (int i = ch.begin + threadIdx.x; i < ch.end; i += kThreads) {
+    int32_t gb = g.tile_base;
+    uint32_t code = 0;
 #pragma unroll
-  for (int d = 0; d < D; ++d) {
-    const float x = c.p[d][i];
-    const int32_t b = tile_bin(x, g->mn[d], g->mx[d], g->ts[d], gs.npd, gs.wrapped[d] != 0);
-    gb += gs.stride[d] * b;
-    if (key_nh) {
-      // position inside the tile on a 2^sub_bits grid, interleaved into a Morton code
-      const float f = (x - g->mn[d]) / g->ts[d] - static_cast<float>(b);
-      int32_t sub = f2i(f * static_cast<float>(1 << sub_bits));
-      sub = sub < 0 ? 0 : (sub > (1 << sub_bits) - 1 ? (1 << sub_bits) - 1 : sub);
-      for (int bit = 0; bit < sub_bits; ++bit) code |= ((sub >> bit) & 1u) << (bit * D + d);
+    for (int d = 0; d < D; ++d) {
+      const float x = c.p[d][i];
+      const int32_t b = tile_bin(x, g.mn[d], g.mx[d], g.ts[d], g.npd, g.wrapped[d]);
+      gb += g.stride[d] * b;
+      if (key_nh) {
+        // position inside the tile on a 2^sub_bits grid, interleaved into a Morton code
+        const float f = (x - g.mn[d]) / g.ts[d] - static_cast<float>(b);
+        int32_t sub = f2i(f * static_cast<float>(1 << sub_bits));
+        sub = sub < 0 ? 0 : (sub > (1 << sub_bits) - 1 ? (1 << sub_bits) - 1 : sub);
+        for (int bit = 0; bit < sub_bits; ++bit) code |= ((sub >> bit) & 1u) << (bit * D + d);
+      }
     }
+    key[i] = gb;
+    pev[i] = ch.event;
+    atomicAdd(&count[gb], 1);
+    if (key_nh) key_nh[i] = static_cast<int32_t>((static_cast<uint32_t>(gb) << (sub_bits * D)) | code);
   }
-  key[i] = gb;
-  atomicAdd(&count[gb], 1);
-  if (key_nh) key_nh[i] = static_cast<int32_t>((static_cast<uint32_t>(gb) << (sub_bits * D)) | code);
 }
 
 // ===========================================================================
@@ -412,9 +448,11 @@ struct PassArgs {
   const float* hw;           // tile-ordered h*w (flat kernel)
   const float* rho_s;        // tile-ordered densities (nearest-higher input)
   const int32_t* perm;       // slot -> point id
-  const int32_t* offsets;    // tile CSR offsets
-  const DevGrid* grid;
-  GridShape gs;
+  const int32_t* offsets;    // tile CSR offsets (global tile ids)
+  const EventDesc* ev;       // per-event grids
+  const int32_t* pev;        // point id -> event
+  int32_t* sc;               // per-event count of points without a nearest-higher
+  Wrap wrap;
   int32_t n;
   float radius;              // dc (density) or dm (nearest-higher)
   float thr;                 // max{s : sqrtf(s) <= radius}
@@ -444,43 +482,43 @@ struct PassArgs {
   float* rho_out;            // density: original order
   int32_t* nh_out;           // nearest-higher: original order
   float* delta_out;          // nearest-higher distance: original order
-  int32_t* seed_candidates;  // # points with nh == -1
 };
-
-template <int D>
-__device__ __forceinline__ void load_grid(const DevGrid* grid, float* mn, float* mx, float* ts) {
-  cptr<DevGrid> g = as_const(grid);
-#pragma unroll
-  for (int d = 0; d < D; ++d) {
-    mn[d] = g->mn[d];
-    mx[d] = g->mx[d];
-    ts[d] = g->ts[d];
-  }
-}
 
 // Search box of a point: [bin(x - r), bin(x + r)] per dimension, wrapped
 // dimensions unrolled past npd (TilesView::searchBox, TilesView.hpp:72-82).
 template <int D>
-__device__ __forceinline__ void search_box(const float* q, float r, const float* mn,
-                                           const float* mx, const float* ts, const GridShape& gs,
-                                           int* lo, int* hi) {
+__device__ __forceinline__ void search_box(const float* q, float r, const EvGrid<D>& g, int* lo,
+                                           int* hi) {
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    const bool wr = gs.wrapped[d] != 0;
-    int inf = tile_bin(q[d] - r, mn[d], mx[d], ts[d], gs.npd, wr);
-    int sup = tile_bin(q[d] + r, mn[d], mx[d], ts[d], gs.npd, wr);
-    if (wr && inf > sup) sup += gs.npd;
+    int inf = tile_bin(q[d] - r, g.mn[d], g.mx[d], g.ts[d], g.npd, g.wrapped[d]);
+    int sup = tile_bin(q[d] + r, g.mn[d], g.mx[d], g.ts[d], g.npd, g.wrapped[d]);
+    if (g.wrapped[d] && inf > sup) sup += g.npd;
     lo[d] = inf;
     hi[d] = sup;
   }
 }
 
+// Global tile id of a box position (TilesView::getGlobalBinByBin, TilesView.hpp:63-70).
 template <int D>
-__device__ __forceinline__ int tile_index(const int* b, const GridShape& gs) {
-  int g = 0;
+__device__ __forceinline__ int tile_index(const int* b, const EvGrid<D>& g) {
+  int t = g.tile_base;
 #pragma unroll
-  for (int d = 0; d < D; ++d) g += gs.stride[d] * (gs.wrapped[d] ? b[d] % gs.npd : b[d]);
-  return g;
+  for (int d = 0; d < D; ++d) t += g.stride[d] * (g.wrapped[d] ? b[d] % g.npd : b[d]);
+  return t;
+}
+
+// Runs body(e, act) once per event present among the wave's valid lanes, with
+// act true on the lanes of that event (wave-uniform e).
+template <typename F>
+__device__ __forceinline__ void for_each_event(bool valid, int my_e, F&& body) {
+  uint64_t todo = __ballot(valid);
+  while (todo) {
+    const int e = __builtin_amdgcn_readlane(my_e, __builtin_ctzll(todo));
+    const bool act = valid && my_e == e;
+    todo &= ~__ballot(act);
+    body(e, act);
+  }
 }
 
 // Visitor over the tiles a lane needs, in the reference order (lexicographic,
@@ -491,7 +529,7 @@ __device__ __forceinline__ int tile_index(const int* b, const GridShape& gs) {
 // Otherwise each lane walks its own box.
 template <int D, bool UNION, typename F>
 __device__ __forceinline__ void visit_box(bool valid, const int* lo, const int* hi,
-                                          const GridShape& gs, cptr<int32_t> offsets, F&& body) {
+                                          const EvGrid<D>& gs, cptr<int32_t> offsets, F&& body) {
   if constexpr (UNION) {
     int ulo[D], uhi[D];
 #pragma unroll
@@ -566,21 +604,21 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
   float q[D];
   float w_self;  // the point's own term is 1 * w_i (kernel value 1 for j == i)
   int s;         // the query's slot in the tile-ordered candidate layout
+  int pid;       // the query's point id
   if constexpr (QNH) {
 #pragma unroll
     for (int d = 0; d < D; ++d) q[d] = a.xn[d][pq];
     w_self = a.wn[pq];
     s = valid ? a.sn[pq] : -1;
+    pid = a.pn[pq];
   } else {
 #pragma unroll
     for (int d = 0; d < D; ++d) q[d] = a.xs[d][pq];
     w_self = a.ws[pq];
     s = valid ? pq : -1;
+    pid = a.perm[pq];
   }
-  float mn[D], mx[D], ts[D];
-  load_grid<D>(a.grid, mn, mx, ts);
-  int lo[D], hi[D];
-  search_box<D>(q, a.radius, mn, mx, ts, a.gs, lo, hi);
+  const int my_e = a.pev[pid];
   cptr<float> X[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) X[d] = as_const(a.xs[d]);
@@ -708,20 +746,25 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
       }
     }
   };
-  visit_box<D, UNION>(valid, lo, hi, a.gs, as_const(a.offsets), [&](int t, int beg, int end, bool need) {
-    // Only the tile holding a lane's own point needs the j == i test.
-    const bool self = !UNION || __ballot(s >= beg && s < end) != 0;
-    if constexpr (kMasked) {
-      if (a.mask_min_points > 0 && end - beg >= a.mask_min_points && end - beg <= kMaskWords * 32) {
-        if (self) run_masked(std::true_type{}, t, beg, end, need);
-        else run_masked(std::false_type{}, t, beg, end, need);
-        return;
+  for_each_event(valid, my_e, [&](int e, bool act) {
+    const EvGrid<D> g = load_event<D>(a.ev, e, a.wrap);
+    int lo[D], hi[D];
+    search_box<D>(q, a.radius, g, lo, hi);
+    visit_box<D, UNION>(act, lo, hi, g, as_const(a.offsets), [&](int t, int beg, int end, bool need) {
+      // Only the tile holding a lane's own point needs the j == i test.
+      const bool self = !UNION || __ballot(s >= beg && s < end) != 0;
+      if constexpr (kMasked) {
+        if (a.mask_min_points > 0 && end - beg >= a.mask_min_points && end - beg <= kMaskWords * 32) {
+          if (self) run_masked(std::true_type{}, t, beg, end, need);
+          else run_masked(std::false_type{}, t, beg, end, need);
+          return;
+        }
       }
-    }
-    if (need) {
-      if (self) run(std::true_type{}, beg, end);
-      else run(std::false_type{}, beg, end);
-    }
+      if (need) {
+        if (self) run(std::true_type{}, beg, end);
+        else run(std::false_type{}, beg, end);
+      }
+    });
   });
   if (valid) {
     a.rho_s_out[s] = rho;
@@ -744,10 +787,7 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher(const PassArgs a) {
   for (int d = 0; d < D; ++d) q[d] = a.xs[d][sq];
   const float rho_i = a.rho_s[sq];
   const int32_t id_i = a.perm[sq];
-  float mn[D], mx[D], ts[D];
-  load_grid<D>(a.grid, mn, mx, ts);
-  int lo[D], hi[D];
-  search_box<D>(q, a.radius, mn, mx, ts, a.gs, lo, hi);
+  const int my_e = a.pev[id_i];
   cptr<float> X[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) X[d] = as_const(a.xs[d]);
@@ -779,7 +819,11 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher(const PassArgs a) {
       }
     }
   };
-  visit_box<D, UNION>(valid, lo, hi, a.gs, as_const(a.offsets), [&](int, int beg, int end, bool need) {
+  for_each_event(valid, my_e, [&](int e, bool act) {
+  const EvGrid<D> g = load_event<D>(a.ev, e, a.wrap);
+  int lo[D], hi[D];
+  search_box<D>(q, a.radius, g, lo, hi);
+  visit_box<D, UNION>(act, lo, hi, g, as_const(a.offsets), [&](int, int beg, int end, bool need) {
     if (!need) return;
     int j = beg;
     if constexpr (UNION) {
@@ -804,12 +848,12 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher(const PassArgs a) {
       eval(c, R[j], P[j]);
     }
   });
+  });
   if (valid) {
     a.nh_out[id_i] = nh;
     a.delta_out[id_i] = delta;
+    if (nh == -1) atomicAdd(&a.sc[my_e], 1);
   }
-  const uint64_t none = __ballot(valid && nh == -1);
-  if (__lane_id() == 0 && none) atomicAdd(a.seed_candidates, static_cast<int>(__popcll(none)));
 }
 
 // Squared distance from q to the nearest point of a box, with the fold order of
@@ -877,10 +921,7 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
   for (int d = 0; d < D; ++d) q[d] = a.xn[d][pos];
   const float rho_i = a.rn[pos];
   const int32_t id_i = a.pn[pos];
-  float mn[D], mx[D], ts[D];
-  load_grid<D>(a.grid, mn, mx, ts);
-  int lo[D], hi[D];
-  search_box<D>(q, a.radius, mn, mx, ts, a.gs, lo, hi);
+  const int my_e = a.pev[id_i];
   cptr<float> X[D], TMIN[D], TMAX[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) {
@@ -906,15 +947,6 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
     qhi[d] = wave_max_f32(valid ? q[d] : -__builtin_inff());
   }
 
-  auto in_box = [&](int t) {
-    bool in = valid;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      const int b = (t / a.gs.stride[d]) % a.gs.npd;
-      in = in && b >= lo[d] && b <= hi[d];
-    }
-    return in;
-  };
   auto scan = [&](int t, int beg, int end) {
     for (int j = beg; j < end; ++j) {
       float c[D];
@@ -940,37 +972,52 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
     }
   };
 
-  if constexpr (PHASE == 1) {
-    // the wave's own points: same tile, neighbouring Morton codes
-    const int base = uniform(gid & ~(kWave - 1));
-    if (base < a.n) {
-      const int t0 = uniform(as_const(a.kn)[base] >> a.mshift);
-      const int beg = uniform(max(base, OFF[t0]));
-      const int end = uniform(min(min(base + kWave, OFF[t0 + 1]), a.n));
-      const bool need0 = in_box(t0);
-      if (__ballot(need0) != 0 && need0) scan(t0, beg, end);
-    }
-  }
-
-  int ulo[D], uhi[D];
-  bool any = true;
+  for_each_event(valid, my_e, [&](int e, bool act) {
+    const EvGrid<D> g = load_event<D>(a.ev, e, a.wrap);
+    int lo[D], hi[D];
+    search_box<D>(q, a.radius, g, lo, hi);
+    auto in_box = [&](int t) {
+      const int tl = t - g.tile_base;
+      bool in = act && tl >= 0 && tl < g.ntiles;
 #pragma unroll
-  for (int d = 0; d < D; ++d) {
-    ulo[d] = uniform(wave_min_i32(valid ? lo[d] : INT_MAX));
-    uhi[d] = uniform(wave_max_i32(valid ? hi[d] : INT_MIN));
-    any = any && ulo[d] <= uhi[d];
-  }
-  if (any) {
+      for (int d = 0; d < D; ++d) {
+        const int b = (tl / g.stride[d]) % g.npd;
+        in = in && b >= lo[d] && b <= hi[d];
+      }
+      return in;
+    };
+
+    if constexpr (PHASE == 1) {
+      // the wave's own points: same tile, neighbouring Morton codes
+      const int base = uniform(gid & ~(kWave - 1));
+      if (base < a.n) {
+        const int t0 = uniform(as_const(a.kn)[base] >> a.mshift);
+        const int beg = uniform(max(base, OFF[t0]));
+        const int end = uniform(min(min(base + kWave, OFF[t0 + 1]), a.n));
+        const bool need0 = in_box(t0);
+        if (__ballot(need0) != 0 && need0) scan(t0, beg, end);
+      }
+    }
+
+    int ulo[D], uhi[D];
+    bool any = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      ulo[d] = uniform(wave_min_i32(act ? lo[d] : INT_MAX));
+      uhi[d] = uniform(wave_max_i32(act ? hi[d] : INT_MIN));
+      any = any && ulo[d] <= uhi[d];
+    }
+    if (!any) return;
     int b[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) b[d] = ulo[d];
     const int lane = __lane_id();
     while (true) {
-      bool need = valid;
+      bool need = act;
 #pragma unroll
       for (int d = 0; d < D; ++d) need = need && b[d] >= lo[d] && b[d] <= hi[d];
       if (__ballot(need) != 0) {
-        const int t = uniform(tile_index<D>(b, a.gs));
+        const int t = uniform(tile_index<D>(b, g));
         float tl[D], th[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
@@ -995,11 +1042,11 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
             float lbw = 0.f;
 #pragma unroll
             for (int d = D - 1; d >= 0; --d) {
-              float g = bl[d] - qhi[d];
+              float gg = bl[d] - qhi[d];
               const float g2 = qlo[d] - bh[d];
-              g = g2 > g ? g2 : g;
-              g = g > 0.f ? g : 0.f;
-              const float tt = g * g;
+              gg = g2 > gg ? g2 : gg;
+              gg = gg > 0.f ? gg : 0.f;
+              const float tt = gg * gg;
               lbw = (d == D - 1) ? tt : tt + lbw;
             }
             uint64_t m = __ballot(inb && lbw <= maxb);
@@ -1029,7 +1076,7 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
       }
       if (d < 0) break;
     }
-  }
+  });
 
   if constexpr (PHASE == 1) {
     // unresolved positions are compacted in Morton order (flag -> scan -> scatter)
@@ -1043,9 +1090,8 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
     if (valid) {
       a.nh_out[id_i] = nh;
       a.delta_out[id_i] = delta;
+      if (nh == -1) atomicAdd(&a.sc[my_e], 1);
     }
-    const uint64_t none = __ballot(valid && nh == -1);
-    if (__lane_id() == 0 && none) atomicAdd(a.seed_candidates, static_cast<int>(__popcll(none)));
   }
 }
 
@@ -1099,27 +1145,46 @@ __global__ __launch_bounds__(kThreads) void k_classify(const ClassifyArgs a) {
   a.root[i] = parent == -1 ? i : parent;
 }
 
-// counters: [0] seed candidates, [1] grow-only capacity carried across calls,
-// [2] capacity of this call, [3] seeds, [4] stored seeds (clusters), [5] dropped.
-__global__ void k_capacity(int32_t* counters, const int32_t* n_seeds_total, int unbounded) {
-  if (threadIdx.x != 0) return;
-  const int32_t sc = counters[0];
-  const int32_t cap = counters[1] > sc ? counters[1] : sc;  // SetupSeeds.hpp:14-18
-  counters[1] = cap;
-  const int32_t ns = *n_seeds_total;
+// Per event: seeds, capacity and the base of its seed ranks.  counters (totals
+// over the events): [0] seed candidates, [1] grow-only capacity carried across
+// single-event calls, [2] capacity used, [3] seeds, [4] stored seeds
+// (clusters), [5] dropped seeds.
+__global__ void k_capacity(const EventDesc* ev, int n_events, const int32_t* sc,
+                           const int32_t* rank, int32_t* counters, int32_t* cap_used,
+                           int32_t* rank_base, int unbounded, int persistent) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_events) return;
+  const int b = ev[e].begin, n = ev[e].n;
+  const int32_t cand = sc[e];
+  int32_t cap = cand;  // SetupSeeds.hpp:14-18: a fresh seed array holds #(nh == -1)
+  if (persistent) {  // a Clusterer keeps the largest capacity it has seen (one event)
+    cap = counters[1] > cand ? counters[1] : cand;
+    counters[1] = cap;
+  }
+  const int32_t ns = rank[b + n] - rank[b];
   const int32_t used = unbounded ? ns : cap;
-  counters[2] = used;
-  counters[3] = ns;
-  counters[4] = ns < used ? ns : used;
-  counters[5] = ns - counters[4];
+  const int32_t stored = ns < used ? ns : used;
+  cap_used[e] = used;
+  rank_base[e] = rank[b];
+  atomicAdd(&counters[0], cand);
+  atomicAdd(&counters[2], used);
+  atomicAdd(&counters[3], ns);
+  atomicAdd(&counters[4], stored);
+  atomicAdd(&counters[5], ns - stored);
 }
 
 // Pointer jumping: every hop follows the current parent array, which other
 // threads only ever shorten; `hops` per round bounds the work of a round.
+// The final round writes the label -- the position of the root in its event's
+// serial seed array, -1 if the root is not a stored seed (KernelAssignClusters,
+// ClusteringKernels.hpp:252-279) -- and turns the nearest-higher into an index
+// local to the event.
 __global__ __launch_bounds__(kThreads) void k_labels(int32_t* root, int32_t n, int hops,
                                                       const int32_t* is_seed, const int32_t* rank,
-                                                      const int32_t* counters, int32_t* cluster,
-                                                      int final_round) {
+                                                      const int32_t* pev, const EventDesc* ev,
+                                                      const int32_t* cap_used,
+                                                      const int32_t* rank_base, int32_t* cluster,
+                                                      int32_t* nh_out, int final_round) {
   const int i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
   int32_t r = root[i];
@@ -1130,10 +1195,11 @@ __global__ __launch_bounds__(kThreads) void k_labels(int32_t* root, int32_t n, i
   }
   root[i] = r;
   if (final_round) {
-    // label = position of the root in the serial seed array, -1 if the root is
-    // not a stored seed (KernelAssignClusters, ClusteringKernels.hpp:252-279)
-    const int32_t cap = counters[2];
-    cluster[i] = (is_seed[r] && rank[r] < cap) ? rank[r] : -1;
+    const int e = pev[i];
+    const int32_t local = rank[r] - rank_base[e];
+    cluster[i] = (is_seed[r] && local < cap_used[e]) ? local : -1;
+    const int32_t nhv = nh_out[i];
+    if (nhv >= 0) nh_out[i] = nhv - ev[e].begin;
   }
 }
 
@@ -1215,21 +1281,37 @@ using namespace clue_hip;
 struct clue_plan_s {
   int device = 0;
   DevBuf key, key2, perm, perm2, count, offsets, hist, part, xs, ws, hw, rho_s, delta, flag, root,
-      nh_tmp, rho_tmp, seed_tmp, bbox_part, grid, counters;
+      nh_tmp, rho_tmp, seed_tmp, bbox_part, counters;
+  // events of the call: descriptors, chunks, point -> event, per-event counters
+  DevBuf events, chunks, pev, sc, cap_used, rank_base;
   // Morton-ordered layout and the block boxes of the nearest-higher search
   DevBuf kn, kn2, pn, pn2, inv, xn, wn, sn, rn, bcnt, blk_off, bbox_lo, bbox_hi, tbox_lo, tbox_hi;
   DevBuf pending, pending_flag;
   clue_tuning tuning{0.25f, 1, 128, -1};
   int64_t ldb = 0;  // leading dimension of the block boxes
-  int64_t ld = 0;  // leading dimension of xs
+  int64_t ld = 0;   // leading dimension of xs / xn
+  // pinned staging of the event and chunk tables (reused once its copy is done)
+  void* stage = nullptr;
+  size_t stage_bytes = 0;
+  hipEvent_t stage_done = nullptr;
   // last call
   bool has_run = false;
   hipStream_t last_stream = nullptr;
-  int32_t last_n = 0, last_d = 0, last_npd = 0, last_T = 0;
+  int32_t last_n = 0, last_d = 0, last_T = 0, last_events = 0;
   int32_t* last_perm = nullptr;
   // optional per-stage HIP event timing of the last call
   bool timing = false;
   hipEvent_t ev[kStages + 1] = {};
+
+  DevBuf* all_buffers(int k) {
+    DevBuf* b[] = {&key,     &key2,   &perm,     &perm2,   &count,   &offsets,  &hist,    &part,
+                   &xs,      &ws,     &hw,       &rho_s,   &delta,   &flag,     &root,    &nh_tmp,
+                   &rho_tmp, &seed_tmp, &bbox_part, &counters, &events, &chunks, &pev,    &sc,
+                   &cap_used, &rank_base, &kn,   &kn2,     &pn,      &pn2,      &inv,     &xn,
+                   &wn,      &sn,     &rn,       &bcnt,    &blk_off, &bbox_lo,  &bbox_hi, &tbox_lo,
+                   &tbox_hi, &pending, &pending_flag};
+    return k < static_cast<int>(sizeof(b) / sizeof(b[0])) ? b[k] : nullptr;
+  }
 };
 
 namespace clue_hip {
@@ -1253,19 +1335,22 @@ float sqrt_le_bound(float r) {
   return t;
 }
 
-clue_status reserve(clue_plan_s* p, int32_t D, int32_t n, int32_t T) {
+clue_status reserve(clue_plan_s* p, int32_t D, int32_t n, int32_t T, int32_t n_events,
+                    int32_t n_chunks) {
   CLUE_HIP(hipSetDevice(p->device));
   const size_t N = static_cast<size_t>(n);
   const size_t nb = static_cast<size_t>(div_up(n, kSortTile));
   const size_t hist_entries = 256 * nb + 1;
   const size_t scan_max = std::max<size_t>({N + 1, hist_entries, static_cast<size_t>(T) + 1});
   const int64_t ld = div_up(n, 64) * 64;
+  const size_t TT = static_cast<size_t>(T) + 1;
+  const size_t E = static_cast<size_t>(n_events);
   CLUE_HIP(p->key.ensure(N * 4));
   CLUE_HIP(p->key2.ensure(N * 4));
   CLUE_HIP(p->perm.ensure(N * 4));
   CLUE_HIP(p->perm2.ensure(N * 4));
-  CLUE_HIP(p->count.ensure((static_cast<size_t>(T) + 1) * 4));
-  CLUE_HIP(p->offsets.ensure((static_cast<size_t>(T) + 1) * 4));
+  CLUE_HIP(p->count.ensure(TT * 4));
+  CLUE_HIP(p->offsets.ensure(TT * 4));
   CLUE_HIP(p->hist.ensure(hist_entries * 4));
   CLUE_HIP(p->part.ensure((div_up(scan_max, kScanTile) + 1) * 4));
   CLUE_HIP(p->xs.ensure(static_cast<size_t>(ld) * D * 4));
@@ -1279,7 +1364,17 @@ clue_status reserve(clue_plan_s* p, int32_t D, int32_t n, int32_t T) {
   CLUE_HIP(p->nh_tmp.ensure(N * 4));
   CLUE_HIP(p->rho_tmp.ensure(N * 4));
   CLUE_HIP(p->seed_tmp.ensure(N * 4));
-  CLUE_HIP(p->bbox_part.ensure(kBBoxBlocks * kMaxDim * 2 * 4));
+  CLUE_HIP(p->bbox_part.ensure(static_cast<size_t>(n_chunks) * D * 2 * 4));
+  CLUE_HIP(p->events.ensure(E * sizeof(EventDesc)));
+  CLUE_HIP(p->chunks.ensure(static_cast<size_t>(n_chunks) * sizeof(Chunk)));
+  CLUE_HIP(p->pev.ensure(N * 4));
+  CLUE_HIP(p->sc.ensure(E * 4));
+  CLUE_HIP(p->cap_used.ensure(E * 4));
+  CLUE_HIP(p->rank_base.ensure(E * 4));
+  if (!p->counters.p) {
+    CLUE_HIP(p->counters.ensure(16 * 4));
+    CLUE_HIP(hipMemset(p->counters.p, 0, 16 * 4));
+  }
   CLUE_HIP(p->kn.ensure(N * 4));
   CLUE_HIP(p->kn2.ensure(N * 4));
   CLUE_HIP(p->pn.ensure(N * 4));
@@ -1289,8 +1384,8 @@ clue_status reserve(clue_plan_s* p, int32_t D, int32_t n, int32_t T) {
   CLUE_HIP(p->wn.ensure(N * 4));
   CLUE_HIP(p->sn.ensure(N * 4));
   CLUE_HIP(p->rn.ensure(N * 4));
-  CLUE_HIP(p->bcnt.ensure((static_cast<size_t>(T) + 1) * 4));
-  CLUE_HIP(p->blk_off.ensure((static_cast<size_t>(T) + 1) * 4));
+  CLUE_HIP(p->bcnt.ensure(TT * 4));
+  CLUE_HIP(p->blk_off.ensure(TT * 4));
   const int64_t ldb = div_up(n, kBlock) + T + 1;
   CLUE_HIP(p->bbox_lo.ensure(static_cast<size_t>(ldb) * D * 4));
   CLUE_HIP(p->bbox_hi.ensure(static_cast<size_t>(ldb) * D * 4));
@@ -1299,11 +1394,6 @@ clue_status reserve(clue_plan_s* p, int32_t D, int32_t n, int32_t T) {
   CLUE_HIP(p->tbox_hi.ensure(static_cast<size_t>(T) * D * 4));
   CLUE_HIP(p->pending.ensure((N + 64) * 4));
   CLUE_HIP(p->pending_flag.ensure((N + 1) * 4));
-  if (!p->grid.p) CLUE_HIP(p->grid.ensure(sizeof(DevGrid)));
-  if (!p->counters.p) {
-    CLUE_HIP(p->counters.ensure(16 * 4));
-    CLUE_HIP(hipMemset(p->counters.p, 0, 16 * 4));
-  }
   return CLUE_OK;
 }
 
@@ -1325,11 +1415,12 @@ clue_status scan_exclusive(const int32_t* in, int32_t* out, int64_t n, int32_t* 
 }
 
 template <int D>
-clue_status launch_bin(const Coords& c, int32_t n, clue_plan_s* p, const GridShape& gs,
+clue_status launch_bin(const Coords& c, int32_t n_chunks, clue_plan_s* p, const Wrap& wrap,
                        int sub_bits, hipStream_t st) {
-  k_bin<D><<<div_up(n, kThreads), kThreads, 0, st>>>(
-      c, n, p->grid.as<DevGrid>(), gs, p->key.as<int32_t>(), p->count.as<int32_t>(),
-      sub_bits >= 0 ? p->kn.as<int32_t>() : nullptr, sub_bits < 0 ? 0 : sub_bits);
+  k_bin<D><<<n_chunks, kThreads, 0, st>>>(
+      c, p->chunks.as<Chunk>(), p->events.as<EventDesc>(), wrap, p->key.as<int32_t>(),
+      p->count.as<int32_t>(), sub_bits >= 0 ? p->kn.as<int32_t>() : nullptr, sub_bits < 0 ? 0 : sub_bits,
+      p->pev.as<int32_t>());
   CLUE_LAUNCH_CHECK("bin");
   return CLUE_OK;
 }
@@ -1520,65 +1611,120 @@ clue_status mark(clue_plan_s* p, int k, hipStream_t st) {
     if (m_ != CLUE_OK) return m_;          \
   } while (0)
 
-clue_status make_clusters(clue_plan_s* p, hipStream_t st, const clue_params* prm, int32_t D,
-                          int32_t n, const float* const* coords, const float* weights,
+// One call over a batch of independent events (a plain call is a batch of one):
+// events[e] = points [event_offsets[e], event_offsets[e+1]) of the arrays.
+clue_status cluster_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm, int32_t D,
+                          int32_t n_events, const int64_t* event_offsets,
+                          const float* const* coords, const float* weights,
                           const clue_kernel_desc* kernel, const clue_metric_desc* metric,
                           const uint8_t* wrapped, int32_t* cluster_out, int32_t* is_seed_out,
-                          float* rho_out, int32_t* nh_out) {
+                          float* rho_out, int32_t* nh_out, bool persistent_capacity) {
+  if (n_events <= 0 || !event_offsets)
+    return fail(CLUE_ERR_INVALID_ARGUMENT, "the batch needs at least one event");
+  const int64_t n64 = event_offsets[n_events] - event_offsets[0];
+  if (event_offsets[0] != 0 || n64 > INT32_MAX)
+    return fail(CLUE_ERR_INVALID_ARGUMENT, "event offsets must start at 0 and stay below 2^31 points");
+  const int32_t n = static_cast<int32_t>(n64);
   clue_status rc = validate(prm, D, n, coords, weights, kernel, metric, cluster_out);
   if (rc != CLUE_OK) return rc;
   const clue_kernel_desc kd = kernel ? *kernel : clue_kernel_desc{CLUE_KERNEL_FLAT, {0.5f, 0.f, 0.f}};
   clue_metric_desc md{};
   if (metric) md = *metric;
-
-  int32_t npd, T;
-  grid_shape(D, n, prm->points_per_tile, npd, T);
-  if (npd <= 0 || T <= 0) return fail(CLUE_ERR_INVALID_ARGUMENT, "degenerate tile grid");
-  rc = reserve(p, D, n, T);
-  if (rc != CLUE_OK) return rc;
-
-  GridShape gs{};
-  gs.npd = npd;
-  gs.ntiles = T;
-  {
-    int32_t s = 1;
-    for (int d = D - 1; d >= 0; --d) {
-      gs.stride[d] = s;
-      s *= npd;
-    }
-    for (int d = 0; d < D; ++d) gs.wrapped[d] = wrapped ? (wrapped[d] ? 1 : 0) : 0;
+  Wrap wrap{};
+  bool wrapped_any = false;
+  for (int d = 0; d < D; ++d) {
+    wrap.w[d] = wrapped ? (wrapped[d] ? 1 : 0) : 0;
+    wrapped_any = wrapped_any || wrap.w[d];
   }
+
+  // ---- event and chunk tables (host), grid shapes with the reference's formulas
+  int64_t total_tiles = 0, n_chunks64 = 0;
+  for (int e = 0; e < n_events; ++e) {
+    const int64_t ne = event_offsets[e + 1] - event_offsets[e];
+    if (ne <= 0)
+      return fail(CLUE_ERR_INVALID_ARGUMENT,
+                  "Number of points passed to PointsDevice constructor must be positive.");
+    int32_t npd, T;
+    grid_shape(D, static_cast<int32_t>(ne), prm->points_per_tile, npd, T);
+    if (npd <= 0 || T <= 0) return fail(CLUE_ERR_INVALID_ARGUMENT, "degenerate tile grid");
+    total_tiles += T;
+    n_chunks64 += div_up(ne, kChunk);
+  }
+  if (total_tiles >= (int64_t{1} << 30))
+    return fail(CLUE_ERR_UNSUPPORTED, "more than 2^30 tiles in one call");
+  const int32_t Ttot = static_cast<int32_t>(total_tiles);
+  const int32_t n_chunks = static_cast<int32_t>(n_chunks64);
+  const size_t table_bytes = static_cast<size_t>(n_events) * sizeof(EventDesc) +
+                             static_cast<size_t>(n_chunks) * sizeof(Chunk);
+  if (p->stage_done) CLUE_HIP(hipEventSynchronize(p->stage_done));  // previous upload finished
+  if (p->stage_bytes < table_bytes) {
+    if (p->stage) CLUE_HIP(hipHostFree(p->stage));
+    p->stage = nullptr;
+    p->stage_bytes = 0;
+    CLUE_HIP(hipHostMalloc(&p->stage, table_bytes, hipHostMallocDefault));
+    p->stage_bytes = table_bytes;
+  }
+  if (!p->stage_done) CLUE_HIP(hipEventCreateWithFlags(&p->stage_done, hipEventDisableTiming));
+  auto* hev = static_cast<EventDesc*>(p->stage);
+  auto* hch = reinterpret_cast<Chunk*>(hev + n_events);
+  {
+    int32_t tile_base = 0, chunk = 0;
+    for (int e = 0; e < n_events; ++e) {
+      EventDesc& ed = hev[e];
+      std::memset(&ed, 0, sizeof(ed));
+      ed.begin = static_cast<int32_t>(event_offsets[e]);
+      ed.n = static_cast<int32_t>(event_offsets[e + 1] - event_offsets[e]);
+      grid_shape(D, ed.n, prm->points_per_tile, ed.npd, ed.ntiles);
+      ed.tile_base = tile_base;
+      tile_base += ed.ntiles;
+      int32_t s1 = 1;
+      for (int d = D - 1; d >= 0; --d) {
+        ed.stride[d] = s1;
+        s1 *= ed.npd;
+      }
+      ed.chunk_begin = chunk;
+      for (int32_t b = ed.begin; b < ed.begin + ed.n; b += kChunk)
+        hch[chunk++] = Chunk{e, b, std::min(b + kChunk, ed.begin + ed.n), 0};
+      ed.chunk_end = chunk;
+    }
+  }
+  rc = reserve(p, D, n, Ttot, n_events, n_chunks);
+  if (rc != CLUE_OK) return rc;
+  CLUE_HIP(hipMemcpyAsync(p->events.p, hev, static_cast<size_t>(n_events) * sizeof(EventDesc),
+                          hipMemcpyHostToDevice, st));
+  CLUE_HIP(hipMemcpyAsync(p->chunks.p, hch, static_cast<size_t>(n_chunks) * sizeof(Chunk),
+                          hipMemcpyHostToDevice, st));
+  CLUE_HIP(hipEventRecord(p->stage_done, st));
+
   Coords c{};
   for (int d = 0; d < D; ++d) c.p[d] = coords[d];
-
   float* rho_buf = rho_out ? rho_out : p->rho_tmp.as<float>();
   int32_t* nh_buf = nh_out ? nh_out : p->nh_tmp.as<int32_t>();
   int32_t* seed_buf = is_seed_out ? is_seed_out : p->seed_tmp.as<int32_t>();
+  int32_t* counters = p->counters.as<int32_t>();
 
-  // ---- bounding box and tile sizes
+  // ---- bounding boxes and tile sizes
   CLUE_MARK(0);
-  const int bb = static_cast<int>(std::min<int64_t>(kBBoxBlocks, div_up(n, kThreads)));
-  k_bbox_partial<<<bb, kThreads, 0, st>>>(c, n, D, p->bbox_part.as<float>());
+  k_bbox_partial<<<n_chunks, kThreads, 0, st>>>(c, p->chunks.as<Chunk>(), D, p->bbox_part.as<float>());
   CLUE_LAUNCH_CHECK("bbox_partial");
-  k_bbox_final<<<1, 64, 0, st>>>(p->bbox_part.as<float>(), bb, D, npd, p->grid.as<DevGrid>());
+  k_bbox_final<<<n_events, kWave * D, 0, st>>>(p->bbox_part.as<float>(), D, n_events,
+                                                p->events.as<EventDesc>());
   CLUE_LAUNCH_CHECK("bbox_final");
 
   // The block search of the nearest-higher pass needs the Euclidean metric on a
   // grid without wrapping; other cases run the tile walk for both passes.
-  bool wrapped_any = false;
-  for (int d = 0; d < D; ++d) wrapped_any = wrapped_any || gs.wrapped[d];
-  const int tbits = bits_for(T);
+  const int tbits = bits_for(Ttot);
   int sub_bits = std::min(D == 1 ? 10 : (D == 2 ? 5 : 4), (31 - tbits) / D);
   if (p->tuning.morton_bits >= 0) sub_bits = std::min(p->tuning.morton_bits, (31 - tbits) / D);
   const bool blocks = md.type == MEUCLID && !wrapped_any && D <= 3 && sub_bits >= 1;
   if (!blocks) sub_bits = -1;
 
-  // ---- tile ids + histogram (+ Morton keys)
+  // ---- tile ids + histogram (+ Morton keys, point -> event)
   CLUE_MARK(1);
-  k_fill_i32<<<div_up(T + 1, kThreads), kThreads, 0, st>>>(p->count.as<int32_t>(), 0, T + 1);
+  k_fill_i32<<<div_up(Ttot + 1, kThreads), kThreads, 0, st>>>(p->count.as<int32_t>(), 0, Ttot + 1);
   CLUE_LAUNCH_CHECK("fill");
   rc = dispatch_dim(D, [&](auto dc_) {
-    return launch_bin<decltype(dc_)::value>(c, n, p, gs, sub_bits, st);
+    return launch_bin<decltype(dc_)::value>(c, n_chunks, p, wrap, sub_bits, st);
   });
   if (rc != CLUE_OK) return rc;
 
@@ -1601,7 +1747,7 @@ clue_status make_clusters(clue_plan_s* p, hipStream_t st, const clue_params* prm
 
   // ---- tile offsets
   CLUE_MARK(3);
-  rc = scan_exclusive(p->count.as<int32_t>(), p->offsets.as<int32_t>(), T, p->part.as<int32_t>(), st);
+  rc = scan_exclusive(p->count.as<int32_t>(), p->offsets.as<int32_t>(), Ttot, p->part.as<int32_t>(), st);
   if (rc != CLUE_OK) return rc;
 
   // ---- tile-ordered SoA (+ Morton-ordered SoA and block boxes)
@@ -1617,22 +1763,23 @@ clue_status make_clusters(clue_plan_s* p, hipStream_t st, const clue_params* prm
                                                               p->ld, p->xn.as<float>(), p->wn.as<float>(),
                                                               p->sn.as<int32_t>());
     CLUE_LAUNCH_CHECK("gather_morton");
-    k_block_count<<<div_up(T, kThreads), kThreads, 0, st>>>(p->offsets.as<int32_t>(), T, p->bcnt.as<int32_t>());
+    k_block_count<<<div_up(Ttot, kThreads), kThreads, 0, st>>>(p->offsets.as<int32_t>(), Ttot,
+                                                               p->bcnt.as<int32_t>());
     CLUE_LAUNCH_CHECK("block_count");
-    rc = scan_exclusive(p->bcnt.as<int32_t>(), p->blk_off.as<int32_t>(), T, p->part.as<int32_t>(), st);
+    rc = scan_exclusive(p->bcnt.as<int32_t>(), p->blk_off.as<int32_t>(), Ttot, p->part.as<int32_t>(), st);
     if (rc != CLUE_OK) return rc;
-    k_block_boxes<<<div_up(div_up(n, kBlock) + T, kThreads), kThreads, 0, st>>>(
-        p->xn.as<float>(), p->ld, D, p->offsets.as<int32_t>(), p->blk_off.as<int32_t>(), T,
+    k_block_boxes<<<div_up(div_up(n, kBlock) + Ttot, kThreads), kThreads, 0, st>>>(
+        p->xn.as<float>(), p->ld, D, p->offsets.as<int32_t>(), p->blk_off.as<int32_t>(), Ttot,
         p->bbox_lo.as<float>(), p->bbox_hi.as<float>(), p->ldb);
     CLUE_LAUNCH_CHECK("block_boxes");
-    k_tile_boxes<<<div_up(T, kThreads), kThreads, 0, st>>>(p->bbox_lo.as<float>(), p->bbox_hi.as<float>(),
-                                                           p->ldb, D, p->blk_off.as<int32_t>(), T,
-                                                           p->tbox_lo.as<float>(), p->tbox_hi.as<float>());
+    k_tile_boxes<<<div_up(Ttot, kThreads), kThreads, 0, st>>>(p->bbox_lo.as<float>(), p->bbox_hi.as<float>(),
+                                                              p->ldb, D, p->blk_off.as<int32_t>(), Ttot,
+                                                              p->tbox_lo.as<float>(), p->tbox_hi.as<float>());
     CLUE_LAUNCH_CHECK("tile_boxes");
   }
 
   // ---- density + nearest-higher
-  k_fill_i32<<<1, 64, 0, st>>>(p->counters.as<int32_t>(), 0, 1);  // seed candidates
+  k_fill_i32<<<div_up(n_events, kThreads), kThreads, 0, st>>>(p->sc.as<int32_t>(), 0, n_events);
   CLUE_LAUNCH_CHECK("fill");
   PassArgs a{};
   for (int d = 0; d < D; ++d) a.xs[d] = p->xs.as<float>() + d * p->ld;
@@ -1641,8 +1788,10 @@ clue_status make_clusters(clue_plan_s* p, hipStream_t st, const clue_params* prm
   a.rho_s = p->rho_s.as<float>();
   a.perm = perm;
   a.offsets = p->offsets.as<int32_t>();
-  a.grid = p->grid.as<DevGrid>();
-  a.gs = gs;
+  a.ev = p->events.as<EventDesc>();
+  a.pev = p->pev.as<int32_t>();
+  a.sc = p->sc.as<int32_t>();
+  a.wrap = wrap;
   a.n = n;
   for (int k = 0; k < 3; ++k) a.kp[k] = kd.params[k];
   a.kden_rcp = 1.0 / static_cast<double>((2.f * kd.params[1]) * kd.params[1]);
@@ -1652,8 +1801,8 @@ clue_status make_clusters(clue_plan_s* p, hipStream_t st, const clue_params* prm
       a.xn[d] = p->xn.as<float>() + d * p->ld;
       a.bmin[d] = p->bbox_lo.as<float>() + d * p->ldb;
       a.bmax[d] = p->bbox_hi.as<float>() + d * p->ldb;
-      a.tmin[d] = p->tbox_lo.as<float>() + static_cast<int64_t>(d) * T;
-      a.tmax[d] = p->tbox_hi.as<float>() + static_cast<int64_t>(d) * T;
+      a.tmin[d] = p->tbox_lo.as<float>() + static_cast<int64_t>(d) * Ttot;
+      a.tmax[d] = p->tbox_hi.as<float>() + static_cast<int64_t>(d) * Ttot;
     }
     a.wn = p->wn.as<float>();
     a.sn = p->sn.as<int32_t>();
@@ -1671,7 +1820,6 @@ clue_status make_clusters(clue_plan_s* p, hipStream_t st, const clue_params* prm
   a.rho_out = rho_buf;
   a.nh_out = nh_buf;
   a.delta_out = p->delta.as<float>();
-  a.seed_candidates = p->counters.as<int32_t>();
   rc = dispatch_dim(D, [&](auto dc_) {
     return dispatch_metric_passes<decltype(dc_)::value>(md.type, a, kd.type, prm->dc, prm->dm, st, p,
                                                         blocks);
@@ -1698,8 +1846,13 @@ clue_status make_clusters(clue_plan_s* p, hipStream_t st, const clue_params* prm
   if (rc != CLUE_OK) return rc;
   rc = scan_exclusive(p->flag.as<int32_t>(), p->flag.as<int32_t>(), n, p->part.as<int32_t>(), st);
   if (rc != CLUE_OK) return rc;
-  k_capacity<<<1, 64, 0, st>>>(p->counters.as<int32_t>(), p->flag.as<int32_t>() + n,
-                               prm->seed_mode == CLUE_SEEDS_UNBOUNDED ? 1 : 0);
+  k_fill_i32<<<1, 64, 0, st>>>(counters, 0, 1);      // totals of this call; [1] carries over
+  k_fill_i32<<<1, 64, 0, st>>>(counters + 2, 0, 6);
+  CLUE_LAUNCH_CHECK("fill");
+  k_capacity<<<div_up(n_events, kThreads), kThreads, 0, st>>>(
+      p->events.as<EventDesc>(), n_events, p->sc.as<int32_t>(), p->flag.as<int32_t>(), counters,
+      p->cap_used.as<int32_t>(), p->rank_base.as<int32_t>(), prm->seed_mode == CLUE_SEEDS_UNBOUNDED ? 1 : 0,
+      persistent_capacity ? 1 : 0);
   CLUE_LAUNCH_CHECK("capacity");
   CLUE_MARK(7);
   // Rounds with 4096 hops each cover any forest depth below 4096^rounds.
@@ -1707,10 +1860,10 @@ clue_status make_clusters(clue_plan_s* p, hipStream_t st, const clue_params* prm
   int rounds = 1;
   for (int64_t span = hops; span < n; span *= hops) ++rounds;
   for (int r = 0; r < rounds; ++r) {
-    k_labels<<<div_up(n, kThreads), kThreads, 0, st>>>(p->root.as<int32_t>(), n, hops, seed_buf,
-                                                        p->flag.as<int32_t>(),
-                                                        p->counters.as<int32_t>(), cluster_out,
-                                                        r == rounds - 1 ? 1 : 0);
+    k_labels<<<div_up(n, kThreads), kThreads, 0, st>>>(
+        p->root.as<int32_t>(), n, hops, seed_buf, p->flag.as<int32_t>(), p->pev.as<int32_t>(),
+        p->events.as<EventDesc>(), p->cap_used.as<int32_t>(), p->rank_base.as<int32_t>(), cluster_out,
+        nh_buf, r == rounds - 1 ? 1 : 0);
     CLUE_LAUNCH_CHECK("labels");
   }
 
@@ -1719,9 +1872,22 @@ clue_status make_clusters(clue_plan_s* p, hipStream_t st, const clue_params* prm
   p->last_stream = st;
   p->last_n = n;
   p->last_d = D;
-  p->last_npd = npd;
-  p->last_T = T;
+  p->last_T = Ttot;
+  p->last_events = n_events;
   return CLUE_OK;
+}
+
+clue_status make_clusters(clue_plan_s* p, hipStream_t st, const clue_params* prm, int32_t D,
+                          int32_t n, const float* const* coords, const float* weights,
+                          const clue_kernel_desc* kernel, const clue_metric_desc* metric,
+                          const uint8_t* wrapped, int32_t* cluster_out, int32_t* is_seed_out,
+                          float* rho_out, int32_t* nh_out) {
+  if (n <= 0)
+    return fail(CLUE_ERR_INVALID_ARGUMENT,
+                "Number of points passed to PointsDevice constructor must be positive.");
+  const int64_t offsets[2] = {0, n};
+  return cluster_batch(p, st, prm, D, 1, offsets, coords, weights, kernel, metric, wrapped, cluster_out,
+                       is_seed_out, rho_out, nh_out, true);
 }
 
 }  // namespace clue_hip
@@ -1830,14 +1996,10 @@ clue_status clue_plan_destroy(clue_plan_t plan) {
   if (!plan) return CLUE_OK;
   (void)hipSetDevice(plan->device);
   if (plan->has_run && plan->last_stream) (void)hipStreamSynchronize(plan->last_stream);
-  for (DevBuf* b : {&plan->key, &plan->key2, &plan->perm, &plan->perm2, &plan->count, &plan->offsets,
-                    &plan->hist, &plan->part, &plan->xs, &plan->ws, &plan->hw, &plan->rho_s,
-                    &plan->delta, &plan->flag, &plan->root, &plan->nh_tmp, &plan->rho_tmp,
-                    &plan->seed_tmp, &plan->bbox_part, &plan->grid, &plan->counters, &plan->kn,
-                    &plan->kn2, &plan->pn, &plan->pn2, &plan->inv, &plan->xn, &plan->wn, &plan->sn,
-                    &plan->rn, &plan->bcnt, &plan->blk_off, &plan->bbox_lo, &plan->bbox_hi,
-                    &plan->tbox_lo, &plan->tbox_hi, &plan->pending, &plan->pending_flag})
-    b->release();
+  for (int k = 0; DevBuf* b = plan->all_buffers(k); ++k) b->release();
+  if (plan->stage_done) (void)hipEventSynchronize(plan->stage_done);
+  if (plan->stage) (void)hipHostFree(plan->stage);
+  if (plan->stage_done) (void)hipEventDestroy(plan->stage_done);
   for (hipEvent_t& e : plan->ev)
     if (e) (void)hipEventDestroy(e);
   delete plan;
@@ -1860,20 +2022,13 @@ clue_status clue_plan_reserve(clue_plan_t plan, int32_t ndim, int32_t npoints) {
     return fail(CLUE_ERR_INVALID_ARGUMENT, "invalid reserve size");
   int32_t npd, T;
   grid_shape(ndim, npoints, 128, npd, T);
-  return reserve(plan, ndim, npoints, T);
+  return reserve(plan, ndim, npoints, T, 1, static_cast<int32_t>(div_up(npoints, kChunk)));
 }
 
 clue_status clue_plan_device_bytes(clue_plan_t plan, size_t* bytes) {
   if (!plan || !bytes) return fail(CLUE_ERR_INVALID_ARGUMENT, "NULL argument");
   size_t s = 0;
-  for (DevBuf* b : {&plan->key, &plan->key2, &plan->perm, &plan->perm2, &plan->count, &plan->offsets,
-                    &plan->hist, &plan->part, &plan->xs, &plan->ws, &plan->hw, &plan->rho_s,
-                    &plan->delta, &plan->flag, &plan->root, &plan->nh_tmp, &plan->rho_tmp,
-                    &plan->seed_tmp, &plan->bbox_part, &plan->grid, &plan->counters, &plan->kn,
-                    &plan->kn2, &plan->pn, &plan->pn2, &plan->inv, &plan->xn, &plan->wn, &plan->sn,
-                    &plan->rn, &plan->bcnt, &plan->blk_off, &plan->bbox_lo, &plan->bbox_hi,
-                    &plan->tbox_lo, &plan->tbox_hi, &plan->pending, &plan->pending_flag})
-    s += b->bytes;
+  for (int k = 0; DevBuf* b = plan->all_buffers(k); ++k) s += b->bytes;
   *bytes = s;
   return CLUE_OK;
 }
@@ -1889,6 +2044,19 @@ clue_status clue_make_clusters(clue_plan_t plan, void* stream, const clue_params
   return make_clusters(plan, static_cast<hipStream_t>(stream), params, ndim, n, coords_dev,
                        weights_dev, kernel, metric, wrapped, cluster_out, is_seed_out, rho_out,
                        nearest_higher_out);
+}
+
+clue_status clue_make_clusters_batched(clue_plan_t plan, void* stream, const clue_params* params,
+                                       int32_t ndim, int32_t n_events, const int64_t* event_offsets,
+                                       const float* const* coords_dev, const float* weights_dev,
+                                       const clue_kernel_desc* kernel, const clue_metric_desc* metric,
+                                       const uint8_t* wrapped, int32_t* cluster_out, int32_t* is_seed_out,
+                                       float* rho_out, int32_t* nearest_higher_out) {
+  if (!plan) return fail(CLUE_ERR_INVALID_ARGUMENT, "plan is NULL");
+  CLUE_HIP(hipSetDevice(plan->device));
+  return cluster_batch(plan, static_cast<hipStream_t>(stream), params, ndim, n_events, event_offsets,
+                       coords_dev, weights_dev, kernel, metric, wrapped, cluster_out, is_seed_out, rho_out,
+                       nearest_higher_out, false);
 }
 
 clue_status clue_main_run(float dc, float rhoc, float dm, float seed_dc, int32_t points_per_tile,
@@ -1950,13 +2118,13 @@ clue_status clue_plan_last_grid(clue_plan_t plan, clue_grid_info* grid) {
   if (!plan->has_run) return fail(CLUE_ERR_INVALID_ARGUMENT, "the plan has not run");
   CLUE_HIP(hipSetDevice(plan->device));
   CLUE_HIP(hipStreamSynchronize(plan->last_stream));
-  DevGrid g;
-  CLUE_HIP(hipMemcpy(&g, plan->grid.p, sizeof(g), hipMemcpyDeviceToHost));
+  EventDesc g;  // the first event of the last call (the only one of a plain call)
+  CLUE_HIP(hipMemcpy(&g, plan->events.p, sizeof(g), hipMemcpyDeviceToHost));
   std::memset(grid, 0, sizeof(*grid));
   grid->ndim = plan->last_d;
-  grid->npoints = plan->last_n;
-  grid->npd = plan->last_npd;
-  grid->ntiles = plan->last_T;
+  grid->npoints = g.n;
+  grid->npd = g.npd;
+  grid->ntiles = g.ntiles;
   for (int d = 0; d < plan->last_d; ++d) {
     grid->minmax[2 * d] = g.mn[d];
     grid->minmax[2 * d + 1] = g.mx[d];

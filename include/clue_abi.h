@@ -171,6 +171,24 @@ clue_status clue_make_clusters(clue_plan_t plan, void* stream, const clue_params
                                int32_t* cluster_out, int32_t* is_seed_out, float* rho_out,
                                int32_t* nearest_higher_out);
 
+/* Batched clustering of independent events in one set of launches.  No
+ * reference counterpart: the reference clusters one event per make_clusters
+ * call, reusing the Clusterer's scratch (SetupTiles.hpp:28-37,
+ * SetupSeeds.hpp:14-18, SetupFollowers.hpp:14-22).
+ * event_offsets: host array of n_events+1 ascending offsets, event_offsets[0]
+ * = 0; event e is points [event_offsets[e], event_offsets[e+1]) of the arrays
+ * (each event at least one point).  Every event is clustered exactly as a
+ * make_clusters call of a fresh Clusterer on that event alone: its own grid,
+ * its own seed capacity, labels 0..k-1 and nearest_higher indexes local to the
+ * event.  The plan's grow-only capacity is neither used nor changed.  Same
+ * output arrays as clue_make_clusters, in batch order. */
+clue_status clue_make_clusters_batched(clue_plan_t plan, void* stream, const clue_params* params,
+                                       int32_t ndim, int32_t n_events, const int64_t* event_offsets,
+                                       const float* const* coords_dev, const float* weights_dev,
+                                       const clue_kernel_desc* kernel, const clue_metric_desc* metric,
+                                       const uint8_t* wrapped, int32_t* cluster_out, int32_t* is_seed_out,
+                                       float* rho_out, int32_t* nearest_higher_out);
+
 /* Host-buffer convenience: the reference's Python FFI entry
  * mainRun(dc, rhoc, dm, seed_dc, pPBin, wrapped, data, results, kernel, Ndim,
  * n_points, block_size, device_id), CLUEstering/BindingModules/serial/
