@@ -126,9 +126,12 @@ __device__ __constant__ double kExp2Table[64] = {
     0x1.d5818dcfba487p+0, 0x1.da9e603db3285p+0, 0x1.dfc97337b9b5fp+0, 0x1.e502ee78b3ff6p+0,
     0x1.ea4afa2a490dap+0, 0x1.efa1bee615a27p+0, 0x1.f50765b6e4540p+0, 0x1.fa7c1819e90d8p+0};
 
+// Inputs are clamped to [-110, 89] (fmaxf/fminf map NaN to a bound): the
+// double-precision evaluation then rounds to 0 below -103.97 and to +inf above
+// 88.72 by itself, and NaN is restored at the end.
 __device__ __forceinline__ float engine_expf(float x) {
-  const float xc = x > 89.0f ? 89.0f : (x < -110.0f ? -110.0f : x);
-  const double xd = static_cast<double>(xc == xc ? xc : 0.0f);
+  const float xc = __builtin_fminf(__builtin_fmaxf(x, -110.0f), 89.0f);
+  const double xd = static_cast<double>(xc);
   const double kd = __builtin_rint(xd * 0x1.71547652b82fep+6);
   const int k = static_cast<int>(kd);
   double r = __builtin_fma(-kd, 0x1.62e42fefp-7, xd);
@@ -140,9 +143,7 @@ __device__ __forceinline__ float engine_expf(float x) {
   const int j = k & 63;
   const int m = (k - j) / 64;
   const double t = kExp2Table[j];
-  float y = static_cast<float>(__builtin_ldexp(__builtin_fma(t, p, t), m));
-  y = x > 89.0f ? __builtin_inff() : y;
-  y = x < -110.0f ? 0.0f : y;
+  const float y = static_cast<float>(__builtin_ldexp(__builtin_fma(t, p, t), m));
   return x != x ? x : y;
 }
 

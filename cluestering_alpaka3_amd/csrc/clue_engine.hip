@@ -482,7 +482,20 @@ struct PassArgs {
   float* rho_out;            // density: original order
   int32_t* nh_out;           // nearest-higher: original order
   float* delta_out;          // nearest-higher distance: original order
+  int xcd;                   // bit 0: XCD-aware block order of density, bit 1: of nearest-higher
 };
+
+// Workgroups are dispatched to the 8 XCDs round robin (block b to XCD b % 8).
+// With xcd set, XCD x takes the contiguous range x of the logical blocks, so
+// neighbouring query blocks -- which read the same tiles -- share one L2.
+__device__ __forceinline__ int logical_block(int xcd) {
+  const int b = blockIdx.x;
+  if (!xcd) return b;
+  const int nb = gridDim.x;
+  const int q = nb >> 3, r = nb & 7;
+  const int x = b & 7, i = b >> 3;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
 
 // Search box of a point: [bin(x - r), bin(x + r)] per dimension, wrapped
 // dimensions unrolled past npd (TilesView::searchBox, TilesView.hpp:72-82).
@@ -598,9 +611,11 @@ __device__ __forceinline__ float readlane_f32(float v, int lane) {
 // so the lanes of a wave share most of their search box); otherwise in tile order.
 template <int D, int METRIC, int KERNEL, bool UNION, bool QNH>
 __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
-  const int pos = blockIdx.x * kThreads + threadIdx.x;
+  const int pos = logical_block(a.xcd & 1) * kThreads + threadIdx.x;
   const bool valid = pos < a.n;
   const int pq = valid ? pos : a.n - 1;
+  const int lane = __lane_id();
+  const int wv = threadIdx.x / kWave;
   float q[D];
   float w_self;  // the point's own term is 1 * w_i (kernel value 1 for j == i)
   int s;         // the query's slot in the tile-ordered candidate layout
@@ -628,6 +643,7 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
   // The Euclidean flat-kernel loop reads the precomputed h*w_j; every other
   // combination reads w_j and evaluates the kernel.
   constexpr bool kUsesHW = (METRIC == MEUCLID) && (KERNEL == KFLAT);
+
   // One candidate: rho_i += ((float)(d <= dc) * k) * w_j in the visit order
   // (ClusteringKernels.hpp:38-44); out-of-radius terms are +0 and skipped.
   auto eval = [&](auto self_c, int j, const float* c, float wj, float hwj) {
@@ -635,9 +651,9 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
     if constexpr (METRIC == MEUCLID) {
       const float s2 = squared_distance<D>(q, c);
       if constexpr (KERNEL == KFLAT) {
-        float term = hwj;
-        if constexpr (SELF) term = (j == s) ? w_self : hwj;
-        if (s2 <= a.thr) rho += term;
+        float t = hwj;
+        if constexpr (SELF) t = (j == s) ? w_self : hwj;
+        if (s2 <= a.thr) rho += t;
       } else {
         if (s2 <= a.thr) {
           const float k =
@@ -653,20 +669,23 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
       }
     }
   };
-  auto run = [&](auto self_c, int beg, int end) {
+  auto run = [&](auto self_c, int beg, int end, bool need) {
     int j = beg;
     if constexpr (UNION) {
-      for (; j + 4 <= end; j += 4) {
-        float c[4][D], wv[4], hv[4];
+      constexpr int U = D <= 2 ? 8 : 4;  // candidates loaded ahead of their use
+      for (; j + U <= end; j += U) {
+        float c[U][D], wv4[U], hv[U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < U; ++u) {
 #pragma unroll
           for (int d = 0; d < D; ++d) c[u][d] = X[d][j + u];
           if constexpr (kUsesHW) hv[u] = HW[j + u];
-          else wv[u] = W[j + u];
+          else wv4[u] = W[j + u];
         }
+        if (need) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) eval(self_c, j + u, c[u], wv[u], hv[u]);
+          for (int u = 0; u < U; ++u) eval(self_c, j + u, c[u], wv4[u], hv[u]);
+        }
       }
     }
     for (; j < end; ++j) {
@@ -676,7 +695,7 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
       float wj = 0.f, hwj = 0.f;
       if constexpr (kUsesHW) hwj = HW[j];
       else wj = W[j];
-      eval(self_c, j, c, wj, hwj);
+      if (need) eval(self_c, j, c, wj, hwj);
     }
   };
   // Bitmask walk (Morton queries, Euclidean, no wrapping): the blocks of the
@@ -687,8 +706,6 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
   // (box lower bounds are exact, see lower_bound2), so their terms are +0.
   constexpr bool kMasked = QNH && UNION && METRIC == MEUCLID;
   __shared__ uint32_t s_mask[kMasked ? kThreads / kWave : 1][kMasked ? kMaskWords : 1];
-  const int lane = __lane_id();
-  const int wv = threadIdx.x / kWave;
   float qlo[D], qhi[D];
   if constexpr (kMasked) {
     for (int w = lane; w < kMaskWords; w += kWave) s_mask[wv][w] = 0u;
@@ -760,10 +777,8 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
           return;
         }
       }
-      if (need) {
-        if (self) run(std::true_type{}, beg, end);
-        else run(std::false_type{}, beg, end);
-      }
+      if (self) run(std::true_type{}, beg, end, need);
+      else run(std::false_type{}, beg, end, need);
     });
   });
   if (valid) {
@@ -779,7 +794,7 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
 
 template <int D, int METRIC, bool UNION>
 __global__ __launch_bounds__(kThreads) void k_nearest_higher(const PassArgs a) {
-  const int s = blockIdx.x * kThreads + threadIdx.x;
+  const int s = logical_block((a.xcd >> 1) & 1) * kThreads + threadIdx.x;
   const bool valid = s < a.n;
   const int sq = valid ? s : a.n - 1;
   float q[D];
@@ -904,7 +919,7 @@ __device__ __forceinline__ float sqrt_le_bound_dev(float delta) {
 // The others are queued for PHASE 2, the full search within dm.
 template <int D, int PHASE>
 __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassArgs a) {
-  const int gid = blockIdx.x * kThreads + threadIdx.x;
+  const int gid = logical_block((a.xcd >> 1) & 1) * kThreads + threadIdx.x;
   int pos;
   bool valid;
   if constexpr (PHASE == 1) {
@@ -947,28 +962,46 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
     qhi[d] = wave_max_f32(valid ? q[d] : -__builtin_inff());
   }
 
+  auto cand = [&](int t, const float* c, float rho_j, int32_t id_j) {
+    const int rb = __float_as_int(rho_j);
+    const bool pos_rho = (rb > 0) && (rb <= 0x7f800000);
+    const bool higher = (rho_j > rho_i) || ((rho_j == rho_i) && pos_rho && (id_j > id_i));
+    const float s2 = squared_distance<D>(q, c);
+    if (higher && s2 <= bound) {
+      const float dist = __builtin_sqrtf(s2);
+      const int64_t key = (static_cast<int64_t>(t) << 32) | static_cast<uint32_t>(id_j);
+      if (dist < delta || key < best) {
+        delta = dist;
+        nh = id_j;
+        best = key;
+        const float lim = sqrt_le_bound_dev(dist);
+        bound = lim < cap ? lim : cap;
+      }
+    }
+  };
+  // candidates [beg, end) of tile t, wave-uniform: eight at a time, all loads
+  // issued before the first use
   auto scan = [&](int t, int beg, int end) {
-    for (int j = beg; j < end; ++j) {
+    constexpr int U = 8;
+    int j = beg;
+    for (; j + U <= end; j += U) {
+      float c[U][D], r[U];
+      int32_t id[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) c[u][d] = X[d][j + u];
+        r[u] = R[j + u];
+        id[u] = P[j + u];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) cand(t, c[u], r[u], id[u]);
+    }
+    for (; j < end; ++j) {
       float c[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) c[d] = X[d][j];
-      const float rho_j = R[j];
-      const int32_t id_j = P[j];
-      const int rb = __float_as_int(rho_j);
-      const bool pos_rho = (rb > 0) && (rb <= 0x7f800000);
-      const bool higher = (rho_j > rho_i) || ((rho_j == rho_i) && pos_rho && (id_j > id_i));
-      const float s2 = squared_distance<D>(q, c);
-      if (higher && s2 <= bound) {
-        const float dist = __builtin_sqrtf(s2);
-        const int64_t key = (static_cast<int64_t>(t) << 32) | static_cast<uint32_t>(id_j);
-        if (dist < delta || key < best) {
-          delta = dist;
-          nh = id_j;
-          best = key;
-          const float lim = sqrt_le_bound_dev(dist);
-          bound = lim < cap ? lim : cap;
-        }
-      }
+      cand(t, c, R[j], P[j]);
     }
   };
 
@@ -1091,6 +1124,101 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
       a.nh_out[id_i] = nh;
       a.delta_out[id_i] = delta;
       if (nh == -1) atomicAdd(&a.sc[my_e], 1);
+    }
+  }
+}
+
+// Second nearest-higher phase: one wave per unresolved point (pending[0..npend),
+// positions in the Morton layout).  The 64 lanes split the candidates of the
+// point's search box (tiles pruned by their boxes against the wave's best
+// bound), each keeps its best (distance, tile id, point id), and a wave
+// reduction takes the least: the serial backend's first strict minimum on a
+// grid without wrapping.  The grid is persistent: every wave strides over the
+// pending list and leaves when it is exhausted.
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
+  const int npend = uniform(*a.pending_count);
+  const int lane = __lane_id();
+  const int nwaves = gridDim.x * (kThreads / kWave);
+  cptr<int32_t> PEND = as_const(a.pending);
+  cptr<int32_t> OFF = as_const(a.offsets);
+  cptr<int32_t> PEV = as_const(a.pev);
+  for (int w = uniform(static_cast<int>((blockIdx.x * kThreads + threadIdx.x) / kWave)); w < npend;
+       w += nwaves) {
+    const int pos = PEND[w];
+    float q[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) q[d] = as_const(a.xn[d])[pos];
+    const float rho_i = as_const(a.rn)[pos];
+    const int32_t id_i = as_const(a.pn)[pos];
+    const int e = PEV[id_i];
+    const EvGrid<D> g = load_event<D>(a.ev, e, a.wrap);
+    int lo[D], hi[D];
+    search_box<D>(q, a.radius, g, lo, hi);
+    float delta = FLT_MAX;
+    int64_t best = INT64_MAX;
+    float bound = a.thr;  // candidates need s2 <= min(thr(dm), max{s : sqrtf(s) <= delta})
+    float wave_bound = a.thr;
+    int b[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) b[d] = lo[d];
+    while (true) {
+      const int t = tile_index<D>(b, g);
+      float tl[D], th[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        tl[d] = as_const(a.tmin[d])[t];
+        th[d] = as_const(a.tmax[d])[t];
+      }
+      if (lower_bound2<D>(q, tl, th) <= wave_bound) {
+        const int end = OFF[t + 1];
+        for (int j = OFF[t] + lane; j < end; j += kWave) {
+          float c[D];
+#pragma unroll
+          for (int d = 0; d < D; ++d) c[d] = a.xn[d][j];
+          const float rho_j = a.rn[j];
+          const int32_t id_j = a.pn[j];
+          const int rb = __float_as_int(rho_j);
+          const bool pos_rho = (rb > 0) && (rb <= 0x7f800000);  // rho_j > 0.f
+          const bool higher = (rho_j > rho_i) || ((rho_j == rho_i) && pos_rho && (id_j > id_i));
+          const float s2 = squared_distance<D>(q, c);
+          if (higher && s2 <= bound) {
+            const float dist = __builtin_sqrtf(s2);
+            const int64_t key = (static_cast<int64_t>(t) << 32) | static_cast<uint32_t>(id_j);
+            if (dist < delta || key < best) {
+              delta = dist;
+              best = key;
+              const float lim = sqrt_le_bound_dev(dist);
+              bound = lim < a.thr ? lim : a.thr;
+            }
+          }
+        }
+        wave_bound = wave_min_f32(bound);  // the point's best so far over all lanes
+      }
+      int d = D - 1;
+      for (; d >= 0; --d) {
+        if (++b[d] <= hi[d]) break;
+        b[d] = lo[d];
+      }
+      if (d < 0) break;
+    }
+    // lexicographic (distance, key) minimum over the wave
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const float od = __shfl_xor(delta, off);
+      const int32_t klo = __shfl_xor(static_cast<int32_t>(best & 0xffffffff), off);
+      const int32_t khi = __shfl_xor(static_cast<int32_t>(best >> 32), off);
+      const int64_t ok = (static_cast<int64_t>(khi) << 32) | static_cast<uint32_t>(klo);
+      if (od < delta || (od == delta && ok < best)) {
+        delta = od;
+        best = ok;
+      }
+    }
+    if (lane == 0) {
+      const int32_t nh = best == INT64_MAX ? -1 : static_cast<int32_t>(best & 0xffffffff);
+      a.nh_out[id_i] = nh;
+      a.delta_out[id_i] = nh == -1 ? FLT_MAX : delta;
+      if (nh == -1) atomicAdd(&a.sc[e], 1);
     }
   }
 }
@@ -1287,7 +1415,7 @@ struct clue_plan_s {
   // Morton-ordered layout and the block boxes of the nearest-higher search
   DevBuf kn, kn2, pn, pn2, inv, xn, wn, sn, rn, bcnt, blk_off, bbox_lo, bbox_hi, tbox_lo, tbox_hi;
   DevBuf pending, pending_flag;
-  clue_tuning tuning{0.25f, 1, 128, -1};
+  clue_tuning tuning{0.15f, 0, 128, -1, 3, 0};
   int64_t ldb = 0;  // leading dimension of the block boxes
   int64_t ld = 0;   // leading dimension of xs / xn
   // pinned staging of the event and chunk tables (reused once its copy is done)
@@ -1467,8 +1595,11 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
       k_compact<<<grid, kThreads, 0, st>>>(a.pending_flag, a.n, a.pending);
       CLUE_LAUNCH_CHECK("compact");
       a.pending_count = a.pending_flag + a.n;
-      k_nearest_higher_blocks<D, 2><<<grid, kThreads, 0, st>>>(a);
-      CLUE_LAUNCH_CHECK("nearest_higher_blocks_full");
+      // enough waves to fill the GPU; surplus waves find the pending list exhausted
+      const int pblocks = p->tuning.pending_blocks > 0 ? p->tuning.pending_blocks : 2048;
+      const int nblk = static_cast<int>(std::min<int64_t>(pblocks, div_up(a.n, kThreads / kWave)));
+      k_nh_pending<D><<<nblk, kThreads, 0, st>>>(a);
+      CLUE_LAUNCH_CHECK("nearest_higher_pending");
       return CLUE_OK;
     }
   }
@@ -1816,6 +1947,7 @@ clue_status cluster_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
     a.pending_flag = p->pending_flag.as<int32_t>();
     a.mask_min_points = p->tuning.density_bitmask ? std::max(p->tuning.bitmask_min_points, 1) : 0;
   }
+  a.xcd = p->tuning.xcd_remap;
   a.rho_s_out = p->rho_s.as<float>();
   a.rho_out = rho_buf;
   a.nh_out = nh_buf;
@@ -2157,7 +2289,8 @@ clue_status clue_plan_copy_delta(clue_plan_t plan, float* delta) {
 
 clue_status clue_plan_set_tuning(clue_plan_t plan, const clue_tuning* tuning) {
   if (!plan || !tuning) return fail(CLUE_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (!(tuning->nh_first_radius > 0.f) || tuning->bitmask_min_points < 0 || tuning->morton_bits > 10)
+  if (!(tuning->nh_first_radius > 0.f) || tuning->bitmask_min_points < 0 || tuning->morton_bits > 10 ||
+      tuning->xcd_remap < 0 || tuning->xcd_remap > 3 || tuning->pending_blocks < 0)
     return fail(CLUE_ERR_INVALID_ARGUMENT, "invalid tuning values");
   plan->tuning = *tuning;
   return CLUE_OK;

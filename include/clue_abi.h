@@ -101,12 +101,19 @@ typedef struct {
  *  density_bitmask     1: large tiles walk only the candidates near the wave
  *                      through an LDS bitmask (Euclidean, no wrapping)
  *  bitmask_min_points  smallest tile that takes the bitmask walk
- *  morton_bits         bits per dimension of the in-tile Morton order, -1 auto */
+ *  morton_bits         bits per dimension of the in-tile Morton order, -1 auto
+ *  xcd_remap           bit 0: density, bit 1: nearest-higher -- give each
+ *                      XCD a contiguous range of the pass's query blocks, so
+ *                      neighbouring queries share one L2
+ *  pending_blocks      workgroups of the second nearest-higher phase (one
+ *                      wave per unresolved point), 0 auto */
 typedef struct {
   float nh_first_radius;
   int32_t density_bitmask;
   int32_t bitmask_min_points;
   int32_t morton_bits;
+  int32_t xcd_remap;
+  int32_t pending_blocks;
 } clue_tuning;
 
 typedef struct clue_plan_s* clue_plan_t;

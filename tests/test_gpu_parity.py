@@ -60,9 +60,23 @@ def test_device_math_matches_host():
     x = np.concatenate([np.linspace(-111.0, 90.0, 400001, dtype=np.float32),
                         np.array([0.0, -0.0, 1e-30, -1e-30, 88.72, 88.73, -103.9, -104.0, np.inf, -np.inf],
                                  np.float32)])
+    f32 = np.float32
+    specials = np.array([np.nan, 89.0, np.nextafter(f32(89), f32(np.inf)), -110.0,
+                         np.nextafter(f32(-110), f32(-np.inf)), np.finfo(f32).max, -np.finfo(f32).max,
+                         -87.34, -103.97, -103.98, 88.7228, 88.7229], f32)
+    x = np.concatenate([x, specials])
     got = clue.selftest_math(0, x)
     want = oracle.expf_engine(x)
     np.testing.assert_array_equal(got.view(np.int32), want.view(np.int32))
+    # every 61st float of [-110, 89] (about 37 million arguments)
+    lo_neg = np.float32(-110.0).view(np.uint32)  # negative floats: bit patterns 0x80000000..lo_neg
+    hi_pos = np.float32(89.0).view(np.uint32)
+    bits = np.concatenate([np.arange(0x80000000, int(lo_neg) + 1, 61, dtype=np.uint64),
+                           np.arange(0, int(hi_pos) + 1, 61, dtype=np.uint64)]).astype(np.uint32)
+    for k in range(0, bits.size, 1 << 23):
+        xs = bits[k:k + (1 << 23)].view(np.float32)
+        np.testing.assert_array_equal(clue.selftest_math(0, xs).view(np.int32),
+                                      oracle.expf_engine(xs).view(np.int32))
     # sqrtf and division are correctly rounded on both sides
     rng = np.random.default_rng(1)
     a = np.abs(rng.standard_normal(200000).astype(np.float32)) * np.float32(1e3)
@@ -99,7 +113,10 @@ def test_workload_parity(name, n):
 
 TUNINGS = [dict(nh_first_radius=0.05, density_bitmask=1, bitmask_min_points=1, morton_bits=5),
            dict(nh_first_radius=1.0, density_bitmask=0, bitmask_min_points=0, morton_bits=1),
-           dict(nh_first_radius=0.5, density_bitmask=1, bitmask_min_points=64, morton_bits=-1)]
+           dict(nh_first_radius=0.5, density_bitmask=1, bitmask_min_points=64, morton_bits=-1),
+           dict(xcd_remap=0, pending_blocks=1),
+           dict(xcd_remap=3, pending_blocks=7, density_bitmask=0),
+           dict(xcd_remap=2, pending_blocks=0, nh_first_radius=0.01)]
 
 
 @pytest.mark.parametrize("tuning", TUNINGS)
