@@ -13,6 +13,7 @@ serial on a bounded spatial window of the same workload, and OpenMP on all of
 it).
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -40,7 +41,115 @@ def parse():
                          "WRITE_SIZE calibrate the HBM byte counters (0 disables)")
     ap.add_argument("--window-fraction", type=float, default=0.25,
                     help="side of the serial CPU sample window, as a fraction of the box side")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-events measurement")
+    ap.add_argument("--batch-events", type=int, default=4096)
+    ap.add_argument("--batch-steps", type=int, default=10)
     return ap.parse_args()
+
+
+def batched_bench(clue, datasets, args):
+    """4096 independent toy-calorimeter events (sizes lognormal, median 5000, clipped
+    to [500, 50000]; dc=1.3, rhoc=2, dm=2.6, FlatKernel(0.5)) in one batched call,
+    autotuned; against per-event plain launches of the same events."""
+    sizes = datasets.event_sizes(args.batch_events, 2024)
+    evs = [datasets.calorimeter_event(int(n), 7_000_000 + k) for k, n in enumerate(sizes)]
+    coords = np.ascontiguousarray(np.concatenate([e.coords for e in evs], axis=1))
+    weights = np.ascontiguousarray(np.concatenate([e.weights for e in evs]))
+    del evs
+    off = np.zeros(sizes.size + 1, np.int64)
+    off[1:] = np.cumsum(sizes)
+    N, D, E = coords.shape[1], coords.shape[0], sizes.size
+    stream = clue.Stream(0)
+    dcoords = clue.DeviceArray.from_host(coords)
+    dw = clue.DeviceArray.from_host(weights)
+    cp = [dcoords.offset(d * N) for d in range(D)]
+    kd, md = clue.kernel_desc(("flat", 0.5)), clue.metric_desc("euclidean")
+    params = clue.make_params(1.3, 2.0, 2.6)
+    outs = [clue.DeviceArray((N,), t) for t in (np.int32, np.int32, np.float32, np.int32)]
+    plan = clue.Plan(0)
+
+    # autotune on the first events (about 4M points), points per tile included
+    k = int(np.searchsorted(off, 4_000_000))
+    k = max(1, min(k, E))
+    t0 = time.perf_counter()
+    tuning, ppb, tune_ms = plan.autotune_batched(stream, params, D, off[: k + 1], cp, dw.ptr, kd, md,
+                                                  tune_points_per_tile=True)
+    tune_s = time.perf_counter() - t0
+    plan.set_tuning(**tuning)
+    params = clue.make_params(1.3, 2.0, 2.6, ppb=ppb)
+
+    def batched():
+        plan.make_clusters_batched_device(stream, params, D, off, cp, dw.ptr, kd, md, None,
+                                          *[o.ptr for o in outs])
+
+    batched()
+    stream.synchronize()
+    lat = []
+    for _ in range(args.batch_steps):
+        t0 = time.perf_counter()
+        batched()
+        stream.synchronize()
+        lat.append(time.perf_counter() - t0)
+    lat = np.array(lat)
+    labels_batched = outs[0].download(stream)
+
+    # the same batch as a HIP graph
+    g = clue.BatchGraph(params, D, off, cp, dw.ptr, kd, md, None, *[o.ptr for o in outs], tuning=tuning)
+    g.replay(stream)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        g.replay(stream)
+    stream.synchronize()
+    graph_s = (time.perf_counter() - t0) / args.batch_steps
+    labels_graph = outs[0].download(stream)
+    g.close()
+
+    # per-event plain launches (a fresh Clusterer per event: capacity reset), same ppb and tuning;
+    # arguments prepared ahead so the loop holds only the C ABI calls
+    lib = clue.lib()
+    prepared = []
+    for e in range(E):
+        a = int(off[e])
+        prepared.append((clue._coord_array([p + 4 * a for p in cp]), dw.ptr + 4 * a,
+                         outs[0].ptr + 4 * a, int(sizes[e])))
+    pk, pm, pp = ctypes.byref(kd), ctypes.byref(md), ctypes.byref(params)
+
+    def plain_all():
+        for cpe, we, oute, ne in prepared:
+            lib.clue_plan_reset_capacity(plan.handle)
+            clue.check(lib.clue_make_clusters(plan.handle, stream.handle, pp, D, ne, cpe, we, pk, pm, None,
+                                              oute, None, None, None))
+
+    plain_all()
+    stream.synchronize()
+    t0 = time.perf_counter()
+    plain_all()
+    stream.synchronize()
+    plain_s = time.perf_counter() - t0
+    labels_plain = outs[0].download(stream)
+    batch_s = float(np.median(lat))
+    res = {
+        "workload": f"{E} toy-calorimeter 2D events, sizes lognormal(median 5000, sigma 1) clipped to "
+                    f"[500, 50000]; dc=1.3 rhoc=2 dm=2.6 FlatKernel(0.5)",
+        "events": E, "points": N,
+        "events_per_s": round(E / batch_s, 1),
+        "mpoints_per_s": round(N / batch_s / 1e6, 3),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                       "p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
+                       "of": f"{args.batch_steps} batched calls, host clock, call to completion"},
+        "graph_events_per_s": round(E / graph_s, 1),
+        "plain_events_per_s": round(E / plain_s, 1),
+        "speedup_vs_plain": round(plain_s / batch_s, 2),
+        "labels_equal_plain": bool(np.array_equal(labels_batched, labels_plain)),
+        "labels_equal_graph": bool(np.array_equal(labels_batched, labels_graph)),
+        "autotune": {"tuning": tuning, "points_per_tile": ppb, "sample_events": k,
+                     "sample_ms": round(tune_ms, 3), "seconds": round(tune_s, 2)},
+    }
+    plan.close()
+    for a in outs + [dcoords, dw]:
+        a.free()
+    return res
 
 
 def cpu_baseline(ps, prm, args, gpu_labels):
@@ -188,6 +297,8 @@ def main():
         "stages_ms": stages,
         "stats": stats,
     }
+    if not args.no_batched:
+        result["batched"] = batched_bench(clue, datasets, args)
     if not args.no_cpu_baseline:
         cb, parity = cpu_baseline(ps, prm, args, labels)
         result["cpu_baseline"] = cb

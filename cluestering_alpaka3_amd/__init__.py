@@ -112,6 +112,14 @@ _SIGNATURES = {
     "clue_make_clusters_batched": ([_vp, _vp, _P(ClueParams), ctypes.c_int32, ctypes.c_int32,
                                     _P(ctypes.c_int64), _P(_vp), _vp, _P(ClueKernelDesc),
                                     _P(ClueMetricDesc), _P(ctypes.c_uint8), _vp, _vp, _vp, _vp], ctypes.c_int),
+    "clue_batch_graph_create": ([ctypes.c_int32, _P(ClueParams), ctypes.c_int32, ctypes.c_int32,
+                                 _P(ctypes.c_int64), _P(_vp), _vp, _P(ClueKernelDesc), _P(ClueMetricDesc),
+                                 _P(ctypes.c_uint8), _vp, _vp, _vp, _vp, _P(ClueTuning), _P(_vp)], ctypes.c_int),
+    "clue_batch_graph_launch": ([_vp, _vp], ctypes.c_int),
+    "clue_batch_graph_destroy": ([_vp], ctypes.c_int),
+    "clue_autotune_batched": ([_vp, _vp, _P(ClueParams), ctypes.c_int32, ctypes.c_int32, _P(ctypes.c_int64),
+                               _P(_vp), _vp, _P(ClueKernelDesc), _P(ClueMetricDesc), _P(ctypes.c_uint8),
+                               ctypes.c_int32, _P(ClueTuning), _P(ctypes.c_float)], ctypes.c_int),
     "clue_main_run": ([ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int32,
                        _P(ctypes.c_uint8), _P(ctypes.c_float), _P(ctypes.c_int32), _P(ClueKernelDesc),
                        ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32], ctypes.c_int),
@@ -316,6 +324,22 @@ class Plan:
                                                ctypes.byref(kernel), ctypes.byref(metric), wr, cluster_ptr,
                                                is_seed_ptr, rho_ptr, nh_ptr))
 
+    def autotune_batched(self, stream, params, ndim, event_offsets, coord_ptrs, weight_ptr, kernel, metric,
+                         wrapped=None, tune_points_per_tile=False):
+        """Time batched calls on these events; returns (tuning dict, points_per_tile, median ms).
+        The plan's own tuning is unchanged; apply the result with set_tuning(**tuning)."""
+        off = np.ascontiguousarray(event_offsets, dtype=np.int64)
+        cp = _coord_array(coord_ptrs)
+        wr = _wrap_array(wrapped)
+        prm = ClueParams.from_buffer_copy(params)
+        best = ClueTuning()
+        ms = ctypes.c_float(0)
+        check(lib().clue_autotune_batched(self.handle, stream.handle if stream else None, ctypes.byref(prm), ndim,
+                                          off.size - 1, off.ctypes.data_as(_P(ctypes.c_int64)), cp, weight_ptr,
+                                          ctypes.byref(kernel), ctypes.byref(metric), wr,
+                                          1 if tune_points_per_tile else 0, ctypes.byref(best), ctypes.byref(ms)))
+        return best.as_dict(), prm.points_per_tile, ms.value
+
     def stats(self):
         s = ClueStats()
         check(lib().clue_plan_last_stats(self.handle, ctypes.byref(s)))
@@ -371,6 +395,51 @@ class Plan:
     def close(self):
         if self.handle:
             lib().clue_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _coord_array(coord_ptrs):
+    return (_vp * MAX_DIM)(*([int(p) for p in coord_ptrs] + [0] * (MAX_DIM - len(coord_ptrs))))
+
+
+def _wrap_array(wrapped):
+    if wrapped is None:
+        return None
+    return (ctypes.c_uint8 * MAX_DIM)(*([int(v) for v in wrapped] + [0] * (MAX_DIM - len(wrapped))))
+
+
+class BatchGraph:
+    """A batched call captured as a HIP graph (clue_batch_graph_*): replay() re-runs
+    it on the same buffers, whose values may change between replays."""
+
+    def __init__(self, params, ndim, event_offsets, coord_ptrs, weight_ptr, kernel, metric, wrapped,
+                 cluster_ptr, is_seed_ptr=None, rho_ptr=None, nh_ptr=None, tuning=None, device=0):
+        off = np.ascontiguousarray(event_offsets, dtype=np.int64)
+        t = None
+        if tuning is not None:
+            t = ClueTuning()
+            for k, v in tuning.items():
+                setattr(t, k, v)
+        h = _vp()
+        check(lib().clue_batch_graph_create(device, ctypes.byref(params), ndim, off.size - 1,
+                                            off.ctypes.data_as(_P(ctypes.c_int64)), _coord_array(coord_ptrs),
+                                            weight_ptr, ctypes.byref(kernel), ctypes.byref(metric),
+                                            _wrap_array(wrapped), cluster_ptr, is_seed_ptr, rho_ptr, nh_ptr,
+                                            ctypes.byref(t) if t is not None else None, ctypes.byref(h)))
+        self.handle = h.value
+
+    def replay(self, stream):
+        check(lib().clue_batch_graph_launch(self.handle, stream.handle if stream else None))
+
+    def close(self):
+        if self.handle:
+            lib().clue_batch_graph_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

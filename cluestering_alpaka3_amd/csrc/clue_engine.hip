@@ -19,6 +19,7 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <memory>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -1418,10 +1419,18 @@ struct clue_plan_s {
   clue_tuning tuning{0.15f, 0, 128, -1, 3, 0};
   int64_t ldb = 0;  // leading dimension of the block boxes
   int64_t ld = 0;   // leading dimension of xs / xn
-  // pinned staging of the event and chunk tables (reused once its copy is done)
-  void* stage = nullptr;
-  size_t stage_bytes = 0;
-  hipEvent_t stage_done = nullptr;
+  // pinned staging ring of the event and chunk tables (a slot is reused once its
+  // copy has run, so the host can enqueue a few calls ahead of the device)
+  struct StageSlot {
+    void* host = nullptr;
+    size_t bytes = 0;
+    hipEvent_t done = nullptr;
+  };
+  static constexpr int kStageSlots = 4;
+  StageSlot stage[kStageSlots];
+  int stage_next = 0;
+  // the next plain call starts with a fresh grow-only seed capacity
+  bool reset_capacity = false;
   // last call
   bool has_run = false;
   hipStream_t last_stream = nullptr;
@@ -1742,14 +1751,28 @@ clue_status mark(clue_plan_s* p, int k, hipStream_t st) {
     if (m_ != CLUE_OK) return m_;          \
   } while (0)
 
-// One call over a batch of independent events (a plain call is a batch of one):
-// events[e] = points [event_offsets[e], event_offsets[e+1]) of the arrays.
-clue_status cluster_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm, int32_t D,
-                          int32_t n_events, const int64_t* event_offsets,
-                          const float* const* coords, const float* weights,
-                          const clue_kernel_desc* kernel, const clue_metric_desc* metric,
-                          const uint8_t* wrapped, int32_t* cluster_out, int32_t* is_seed_out,
-                          float* rho_out, int32_t* nh_out, bool persistent_capacity) {
+// A validated call over a batch of independent events (a plain call is a batch
+// of one): event e is points [event_offsets[e], event_offsets[e+1]).
+struct BatchCall {
+  int32_t D = 0, n = 0, n_events = 0, n_chunks = 0, Ttot = 0;
+  clue_kernel_desc kd{};
+  clue_metric_desc md{};
+  Wrap wrap{};
+  bool wrapped_any = false;
+  Coords c{};
+  const float* weights = nullptr;
+  int32_t* cluster_out = nullptr;
+  int32_t* is_seed_out = nullptr;
+  float* rho_out = nullptr;
+  int32_t* nh_out = nullptr;
+  bool persistent = false;  // the plan's grow-only seed capacity applies (plain calls)
+};
+
+clue_status describe_batch(const clue_params* prm, int32_t D, int32_t n_events, const int64_t* event_offsets,
+                           const float* const* coords, const float* weights, const clue_kernel_desc* kernel,
+                           const clue_metric_desc* metric, const uint8_t* wrapped, int32_t* cluster_out,
+                           int32_t* is_seed_out, float* rho_out, int32_t* nh_out, bool persistent,
+                           BatchCall& bc) {
   if (n_events <= 0 || !event_offsets)
     return fail(CLUE_ERR_INVALID_ARGUMENT, "the batch needs at least one event");
   const int64_t n64 = event_offsets[n_events] - event_offsets[0];
@@ -1758,17 +1781,24 @@ clue_status cluster_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
   const int32_t n = static_cast<int32_t>(n64);
   clue_status rc = validate(prm, D, n, coords, weights, kernel, metric, cluster_out);
   if (rc != CLUE_OK) return rc;
-  const clue_kernel_desc kd = kernel ? *kernel : clue_kernel_desc{CLUE_KERNEL_FLAT, {0.5f, 0.f, 0.f}};
-  clue_metric_desc md{};
-  if (metric) md = *metric;
-  Wrap wrap{};
-  bool wrapped_any = false;
+  bc = BatchCall{};
+  bc.D = D;
+  bc.n = n;
+  bc.n_events = n_events;
+  bc.kd = kernel ? *kernel : clue_kernel_desc{CLUE_KERNEL_FLAT, {0.5f, 0.f, 0.f}};
+  if (metric) bc.md = *metric;
   for (int d = 0; d < D; ++d) {
-    wrap.w[d] = wrapped ? (wrapped[d] ? 1 : 0) : 0;
-    wrapped_any = wrapped_any || wrap.w[d];
+    bc.wrap.w[d] = wrapped ? (wrapped[d] ? 1 : 0) : 0;
+    bc.wrapped_any = bc.wrapped_any || bc.wrap.w[d];
+    bc.c.p[d] = coords[d];
   }
-
-  // ---- event and chunk tables (host), grid shapes with the reference's formulas
+  bc.weights = weights;
+  bc.cluster_out = cluster_out;
+  bc.is_seed_out = is_seed_out;
+  bc.rho_out = rho_out;
+  bc.nh_out = nh_out;
+  bc.persistent = persistent;
+  // grid shapes with the reference's formulas
   int64_t total_tiles = 0, n_chunks64 = 0;
   for (int e = 0; e < n_events; ++e) {
     const int64_t ne = event_offsets[e + 1] - event_offsets[e];
@@ -1783,55 +1813,93 @@ clue_status cluster_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
   }
   if (total_tiles >= (int64_t{1} << 30))
     return fail(CLUE_ERR_UNSUPPORTED, "more than 2^30 tiles in one call");
-  const int32_t Ttot = static_cast<int32_t>(total_tiles);
-  const int32_t n_chunks = static_cast<int32_t>(n_chunks64);
-  const size_t table_bytes = static_cast<size_t>(n_events) * sizeof(EventDesc) +
-                             static_cast<size_t>(n_chunks) * sizeof(Chunk);
-  if (p->stage_done) CLUE_HIP(hipEventSynchronize(p->stage_done));  // previous upload finished
-  if (p->stage_bytes < table_bytes) {
-    if (p->stage) CLUE_HIP(hipHostFree(p->stage));
-    p->stage = nullptr;
-    p->stage_bytes = 0;
-    CLUE_HIP(hipHostMalloc(&p->stage, table_bytes, hipHostMallocDefault));
-    p->stage_bytes = table_bytes;
-  }
-  if (!p->stage_done) CLUE_HIP(hipEventCreateWithFlags(&p->stage_done, hipEventDisableTiming));
-  auto* hev = static_cast<EventDesc*>(p->stage);
-  auto* hch = reinterpret_cast<Chunk*>(hev + n_events);
-  {
-    int32_t tile_base = 0, chunk = 0;
-    for (int e = 0; e < n_events; ++e) {
-      EventDesc& ed = hev[e];
-      std::memset(&ed, 0, sizeof(ed));
-      ed.begin = static_cast<int32_t>(event_offsets[e]);
-      ed.n = static_cast<int32_t>(event_offsets[e + 1] - event_offsets[e]);
-      grid_shape(D, ed.n, prm->points_per_tile, ed.npd, ed.ntiles);
-      ed.tile_base = tile_base;
-      tile_base += ed.ntiles;
-      int32_t s1 = 1;
-      for (int d = D - 1; d >= 0; --d) {
-        ed.stride[d] = s1;
-        s1 *= ed.npd;
-      }
-      ed.chunk_begin = chunk;
-      for (int32_t b = ed.begin; b < ed.begin + ed.n; b += kChunk)
-        hch[chunk++] = Chunk{e, b, std::min(b + kChunk, ed.begin + ed.n), 0};
-      ed.chunk_end = chunk;
-    }
-  }
-  rc = reserve(p, D, n, Ttot, n_events, n_chunks);
-  if (rc != CLUE_OK) return rc;
-  CLUE_HIP(hipMemcpyAsync(p->events.p, hev, static_cast<size_t>(n_events) * sizeof(EventDesc),
-                          hipMemcpyHostToDevice, st));
-  CLUE_HIP(hipMemcpyAsync(p->chunks.p, hch, static_cast<size_t>(n_chunks) * sizeof(Chunk),
-                          hipMemcpyHostToDevice, st));
-  CLUE_HIP(hipEventRecord(p->stage_done, st));
+  bc.Ttot = static_cast<int32_t>(total_tiles);
+  bc.n_chunks = static_cast<int32_t>(n_chunks64);
+  return CLUE_OK;
+}
 
-  Coords c{};
-  for (int d = 0; d < D; ++d) c.p[d] = coords[d];
-  float* rho_buf = rho_out ? rho_out : p->rho_tmp.as<float>();
-  int32_t* nh_buf = nh_out ? nh_out : p->nh_tmp.as<int32_t>();
-  int32_t* seed_buf = is_seed_out ? is_seed_out : p->seed_tmp.as<int32_t>();
+size_t table_bytes(const BatchCall& bc) {
+  return static_cast<size_t>(bc.n_events) * sizeof(EventDesc) + static_cast<size_t>(bc.n_chunks) * sizeof(Chunk);
+}
+
+// Event descriptors (grid shape, tile base, integer strides, chunk range) and
+// the chunk table, written to host memory laid out as [events | chunks].
+void fill_tables(const BatchCall& bc, const clue_params* prm, const int64_t* event_offsets, void* host) {
+  auto* hev = static_cast<EventDesc*>(host);
+  auto* hch = reinterpret_cast<Chunk*>(hev + bc.n_events);
+  int32_t tile_base = 0, chunk = 0;
+  for (int e = 0; e < bc.n_events; ++e) {
+    EventDesc& ed = hev[e];
+    std::memset(&ed, 0, sizeof(ed));
+    ed.begin = static_cast<int32_t>(event_offsets[e]);
+    ed.n = static_cast<int32_t>(event_offsets[e + 1] - event_offsets[e]);
+    grid_shape(bc.D, ed.n, prm->points_per_tile, ed.npd, ed.ntiles);
+    ed.tile_base = tile_base;
+    tile_base += ed.ntiles;
+    int32_t s1 = 1;
+    for (int d = bc.D - 1; d >= 0; --d) {
+      ed.stride[d] = s1;
+      s1 *= ed.npd;
+    }
+    ed.chunk_begin = chunk;
+    for (int32_t b = ed.begin; b < ed.begin + ed.n; b += kChunk)
+      hch[chunk++] = Chunk{e, b, std::min(b + kChunk, ed.begin + ed.n), 0};
+    ed.chunk_end = chunk;
+  }
+}
+
+clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm, const BatchCall& bc);
+
+// Validate, upload the tables through the plan's pinned staging ring (a slot is
+// reused once its previous copy has run) and enqueue the kernels.
+clue_status cluster_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm, int32_t D,
+                          int32_t n_events, const int64_t* event_offsets,
+                          const float* const* coords, const float* weights,
+                          const clue_kernel_desc* kernel, const clue_metric_desc* metric,
+                          const uint8_t* wrapped, int32_t* cluster_out, int32_t* is_seed_out,
+                          float* rho_out, int32_t* nh_out, bool persistent_capacity) {
+  BatchCall bc;
+  clue_status rc = describe_batch(prm, D, n_events, event_offsets, coords, weights, kernel, metric, wrapped,
+                                  cluster_out, is_seed_out, rho_out, nh_out, persistent_capacity, bc);
+  if (rc != CLUE_OK) return rc;
+  const size_t tb = table_bytes(bc);
+  clue_plan_s::StageSlot& slot = p->stage[p->stage_next];
+  p->stage_next = (p->stage_next + 1) % clue_plan_s::kStageSlots;
+  if (slot.done) CLUE_HIP(hipEventSynchronize(slot.done));
+  if (slot.bytes < tb) {
+    if (slot.host) CLUE_HIP(hipHostFree(slot.host));
+    slot.host = nullptr;
+    slot.bytes = 0;
+    CLUE_HIP(hipHostMalloc(&slot.host, tb, hipHostMallocDefault));
+    slot.bytes = tb;
+  }
+  if (!slot.done) CLUE_HIP(hipEventCreateWithFlags(&slot.done, hipEventDisableTiming));
+  fill_tables(bc, prm, event_offsets, slot.host);
+  rc = reserve(p, D, bc.n, bc.Ttot, bc.n_events, bc.n_chunks);
+  if (rc != CLUE_OK) return rc;
+  CLUE_HIP(hipMemcpyAsync(p->events.p, slot.host, static_cast<size_t>(bc.n_events) * sizeof(EventDesc),
+                          hipMemcpyHostToDevice, st));
+  CLUE_HIP(hipMemcpyAsync(p->chunks.p, static_cast<EventDesc*>(slot.host) + bc.n_events,
+                          static_cast<size_t>(bc.n_chunks) * sizeof(Chunk), hipMemcpyHostToDevice, st));
+  CLUE_HIP(hipEventRecord(slot.done, st));
+  return enqueue_batch(p, st, prm, bc);
+}
+
+// The kernels of one call; the event and chunk tables are already on the device.
+clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm, const BatchCall& bc) {
+  const int32_t D = bc.D, n = bc.n, n_events = bc.n_events, n_chunks = bc.n_chunks, Ttot = bc.Ttot;
+  const clue_kernel_desc& kd = bc.kd;
+  const clue_metric_desc& md = bc.md;
+  const Wrap& wrap = bc.wrap;
+  const bool wrapped_any = bc.wrapped_any;
+  const bool persistent_capacity = bc.persistent;
+  const Coords& c = bc.c;
+  const float* weights = bc.weights;
+  int32_t* cluster_out = bc.cluster_out;
+  clue_status rc = CLUE_OK;
+  float* rho_buf = bc.rho_out ? bc.rho_out : p->rho_tmp.as<float>();
+  int32_t* nh_buf = bc.nh_out ? bc.nh_out : p->nh_tmp.as<int32_t>();
+  int32_t* seed_buf = bc.is_seed_out ? bc.is_seed_out : p->seed_tmp.as<int32_t>();
   int32_t* counters = p->counters.as<int32_t>();
 
   // ---- bounding boxes and tile sizes
@@ -1978,7 +2046,9 @@ clue_status cluster_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
   if (rc != CLUE_OK) return rc;
   rc = scan_exclusive(p->flag.as<int32_t>(), p->flag.as<int32_t>(), n, p->part.as<int32_t>(), st);
   if (rc != CLUE_OK) return rc;
-  k_fill_i32<<<1, 64, 0, st>>>(counters, 0, 1);      // totals of this call; [1] carries over
+  // totals of this call; [1], the grow-only capacity, carries over unless reset
+  k_fill_i32<<<1, 64, 0, st>>>(counters, 0, p->reset_capacity && persistent_capacity ? 2 : 1);
+  if (persistent_capacity) p->reset_capacity = false;
   k_fill_i32<<<1, 64, 0, st>>>(counters + 2, 0, 6);
   CLUE_LAUNCH_CHECK("fill");
   k_capacity<<<div_up(n_events, kThreads), kThreads, 0, st>>>(
@@ -2021,6 +2091,14 @@ clue_status make_clusters(clue_plan_s* p, hipStream_t st, const clue_params* prm
   return cluster_batch(p, st, prm, D, 1, offsets, coords, weights, kernel, metric, wrapped, cluster_out,
                        is_seed_out, rho_out, nh_out, true);
 }
+
+// A captured batched call: its own plan (scratch, tables) and the instantiated
+// graph of the call's kernels.
+struct BatchGraph {
+  clue_plan_s* plan = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
 
 }  // namespace clue_hip
 
@@ -2129,9 +2207,11 @@ clue_status clue_plan_destroy(clue_plan_t plan) {
   (void)hipSetDevice(plan->device);
   if (plan->has_run && plan->last_stream) (void)hipStreamSynchronize(plan->last_stream);
   for (int k = 0; DevBuf* b = plan->all_buffers(k); ++k) b->release();
-  if (plan->stage_done) (void)hipEventSynchronize(plan->stage_done);
-  if (plan->stage) (void)hipHostFree(plan->stage);
-  if (plan->stage_done) (void)hipEventDestroy(plan->stage_done);
+  for (auto& slot : plan->stage) {
+    if (slot.done) (void)hipEventSynchronize(slot.done);
+    if (slot.host) (void)hipHostFree(slot.host);
+    if (slot.done) (void)hipEventDestroy(slot.done);
+  }
   for (hipEvent_t& e : plan->ev)
     if (e) (void)hipEventDestroy(e);
   delete plan;
@@ -2140,11 +2220,7 @@ clue_status clue_plan_destroy(clue_plan_t plan) {
 
 clue_status clue_plan_reset_capacity(clue_plan_t plan) {
   if (!plan) return fail(CLUE_ERR_INVALID_ARGUMENT, "plan is NULL");
-  if (plan->counters.p) {
-    CLUE_HIP(hipSetDevice(plan->device));
-    CLUE_HIP(hipDeviceSynchronize());
-    CLUE_HIP(hipMemset(plan->counters.p, 0, 16 * 4));
-  }
+  plan->reset_capacity = true;  // applied in stream order by the next plain call
   return CLUE_OK;
 }
 
@@ -2332,6 +2408,193 @@ clue_status clue_selftest_math(int32_t op, const float* a_dev, const float* b_de
   k_selftest<<<div_up(n, kThreads), kThreads, 0, static_cast<hipStream_t>(stream)>>>(op, a_dev, b_dev,
                                                                                    out_dev, n);
   CLUE_LAUNCH_CHECK("selftest");
+  return CLUE_OK;
+}
+
+clue_status clue_batch_graph_create(int32_t device, const clue_params* params, int32_t ndim,
+                                    int32_t n_events, const int64_t* event_offsets,
+                                    const float* const* coords_dev, const float* weights_dev,
+                                    const clue_kernel_desc* kernel, const clue_metric_desc* metric,
+                                    const uint8_t* wrapped, int32_t* cluster_out, int32_t* is_seed_out,
+                                    float* rho_out, int32_t* nearest_higher_out, const clue_tuning* tuning,
+                                    clue_batch_graph_t* graph) {
+  if (!graph) return fail(CLUE_ERR_INVALID_ARGUMENT, "graph is NULL");
+  *graph = nullptr;
+  BatchCall bc;
+  clue_status rc = describe_batch(params, ndim, n_events, event_offsets, coords_dev, weights_dev, kernel, metric,
+                                  wrapped, cluster_out, is_seed_out, rho_out, nearest_higher_out, false, bc);
+  if (rc != CLUE_OK) return rc;
+  clue_plan_t plan = nullptr;
+  rc = clue_plan_create(device, &plan);
+  if (rc != CLUE_OK) return rc;
+  auto bg = std::make_unique<BatchGraph>();
+  bg->plan = plan;
+  auto cleanup = [&]() {
+    const std::string keep = g_error;
+    if (bg->exec) (void)hipGraphExecDestroy(bg->exec);
+    if (bg->graph) (void)hipGraphDestroy(bg->graph);
+    clue_plan_destroy(bg->plan);
+    g_error = keep;
+  };
+  if (tuning) rc = clue_plan_set_tuning(plan, tuning);
+  if (rc == CLUE_OK) rc = reserve(plan, ndim, bc.n, bc.Ttot, bc.n_events, bc.n_chunks);
+  if (rc != CLUE_OK) {
+    cleanup();
+    return rc;
+  }
+  // The tables live in the graph's own plan: upload them once, before capture.
+  std::vector<unsigned char> host(table_bytes(bc));
+  fill_tables(bc, params, event_offsets, host.data());
+  hipStream_t cs = nullptr;
+  hipError_t e = hipMemcpy(plan->events.p, host.data(), static_cast<size_t>(bc.n_events) * sizeof(EventDesc),
+                           hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(plan->chunks.p, host.data() + static_cast<size_t>(bc.n_events) * sizeof(EventDesc),
+                  static_cast<size_t>(bc.n_chunks) * sizeof(Chunk), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    if (cs) (void)hipStreamDestroy(cs);
+    cleanup();
+    return fail(CLUE_ERR_RUNTIME, std::string("graph capture: ") + hipGetErrorString(e));
+  }
+  rc = enqueue_batch(plan, cs, params, bc);
+  e = hipStreamEndCapture(cs, &bg->graph);
+  (void)hipStreamDestroy(cs);
+  if (rc == CLUE_OK && e != hipSuccess) rc = fail(CLUE_ERR_RUNTIME, std::string("graph capture: ") + hipGetErrorString(e));
+  if (rc == CLUE_OK) {
+    e = hipGraphInstantiate(&bg->exec, bg->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) rc = fail(CLUE_ERR_RUNTIME, std::string("graph instantiate: ") + hipGetErrorString(e));
+  }
+  if (rc != CLUE_OK) {
+    cleanup();
+    return rc;
+  }
+  plan->has_run = true;
+  plan->last_n = bc.n;
+  plan->last_d = ndim;
+  plan->last_T = bc.Ttot;
+  plan->last_events = bc.n_events;
+  *graph = reinterpret_cast<clue_batch_graph_t>(bg.release());
+  return CLUE_OK;
+}
+
+clue_status clue_batch_graph_launch(clue_batch_graph_t graph, void* stream) {
+  if (!graph) return fail(CLUE_ERR_INVALID_ARGUMENT, "graph is NULL");
+  auto* bg = reinterpret_cast<BatchGraph*>(graph);
+  CLUE_HIP(hipSetDevice(bg->plan->device));
+  CLUE_HIP(hipGraphLaunch(bg->exec, static_cast<hipStream_t>(stream)));
+  bg->plan->last_stream = static_cast<hipStream_t>(stream);
+  return CLUE_OK;
+}
+
+clue_status clue_batch_graph_destroy(clue_batch_graph_t graph) {
+  if (!graph) return CLUE_OK;
+  auto* bg = reinterpret_cast<BatchGraph*>(graph);
+  (void)hipSetDevice(bg->plan->device);
+  (void)hipDeviceSynchronize();
+  if (bg->exec) (void)hipGraphExecDestroy(bg->exec);
+  if (bg->graph) (void)hipGraphDestroy(bg->graph);
+  bg->plan->has_run = false;
+  clue_plan_destroy(bg->plan);
+  delete bg;
+  return CLUE_OK;
+}
+
+// Coordinate descent over the launch tuning (and, when asked, the points per
+// tile) on the given batch: every candidate is run once untimed and three times
+// timed with HIP events on `stream`; the fastest median wins.
+clue_status clue_autotune_batched(clue_plan_t plan, void* stream, clue_params* params, int32_t ndim,
+                                  int32_t n_events, const int64_t* event_offsets,
+                                  const float* const* coords_dev, const float* weights_dev,
+                                  const clue_kernel_desc* kernel, const clue_metric_desc* metric,
+                                  const uint8_t* wrapped, int32_t flags, clue_tuning* best, float* best_ms) {
+  if (!plan || !params || !best || !event_offsets || n_events <= 0)
+    return fail(CLUE_ERR_INVALID_ARGUMENT, "NULL argument");
+  CLUE_HIP(hipSetDevice(plan->device));
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  const int64_t n = event_offsets[n_events];
+  if (n <= 0 || n > INT32_MAX) return fail(CLUE_ERR_INVALID_ARGUMENT, "invalid event offsets");
+  int32_t* labels = nullptr;
+  CLUE_HIP(hipMalloc(&labels, static_cast<size_t>(n) * 4));
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  hipError_t he = hipEventCreate(&t0);
+  if (he == hipSuccess) he = hipEventCreate(&t1);
+  const clue_tuning saved = plan->tuning;
+  clue_params prm = *params;
+  clue_status rc = he == hipSuccess ? CLUE_OK : fail(CLUE_ERR_RUNTIME, hipGetErrorString(he));
+  auto run_once = [&](float* ms) -> clue_status {
+    if (ms && hipEventRecord(t0, st) != hipSuccess) return fail(CLUE_ERR_RUNTIME, "event record");
+    clue_status r = cluster_batch(plan, st, &prm, ndim, n_events, event_offsets, coords_dev, weights_dev, kernel,
+                                  metric, wrapped, labels, nullptr, nullptr, nullptr, false);
+    if (r != CLUE_OK) return r;
+    if (ms) {
+      if (hipEventRecord(t1, st) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
+          hipEventElapsedTime(ms, t0, t1) != hipSuccess)
+        return fail(CLUE_ERR_RUNTIME, "event timing");
+    }
+    return CLUE_OK;
+  };
+  auto measure = [&](float* ms) -> clue_status {
+    clue_status r = run_once(nullptr);
+    float v[3];
+    for (int k = 0; r == CLUE_OK && k < 3; ++k) r = run_once(&v[k]);
+    if (r != CLUE_OK) return r;
+    std::sort(v, v + 3);
+    *ms = v[1];
+    return CLUE_OK;
+  };
+  clue_tuning cur = saved;
+  float cur_ms = 0.f;
+  if (rc == CLUE_OK) rc = measure(&cur_ms);
+  // points per tile (changes the grid and hence the results' summation order)
+  if (rc == CLUE_OK && (flags & CLUE_TUNE_POINTS_PER_TILE)) {
+    const int32_t keep = prm.points_per_tile;
+    int32_t best_ppb = keep;
+    for (int32_t ppb : {32, 64, 128, 256, 512}) {
+      if (ppb == keep || rc != CLUE_OK) continue;
+      prm.points_per_tile = ppb;
+      float ms;
+      rc = measure(&ms);
+      if (rc == CLUE_OK && ms < cur_ms) {
+        cur_ms = ms;
+        best_ppb = ppb;
+      }
+    }
+    prm.points_per_tile = best_ppb;
+  }
+  auto try_field = [&](auto setter, std::initializer_list<float> values) {
+    for (float v : values) {
+      if (rc != CLUE_OK) return;
+      clue_tuning t = cur;
+      setter(t, v);
+      if (std::memcmp(&t, &cur, sizeof(t)) == 0) continue;
+      plan->tuning = t;
+      float ms;
+      rc = measure(&ms);
+      if (rc == CLUE_OK && ms < cur_ms) {
+        cur_ms = ms;
+        cur = t;
+      }
+    }
+    plan->tuning = cur;
+  };
+  auto set_radius = [](clue_tuning& t, float v) { t.nh_first_radius = v; };
+  auto set_xcd = [](clue_tuning& t, float v) { t.xcd_remap = static_cast<int32_t>(v); };
+  auto set_mask = [](clue_tuning& t, float v) { t.density_bitmask = static_cast<int32_t>(v); };
+  try_field(set_radius, {0.08f, 0.12f, 0.18f, 0.25f, 0.35f, 0.5f});
+  try_field(set_xcd, {0.f, 1.f, 2.f, 3.f});
+  try_field(set_mask, {0.f, 1.f});
+  try_field(set_radius, {cur.nh_first_radius * 0.8f, cur.nh_first_radius * 1.25f});
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(labels);
+  if (t0) (void)hipEventDestroy(t0);
+  if (t1) (void)hipEventDestroy(t1);
+  plan->tuning = saved;
+  if (rc != CLUE_OK) return rc;
+  *best = cur;
+  params->points_per_tile = prm.points_per_tile;
+  if (best_ms) *best_ms = cur_ms;
   return CLUE_OK;
 }
 

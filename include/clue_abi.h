@@ -117,6 +117,10 @@ typedef struct {
 } clue_tuning;
 
 typedef struct clue_plan_s* clue_plan_t;
+typedef struct clue_batch_graph_s* clue_batch_graph_t;
+
+/* clue_autotune_batched flags */
+#define CLUE_TUNE_POINTS_PER_TILE 1
 
 /* ---- errors ------------------------------------------------------------- */
 const char* clue_last_error_message(void);
@@ -195,6 +199,34 @@ clue_status clue_make_clusters_batched(clue_plan_t plan, void* stream, const clu
                                        const clue_kernel_desc* kernel, const clue_metric_desc* metric,
                                        const uint8_t* wrapped, int32_t* cluster_out, int32_t* is_seed_out,
                                        float* rho_out, int32_t* nearest_higher_out);
+
+/* A batched call captured once as a HIP graph and replayed on any stream (no
+ * reference counterpart; the reference launches every kernel of every call).
+ * The graph owns its scratch.  Replays read the coordinate and weight buffers
+ * given here and write the given outputs: their values may change between
+ * replays, the buffers and the event offsets may not.  tuning may be NULL. */
+clue_status clue_batch_graph_create(int32_t device, const clue_params* params, int32_t ndim,
+                                    int32_t n_events, const int64_t* event_offsets,
+                                    const float* const* coords_dev, const float* weights_dev,
+                                    const clue_kernel_desc* kernel, const clue_metric_desc* metric,
+                                    const uint8_t* wrapped, int32_t* cluster_out, int32_t* is_seed_out,
+                                    float* rho_out, int32_t* nearest_higher_out, const clue_tuning* tuning,
+                                    clue_batch_graph_t* graph);
+clue_status clue_batch_graph_launch(clue_batch_graph_t graph, void* stream);
+clue_status clue_batch_graph_destroy(clue_batch_graph_t graph);
+
+/* Autotuner (no reference counterpart): times batched calls on the given
+ * events and returns the fastest launch tuning in *best (results do not depend
+ * on it) and its median time in *best_ms.  With CLUE_TUNE_POINTS_PER_TILE in
+ * flags it also picks params->points_per_tile (the reference's pPBin, which
+ * does change the tile grid and so the summation order of the densities:
+ * results then equal the reference's at the chosen value).  The plan's own
+ * tuning is left unchanged. */
+clue_status clue_autotune_batched(clue_plan_t plan, void* stream, clue_params* params, int32_t ndim,
+                                  int32_t n_events, const int64_t* event_offsets,
+                                  const float* const* coords_dev, const float* weights_dev,
+                                  const clue_kernel_desc* kernel, const clue_metric_desc* metric,
+                                  const uint8_t* wrapped, int32_t flags, clue_tuning* best, float* best_ms);
 
 /* Host-buffer convenience: the reference's Python FFI entry
  * mainRun(dc, rhoc, dm, seed_dc, pPBin, wrapped, data, results, kernel, Ndim,
