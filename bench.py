@@ -44,7 +44,110 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-events measurement")
     ap.add_argument("--batch-events", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--no-served", action="store_true", help="skip the served-requests measurement")
+    ap.add_argument("--served-events", type=int, default=600)
+    ap.add_argument("--served-load", type=float, default=0.7,
+                    help="Poisson arrival rate as a fraction of the measured saturated throughput")
     return ap.parse_args()
+
+
+def served_bench(clue, datasets, args):
+    """A stream of independent requests (70% 2D / 30% 3D blob events, sizes log-uniform
+    in [1k, 200k], exponential weights; each with its workload's parameters) through the
+    server, against sequential plain launches (copy in, cluster, copy out per event)."""
+    events = [datasets.served_event(k, 31) for k in range(args.served_events)]
+    M = len(events)
+    npts = sum(ps.n for ps, _ in events)
+
+    # sequential plain launches on preallocated buffers and one plan
+    nmax = max(ps.n for ps, _ in events)
+    stream = clue.Stream(0)
+    plan = clue.Plan(0)
+    dbuf = clue.DeviceArray((4 * nmax,), np.float32)
+    dout = clue.DeviceArray((nmax,), np.int32)
+    md = clue.metric_desc("euclidean")
+    lib = clue.lib()
+    plain_labels, plain_lat = [], []
+
+    def plain(ps, prm):
+        D, n = ps.ndim, ps.n
+        t0 = time.perf_counter()
+        clue.check(lib.clue_memcpy_h2d(dbuf.ptr, ps.coords.ctypes.data, D * n * 4, stream.handle))
+        clue.check(lib.clue_memcpy_h2d(dbuf.offset(D * n), ps.weights.ctypes.data, n * 4, stream.handle))
+        plan.reset_capacity()
+        plan.make_clusters_device(stream, clue.make_params(prm["dc"], prm["rhoc"], prm["dm"]), D, n,
+                                  [dbuf.offset(d * n) for d in range(D)], dbuf.offset(D * n),
+                                  clue.kernel_desc(prm["kernel"]), md, None, dout.ptr)
+        out = np.empty(n, np.int32)
+        clue.check(lib.clue_memcpy_d2h(out.ctypes.data, dout.ptr, n * 4, stream.handle))
+        stream.synchronize()
+        return out, time.perf_counter() - t0
+
+    plain(*events[0])
+    t0 = time.perf_counter()
+    for ps, prm in events:
+        out, dt = plain(ps, prm)
+        plain_labels.append(out)
+        plain_lat.append(dt)
+    plain_s = time.perf_counter() - t0
+    plan.close()
+    dbuf.free()
+    dout.free()
+
+    srv = clue.Server(0)
+
+    def submit(ps, prm):
+        return srv.submit(ps.coords, ps.weights, prm["dc"], prm["rhoc"], prm["dm"], kernel=prm["kernel"],
+                          outputs="labels")
+
+    # saturated: every request queued at once
+    submit(*events[0]).result()
+    t0 = time.perf_counter()
+    tickets = [submit(ps, prm) for ps, prm in events]
+    res = [t.result() for t in tickets]
+    sat_s = time.perf_counter() - t0
+    same = all(np.array_equal(r.cluster, l) for r, l in zip(res, plain_labels))
+    sat_rate = M / sat_s
+
+    # Poisson arrivals at a fraction of the saturated rate
+    rate = args.served_load * sat_rate
+    gaps = np.random.default_rng(7).exponential(1.0 / rate, M)
+    arrivals = np.cumsum(gaps)
+    tickets = []
+    t0 = time.perf_counter()
+    for (ps, prm), ta in zip(events, arrivals):
+        while True:
+            dt = t0 + ta - time.perf_counter()
+            if dt <= 0:
+                break
+            time.sleep(min(dt, 0.002))
+        tickets.append(submit(ps, prm))
+    for t in tickets:
+        t.result()
+    lat = np.array([(t.timing["done_ns"] - t.timing["submit_ns"]) * 1e-6 for t in tickets])
+    span = (max(t.timing["done_ns"] for t in tickets) - min(t.timing["submit_ns"] for t in tickets)) * 1e-9
+    batch_sizes = np.array([t.timing["batch_events"] for t in tickets])
+    st = srv.stats()
+    srv.close()
+    pl = np.array(plain_lat) * 1e3
+    return {
+        "workload": f"{M} requests, 70% 2D (dc=2 rhoc=8 dm=4 Gaussian) / 30% 3D (dc=1.5 rhoc=6 dm=3 "
+                    f"Exponential) blob events, sizes log-uniform in [1k, 200k], exponential weights",
+        "events": M, "points": npts,
+        "saturated_events_per_s": round(sat_rate, 1),
+        "plain_events_per_s": round(M / plain_s, 1),
+        "speedup_vs_plain": round(plain_s / sat_s, 2),
+        "plain_latency_ms": {"p50": round(float(np.percentile(pl, 50)), 3),
+                             "p99": round(float(np.percentile(pl, 99)), 3)},
+        "poisson": {"offered_events_per_s": round(rate, 1), "achieved_events_per_s": round(M / span, 1),
+                    "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                                   "p99": round(float(np.percentile(lat, 99)), 3)},
+                    "mean_batch_events": round(float(batch_sizes.mean()), 2)},
+        "labels_equal_plain": bool(same),
+        "server": {"slots": 3, "budget_bytes": 32 << 30, "slot_capacity_points": st["slot_capacity_points"],
+                   "device_bytes": st["device_bytes"], "batches": st["batches"],
+                   "max_batch_events": st["max_batch_events"]},
+    }
 
 
 def batched_bench(clue, datasets, args):
@@ -299,6 +402,8 @@ def main():
     }
     if not args.no_batched:
         result["batched"] = batched_bench(clue, datasets, args)
+    if not args.no_served:
+        result["served"] = served_bench(clue, datasets, args)
     if not args.no_cpu_baseline:
         cb, parity = cpu_baseline(ps, prm, args, labels)
         result["cpu_baseline"] = cb

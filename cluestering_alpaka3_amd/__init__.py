@@ -82,6 +82,34 @@ class ClueGridInfo(ctypes.Structure):
                 ("tile_size", ctypes.c_float * MAX_DIM)]
 
 
+class ClueServerConfig(ctypes.Structure):
+    _fields_ = [("device_budget_bytes", ctypes.c_size_t), ("slots", ctypes.c_int32),
+                ("max_batch_points", ctypes.c_int32), ("max_batch_events", ctypes.c_int32),
+                ("max_dim", ctypes.c_int32)]
+
+
+class ClueRequest(ctypes.Structure):
+    _fields_ = [("ndim", ctypes.c_int32), ("n", ctypes.c_int32), ("coords", ctypes.c_void_p),
+                ("weights", ctypes.c_void_p), ("params", ClueParams), ("kernel", ClueKernelDesc),
+                ("metric", ClueMetricDesc), ("wrapped", ctypes.c_uint8 * MAX_DIM),
+                ("cluster_out", ctypes.c_void_p), ("is_seed_out", ctypes.c_void_p), ("rho_out", ctypes.c_void_p),
+                ("nearest_higher_out", ctypes.c_void_p)]
+
+
+class ClueRequestTiming(ctypes.Structure):
+    _fields_ = [("submit_ns", ctypes.c_int64), ("launch_ns", ctypes.c_int64), ("done_ns", ctypes.c_int64),
+                ("batch_events", ctypes.c_int32)]
+
+
+class ClueServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in ("events_submitted", "events_done", "points_done", "batches",
+                                              "max_batch_events", "max_batch_points", "slot_capacity_points",
+                                              "device_bytes")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 # every symbol include/clue_abi.h declares, with its ctypes signature
 _P = ctypes.POINTER
 _vp = ctypes.c_void_p
@@ -120,6 +148,11 @@ _SIGNATURES = {
     "clue_autotune_batched": ([_vp, _vp, _P(ClueParams), ctypes.c_int32, ctypes.c_int32, _P(ctypes.c_int64),
                                _P(_vp), _vp, _P(ClueKernelDesc), _P(ClueMetricDesc), _P(ctypes.c_uint8),
                                ctypes.c_int32, _P(ClueTuning), _P(ctypes.c_float)], ctypes.c_int),
+    "clue_server_create": ([ctypes.c_int32, _P(ClueServerConfig), _P(_vp)], ctypes.c_int),
+    "clue_server_submit": ([_vp, _P(ClueRequest), _P(ctypes.c_uint64)], ctypes.c_int),
+    "clue_server_wait": ([_vp, ctypes.c_uint64, _P(ClueRequestTiming)], ctypes.c_int),
+    "clue_server_get_stats": ([_vp, _P(ClueServerStats)], ctypes.c_int),
+    "clue_server_destroy": ([_vp], ctypes.c_int),
     "clue_main_run": ([ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int32,
                        _P(ctypes.c_uint8), _P(ctypes.c_float), _P(ctypes.c_int32), _P(ClueKernelDesc),
                        ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32], ctypes.c_int),
@@ -441,6 +474,81 @@ class BatchGraph:
         if self.handle:
             lib().clue_batch_graph_destroy(self.handle)
             self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Served clustering (clue_server_*): submit() queues one event and returns a
+    Ticket; Ticket.result() blocks until it is done.  Requests of one class
+    (dimension, parameters, kernel, metric) are batched continuously on the GPU."""
+
+    class Ticket:
+        def __init__(self, server, ticket, keep, outputs):
+            self.server, self.ticket, self._keep, self.outputs = server, ticket, keep, outputs
+            self.timing = None
+
+        def result(self):
+            t = ClueRequestTiming()
+            try:
+                check(lib().clue_server_wait(self.server.handle, self.ticket, ctypes.byref(t)))
+            finally:
+                self.server._outstanding.pop(self.ticket, None)
+            self.timing = {"submit_ns": t.submit_ns, "launch_ns": t.launch_ns, "done_ns": t.done_ns,
+                           "batch_events": t.batch_events}
+            self._keep = None
+            return Result(*self.outputs)
+
+    def __init__(self, device=0, budget_bytes=0, slots=0, max_batch_points=0, max_batch_events=0, max_dim=0):
+        cfg = ClueServerConfig(budget_bytes, slots, max_batch_points, max_batch_events, max_dim)
+        h = _vp()
+        check(lib().clue_server_create(device, ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h.value
+        self._outstanding = {}  # ticket -> host arrays the server still reads or writes
+
+    def submit(self, coords, weights, dc, rhoc, dm=None, seed_dc=None, ppb=128, kernel=("flat", 0.5),
+               metric="euclidean", metric_params=None, wrapped=None, seed_mode=SEEDS_REFERENCE, outputs="all"):
+        c = np.ascontiguousarray(coords, dtype=np.float32)
+        if c.ndim == 1:
+            c = c[None, :]
+        ndim, n = c.shape
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        r = ClueRequest()
+        r.ndim, r.n = ndim, n
+        r.coords, r.weights = c.ctypes.data, w.ctypes.data
+        r.params = make_params(dc, rhoc, dm, seed_dc, ppb, seed_mode)
+        r.kernel = kernel_desc(kernel)
+        r.metric = metric_desc(metric, metric_params)
+        for d, v in enumerate(wrapped or []):
+            r.wrapped[d] = int(v)
+        cl = np.empty(n, np.int32)
+        full = outputs == "all"
+        seed = np.empty(n, np.int32) if full else None
+        rho = np.empty(n, np.float32) if full else None
+        nh = np.empty(n, np.int32) if full else None
+        r.cluster_out = cl.ctypes.data
+        r.is_seed_out = seed.ctypes.data if full else None
+        r.rho_out = rho.ctypes.data if full else None
+        r.nearest_higher_out = nh.ctypes.data if full else None
+        t = ctypes.c_uint64(0)
+        check(lib().clue_server_submit(self.handle, ctypes.byref(r), ctypes.byref(t)))
+        self._outstanding[t.value] = (c, w, cl, seed, rho, nh)
+        return Server.Ticket(self, t.value, (c, w), (cl, seed, rho, nh))
+
+    def stats(self):
+        st = ClueServerStats()
+        check(lib().clue_server_get_stats(self.handle, ctypes.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if self.handle:
+            lib().clue_server_destroy(self.handle)  # finishes every submitted request first
+            self.handle = None
+            self._outstanding.clear()
 
     def __del__(self):
         try:

@@ -35,10 +35,10 @@ def run(cmd, cwd=None):
 
 
 def build_engine():
-    src = os.path.join(HERE, "csrc", "clue_engine.hip")
+    srcs = [os.path.join(HERE, "csrc", "clue_engine.hip"), os.path.join(HERE, "csrc", "clue_server.cpp")]
     out = os.path.join(HERE, "libclue_hip.so")
     tmp = out + ".tmp"
-    run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, src])
+    run([HIPCC, *HIP_FLAGS, "-pthread", "-shared", "-o", tmp, *srcs])
     os.replace(tmp, out)
     return out
 

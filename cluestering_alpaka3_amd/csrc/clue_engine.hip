@@ -1359,6 +1359,8 @@ clue_status fail(clue_status st, const std::string& msg) {
   return st;
 }
 
+void set_error(const std::string& msg) { g_error = msg; }
+
 #define CLUE_HIP(call)                                                                     \
   do {                                                                                     \
     hipError_t e_ = (call);                                                                \

@@ -165,3 +165,26 @@ def event_sizes(n_events, seed, median=5000.0, sigma_log=1.0, lo=500, hi=50000):
     rng = SplitMix64(seed)
     z = rng.normal(n_events)
     return np.clip(np.round(median * np.exp(sigma_log * z)), lo, hi).astype(np.int64)
+
+
+def served_event(k, seed, lo=1000, hi=200000, frac_2d=0.7):
+    """Event k of the served mix: 70% 2D, 30% 3D, sizes log-uniform in [lo, hi],
+    drawn from the 2D 1M-point and 3D 4M-point generators with the number of
+    blobs and the box scaled with the size (so local densities stay those of the
+    full workloads), exponential weights (mean 1).  Returns (PointSet, params)."""
+    rng = SplitMix64(seed * 1_000_003 + k)
+    u = rng.uniform(2)
+    n = int(np.floor(np.exp(np.log(lo) + u[0] * (np.log(hi) - np.log(lo)))))
+    if u[1] < frac_2d:
+        f = n / 1048576.0
+        side = 1000.0 * np.sqrt(f)
+        nb = max(1, int(round(200 * f)))
+        ps = blobs(n, 2, nb, 0.0, side, ("uniform", 1.0, 4.0), 0.20, ("exponential", 1.0),
+                   seed * 7 + k, "served_2d")
+        return ps, WORKLOADS["2d_1m_gauss"]
+    f = n / 4194304.0
+    side = 500.0 * f ** (1.0 / 3.0)
+    nb = max(1, int(round(1000 * f)))
+    ps = blobs(n, 3, nb, 0.0, side, ("uniform_axis", 0.5, 3.0), 0.15, ("exponential", 1.0), seed * 7 + k,
+               "served_3d")
+    return ps, WORKLOADS["3d_4m_exp"]

@@ -228,6 +228,69 @@ clue_status clue_autotune_batched(clue_plan_t plan, void* stream, clue_params* p
                                   const clue_kernel_desc* kernel, const clue_metric_desc* metric,
                                   const uint8_t* wrapped, int32_t flags, clue_tuning* best, float* best_ms);
 
+/* ---- served clustering (no reference counterpart: the reference clusters
+ *      one event per synchronous make_clusters call).  A server owns `slots`
+ *      batch slots (stream, plan, device and pinned buffers), sized once so
+ *      that all of them fit device_budget_bytes; a scheduler thread batches
+ *      pending requests of one class (dimension, parameters, kernel, metric,
+ *      wrapping) into a free slot in arrival order, and requests that arrive
+ *      while the slots are busy join the next batch.  Every request is
+ *      clustered exactly as a fresh Clusterer would cluster it alone. ------ */
+typedef struct clue_server_s* clue_server_t;
+
+typedef struct {
+  size_t device_budget_bytes; /* 0: 32 GiB */
+  int32_t slots;              /* batches in flight, 0: 3 */
+  int32_t max_batch_points;   /* per slot before the budget cap, 0: 8 Mi */
+  int32_t max_batch_events;   /* 0: 4096 */
+  int32_t max_dim;            /* largest dimension served, 0: 3 */
+} clue_server_config;
+
+/* One event.  coords: host (ndim, n) SoA, weights: host n floats; outputs:
+ * host arrays of n (cluster required, the others may be NULL).  The host
+ * buffers must stay valid until clue_server_wait returns for the ticket. */
+typedef struct {
+  int32_t ndim;
+  int32_t n;
+  const float* coords;
+  const float* weights;
+  clue_params params;
+  clue_kernel_desc kernel;
+  clue_metric_desc metric;
+  uint8_t wrapped[CLUE_MAX_DIM];
+  int32_t* cluster_out;
+  int32_t* is_seed_out;
+  float* rho_out;
+  int32_t* nearest_higher_out;
+} clue_request;
+
+typedef struct {
+  int64_t submit_ns; /* steady clock */
+  int64_t launch_ns;
+  int64_t done_ns;
+  int32_t batch_events; /* events in the request's batch */
+} clue_request_timing;
+
+typedef struct {
+  int64_t events_submitted;
+  int64_t events_done;
+  int64_t points_done;
+  int64_t batches;
+  int64_t max_batch_events;
+  int64_t max_batch_points;
+  int64_t slot_capacity_points;
+  int64_t device_bytes;
+} clue_server_stats;
+
+clue_status clue_server_create(int32_t device, const clue_server_config* config, clue_server_t* server);
+/* Queue one request (non-blocking); CLUE_ERR_OUT_OF_RANGE if it exceeds a slot. */
+clue_status clue_server_submit(clue_server_t server, const clue_request* request, uint64_t* ticket);
+/* Block until the request is done; its outputs are then in the caller's buffers. */
+clue_status clue_server_wait(clue_server_t server, uint64_t ticket, clue_request_timing* timing);
+clue_status clue_server_get_stats(clue_server_t server, clue_server_stats* stats);
+/* Finish every submitted request, then free the server. */
+clue_status clue_server_destroy(clue_server_t server);
+
 /* Host-buffer convenience: the reference's Python FFI entry
  * mainRun(dc, rhoc, dm, seed_dc, pPBin, wrapped, data, results, kernel, Ndim,
  * n_points, block_size, device_id), CLUEstering/BindingModules/serial/
