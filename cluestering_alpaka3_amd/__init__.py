@@ -153,6 +153,10 @@ _SIGNATURES = {
     "clue_server_wait": ([_vp, ctypes.c_uint64, _P(ClueRequestTiming)], ctypes.c_int),
     "clue_server_get_stats": ([_vp, _P(ClueServerStats)], ctypes.c_int),
     "clue_server_destroy": ([_vp], ctypes.c_int),
+    "clue_cluster_map": ([_vp, _vp, ctypes.c_int32, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "clue_cluster_centroids": ([_vp, _vp, ctypes.c_int32, _P(_vp), _vp, _vp, _vp, ctypes.c_int32, _vp],
+                               ctypes.c_int),
+    "clue_selftest_sqrt": ([ctypes.c_uint32, ctypes.c_uint64, _P(ctypes.c_uint64)], ctypes.c_int),
     "clue_main_run": ([ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int32,
                        _P(ctypes.c_uint8), _P(ctypes.c_float), _P(ctypes.c_int32), _P(ClueKernelDesc),
                        ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32], ctypes.c_int),
@@ -647,6 +651,61 @@ def cluster_events(coords, weights, event_offsets, dc, rhoc, dm=None, seed_dc=No
             plan.close()
 
 
+def cluster_map(labels, device=0, plan=None, stream=None):
+    """Cluster -> points map of a label array, built on the GPU (clue_cluster_map):
+    returns (offsets[n_clusters+1], indexes[clustered]); cluster sizes are
+    np.diff(offsets)."""
+    lab = np.ascontiguousarray(labels, dtype=np.int32)
+    n = lab.size
+    own_plan, own_stream = plan is None, stream is None
+    plan = plan or Plan(device)
+    stream = stream or Stream(device)
+    try:
+        dl = DeviceArray.from_host(lab, device)
+        off = DeviceArray((n + 1,), np.int32, device)
+        idx = DeviceArray((n,), np.int32, device)
+        cnt = DeviceArray((2,), np.int32, device)
+        check(lib().clue_cluster_map(plan.handle, stream.handle, n, dl.ptr, off.ptr, idx.ptr, cnt.ptr))
+        counts = cnt.download(stream)
+        return off.download(stream)[: counts[0] + 1], idx.download(stream)[: counts[1]]
+    finally:
+        if own_stream:
+            stream.close()
+        if own_plan:
+            plan.close()
+
+
+def cluster_centroids(coords, labels, device=0, plan=None, stream=None):
+    """(n_clusters, ndim) centroids on the GPU (clue_cluster_map + clue_cluster_centroids),
+    with the reference's serial float arithmetic."""
+    c = np.ascontiguousarray(coords, dtype=np.float32)
+    if c.ndim == 1:
+        c = c[None, :]
+    ndim, n = c.shape
+    lab = np.ascontiguousarray(labels, dtype=np.int32)
+    own_plan, own_stream = plan is None, stream is None
+    plan = plan or Plan(device)
+    stream = stream or Stream(device)
+    try:
+        dc_arr = DeviceArray.from_host(c, device)
+        dl = DeviceArray.from_host(lab, device)
+        off = DeviceArray((n + 1,), np.int32, device)
+        idx = DeviceArray((n,), np.int32, device)
+        cnt = DeviceArray((2,), np.int32, device)
+        out = DeviceArray((n, ndim), np.float32, device)
+        check(lib().clue_cluster_map(plan.handle, stream.handle, n, dl.ptr, off.ptr, idx.ptr, cnt.ptr))
+        cp = _coord_array([dc_arr.offset(d * n) for d in range(ndim)])
+        check(lib().clue_cluster_centroids(plan.handle, stream.handle, ndim, cp, off.ptr, idx.ptr, cnt.ptr, n,
+                                           out.ptr))
+        nc = int(cnt.download(stream)[0])
+        return out.download(stream)[:nc]
+    finally:
+        if own_stream:
+            stream.close()
+        if own_plan:
+            plan.close()
+
+
 def main_run(dc, rhoc, dm, seed_dc, ppbin, wrapped, data, results, kernel, ndim, n_points,
              block_size=256, device_id=0):
     """mainRun(dc, rhoc, dm, seed_dc, pPBin, wrapped, data, results, kernel, Ndim, n_points,
@@ -670,3 +729,10 @@ def selftest_math(op, a, b=None, device=0):
     out = DeviceArray(a.shape, np.float32, device)
     check(lib().clue_selftest_math(op, da.ptr, db.ptr if db else None, out.ptr, a.size, None))
     return out.download()
+
+
+def selftest_sqrt(first_bits, count):
+    """Mismatches of the kernels' fast sqrt against IEEE sqrtf over a range of float bit patterns."""
+    m = ctypes.c_uint64(0)
+    check(lib().clue_selftest_sqrt(first_bits, count, ctypes.byref(m)))
+    return m.value

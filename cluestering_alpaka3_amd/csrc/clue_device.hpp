@@ -147,6 +147,39 @@ __device__ __forceinline__ float engine_expf(float x) {
   return x != x ? x : y;
 }
 
+// engine_expf for arguments known to be <= 0 and not NaN (the Gaussian and
+// exponential kernels with finite positive width / decay): the same sequence
+// without the upper clamp and the NaN pass-through, hence the same bits.
+__device__ __forceinline__ float engine_expf_nonpos(float x) {
+  const double xd = static_cast<double>(__builtin_fmaxf(x, -110.0f));
+  const double kd = __builtin_rint(xd * 0x1.71547652b82fep+6);
+  const int k = static_cast<int>(kd);
+  double r = __builtin_fma(-kd, 0x1.62e42fefp-7, xd);
+  r = __builtin_fma(-kd, 0x1.473de6af278edp-40, r);
+  double p = __builtin_fma(r, 1.0 / 120.0, 1.0 / 24.0);
+  p = __builtin_fma(r, p, 1.0 / 6.0);
+  p = __builtin_fma(r, p, 0.5);
+  p = __builtin_fma(r * r, p, r);
+  const int j = k & 63;
+  const int m = (k - j) / 64;
+  const double t = kExp2Table[j];
+  return static_cast<float>(__builtin_ldexp(__builtin_fma(t, p, t), m));
+}
+
+// Correctly rounded sqrt of x in [2^-96, FLT_MAX]: the hardware estimate (within
+// one ulp) corrected by the signs of the residuals of its two neighbours -- the
+// IEEE sqrtf lowering without its small-input scaling and special-value paths.
+__device__ __forceinline__ float sqrt_rn_normal(float x) {
+  const float r = __builtin_amdgcn_sqrtf(x);
+  const float rm = __int_as_float(__float_as_int(r) - 1);
+  const float rp = __int_as_float(__float_as_int(r) + 1);
+  const float em = __builtin_fmaf(-rm, r, x);
+  const float ep = __builtin_fmaf(-rp, r, x);
+  float y = em <= 0.f ? rm : r;
+  y = ep > 0.f ? rp : y;
+  return y;
+}
+
 // Gaussian kernel argument -(a*a) / den as the correctly rounded float quotient,
 // computed as (float)((double)num * (1/den)) with 1/den rounded to double on the
 // host: a quotient of two floats lies at least ~2^-48 (relative) from any float
@@ -159,16 +192,19 @@ __device__ __forceinline__ float quotient_via_rcp(float num, double rcp_den) {
 // Convolutional kernels (core/detail/ConvolutionalKernel.hpp:16-65), j != i.
 enum { KFLAT = 0, KGAUSS = 1, KEXP = 2 };
 // rcp_den: 1 / ((2*sigma)*sigma) rounded to double (Gaussian only).
-template <int KERNEL>
+// NONPOS: the exp argument is known to be <= 0 and not NaN (checked on the host).
+template <int KERNEL, bool NONPOS = false>
 __device__ __forceinline__ float kernel_value(float dist, float p0, float p1, float p2, double rcp_den) {
   if constexpr (KERNEL == KFLAT) {
     return p0;
   } else if constexpr (KERNEL == KGAUSS) {
     const float a = dist - p0;
     const float num = (-a) * a;  // -(d-mu)*(d-mu), divided by (2*sigma)*sigma
-    return p2 * engine_expf(quotient_via_rcp(num, rcp_den));
+    const float x = quotient_via_rcp(num, rcp_den);
+    return p2 * (NONPOS ? engine_expf_nonpos(x) : engine_expf(x));
   } else {
-    return p1 * engine_expf((-p0) * dist);
+    const float x = (-p0) * dist;
+    return p1 * (NONPOS ? engine_expf_nonpos(x) : engine_expf(x));
   }
 }
 

@@ -484,6 +484,7 @@ struct PassArgs {
   int32_t* nh_out;           // nearest-higher: original order
   float* delta_out;          // nearest-higher distance: original order
   int xcd;                   // bit 0: XCD-aware block order of density, bit 1: of nearest-higher
+  int exp_nonpos;            // the kernel's exp argument is always <= 0 and never NaN
 };
 
 // Workgroups are dispatched to the 8 XCDs round robin (block b to XCD b % 8).
@@ -657,8 +658,14 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
         if (s2 <= a.thr) rho += t;
       } else {
         if (s2 <= a.thr) {
-          const float k =
-              (j == s) ? 1.f : kernel_value<KERNEL>(__builtin_sqrtf(s2), a.kp[0], a.kp[1], a.kp[2], a.kden_rcp);
+          float k = 1.f;
+          if (j != s) {
+            if (a.exp_nonpos && s2 >= 0x1p-96f) {
+              k = kernel_value<KERNEL, true>(sqrt_rn_normal(s2), a.kp[0], a.kp[1], a.kp[2], a.kden_rcp);
+            } else {
+              k = kernel_value<KERNEL>(__builtin_sqrtf(s2), a.kp[0], a.kp[1], a.kp[2], a.kden_rcp);
+            }
+          }
           rho += k * wj;
         }
       }
@@ -1333,6 +1340,55 @@ __global__ __launch_bounds__(kThreads) void k_labels(int32_t* root, int32_t n, i
 }
 
 // ===========================================================================
+// Post-processing: cluster -> points map and centroids.
+// ===========================================================================
+// Sort keys of the map: the label, outliers (negative labels) last.  Labels
+// must lie below n; larger ones are treated as outliers.
+__global__ void k_map_keys(const int32_t* cluster, int32_t n, int32_t* key) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) {
+    const int32_t c = cluster[i];
+    key[i] = (c >= 0 && c < n) ? c : n;
+  }
+}
+
+// offsets[c] = first sorted position whose key is >= c, for c in [0, n_clusters]
+// (the sorted keys ascend, so every c is written by exactly one position).
+// counts[0] = n_clusters = largest label + 1, counts[1] = clustered points.
+__global__ void k_map_offsets(const int32_t* skey, int32_t n, int32_t* offsets, int32_t* counts) {
+  const int k = blockIdx.x * kThreads + threadIdx.x;
+  if (k >= n) return;
+  const int32_t cur = skey[k];
+  const int32_t prev = k > 0 ? skey[k - 1] : -1;
+  // labels in (prev, cur] start here; at the first outlier only n_clusters itself
+  const int32_t hi = cur < n ? cur : (prev < n ? prev + 1 : prev);
+  for (int32_t c = prev + 1; c <= hi; ++c) offsets[c] = k;
+  const bool last = k == n - 1;
+  if (cur < n && (last || skey[k + 1] >= n)) {
+    counts[0] = cur + 1;
+    counts[1] = k + 1;
+    if (last) offsets[cur + 1] = n;
+  }
+}
+
+// Centroid row c, coordinate d: the coordinates of the cluster's points summed in
+// ascending point order from 0.f, divided by (float)size
+// (utils/detail/cluster_centroid.hpp:38-65).
+__global__ void k_centroids(Coords c, int ndim, const int32_t* offsets, const int32_t* indexes,
+                            const int32_t* counts, int32_t max_clusters, float* out) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  const int32_t cl = static_cast<int32_t>(t / ndim);
+  const int d = static_cast<int>(t % ndim);
+  if (cl >= max_clusters) return;
+  if (cl >= counts[0]) return;
+  const int32_t b = offsets[cl], e = offsets[cl + 1];
+  const float* x = c.p[d];
+  float s = 0.f;
+  for (int32_t k = b; k < e; ++k) s += x[indexes[k]];
+  out[static_cast<int64_t>(cl) * ndim + d] = s / static_cast<float>(e - b);
+}
+
+// ===========================================================================
 // Self-test arithmetic (parity of the device math with the host's).
 // ===========================================================================
 __global__ void k_selftest(int op, const float* a, const float* b, float* out, int64_t n) {
@@ -1345,8 +1401,21 @@ __global__ void k_selftest(int op, const float* a, const float* b, float* out, i
     case 2: out[i] = x / b[i]; break;
     case 3: out[i] = sqrt_lt_bound(x); break;
     case 4: out[i] = quotient_via_rcp(x, 1.0 / static_cast<double>(b[i])); break;
+    case 5: out[i] = engine_expf_nonpos(x); break;
     default: out[i] = 0.f;
   }
+}
+
+// Counts the floats with bit patterns in [first, first + count) on which
+// sqrt_rn_normal differs from the IEEE sqrtf.
+__global__ void k_selftest_sqrt(uint32_t first, uint64_t count, unsigned long long* mismatches) {
+  unsigned long long bad = 0;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const float x = __uint_as_float(first + static_cast<uint32_t>(i));
+    bad += __float_as_uint(sqrt_rn_normal(x)) != __float_as_uint(__builtin_sqrtf(x));
+  }
+  if (bad) atomicAdd(mismatches, bad);
 }
 
 // ===========================================================================
@@ -1418,6 +1487,8 @@ struct clue_plan_s {
   // Morton-ordered layout and the block boxes of the nearest-higher search
   DevBuf kn, kn2, pn, pn2, inv, xn, wn, sn, rn, bcnt, blk_off, bbox_lo, bbox_hi, tbox_lo, tbox_hi;
   DevBuf pending, pending_flag;
+  // cluster -> points map (post-processing): sort keys and values
+  DevBuf mkey, mkey2, mval, mval2;
   clue_tuning tuning{0.15f, 0, 128, -1, 3, 0};
   int64_t ldb = 0;  // leading dimension of the block boxes
   int64_t ld = 0;   // leading dimension of xs / xn
@@ -1448,7 +1519,7 @@ struct clue_plan_s {
                    &rho_tmp, &seed_tmp, &bbox_part, &counters, &events, &chunks, &pev,    &sc,
                    &cap_used, &rank_base, &kn,   &kn2,     &pn,      &pn2,      &inv,     &xn,
                    &wn,      &sn,     &rn,       &bcnt,    &blk_off, &bbox_lo,  &bbox_hi, &tbox_lo,
-                   &tbox_hi, &pending, &pending_flag};
+                   &tbox_hi, &pending, &pending_flag, &mkey,  &mkey2,  &mval,    &mval2};
     return k < static_cast<int>(sizeof(b) / sizeof(b[0])) ? b[k] : nullptr;
   }
 };
@@ -1996,6 +2067,11 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
   a.n = n;
   for (int k = 0; k < 3; ++k) a.kp[k] = kd.params[k];
   a.kden_rcp = 1.0 / static_cast<double>((2.f * kd.params[1]) * kd.params[1]);
+  // Gaussian: -(d-mu)^2 / den <= 0 when 1/den is finite and positive; exponential:
+  // -lambda * d <= 0 when lambda >= 0 (d is finite and >= 0 for in-radius candidates)
+  a.exp_nonpos = (kd.type == CLUE_KERNEL_GAUSSIAN && std::isfinite(a.kden_rcp) && a.kden_rcp > 0.0 &&
+                  std::isfinite(kd.params[0])) ||
+                 (kd.type == CLUE_KERNEL_EXPONENTIAL && kd.params[0] >= 0.f && std::isfinite(kd.params[0]));
   for (int d = 0; d < CLUE_MAX_DIM; ++d) a.mp[d] = md.params[d];
   if (blocks) {
     for (int d = 0; d < D; ++d) {
@@ -2400,6 +2476,76 @@ clue_status clue_plan_stage_times(clue_plan_t plan, float* ms, int32_t capacity,
 
 const char* clue_stage_name(int32_t stage) {
   return (stage >= 0 && stage < kStages) ? kStageNames[stage] : "";
+}
+
+clue_status clue_cluster_map(clue_plan_t plan, void* stream, int32_t n, const int32_t* cluster_dev,
+                             int32_t* offsets_dev, int32_t* indexes_dev, int32_t* counts_dev) {
+  if (!plan || !cluster_dev || !offsets_dev || !indexes_dev || !counts_dev)
+    return fail(CLUE_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (n <= 0 || n >= INT32_MAX) return fail(CLUE_ERR_INVALID_ARGUMENT, "invalid number of labels");
+  CLUE_HIP(hipSetDevice(plan->device));
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  const size_t N = static_cast<size_t>(n);
+  const size_t nb = static_cast<size_t>(div_up(n, kSortTile));
+  const int bits = bits_for(static_cast<int64_t>(n) + 1);
+  const size_t hist_entries = static_cast<size_t>(1) << std::min(8, bits);
+  CLUE_HIP(plan->mkey.ensure(N * 4));
+  CLUE_HIP(plan->mkey2.ensure(N * 4));
+  CLUE_HIP(plan->mval.ensure(N * 4));
+  CLUE_HIP(plan->mval2.ensure(N * 4));
+  CLUE_HIP(plan->hist.ensure((hist_entries * nb + 1) * 4));
+  CLUE_HIP(plan->part.ensure((div_up(static_cast<int64_t>(hist_entries * nb + 1), kScanTile) + 1) * 4));
+  k_fill_i32<<<1, 64, 0, st>>>(counts_dev, 0, 2);
+  CLUE_LAUNCH_CHECK("fill");
+  k_map_keys<<<div_up(n, kThreads), kThreads, 0, st>>>(cluster_dev, n, plan->mkey.as<int32_t>());
+  CLUE_LAUNCH_CHECK("map_keys");
+  int32_t* skeys = nullptr;
+  int32_t* perm = nullptr;
+  clue_status rc = radix_sort(plan, st, n, bits, plan->mkey.as<int32_t>(), plan->mkey2.as<int32_t>(),
+                              plan->mval.as<int32_t>(), plan->mval2.as<int32_t>(), true, &skeys, &perm);
+  if (rc != CLUE_OK) return rc;
+  k_map_offsets<<<div_up(n, kThreads), kThreads, 0, st>>>(skeys, n, offsets_dev, counts_dev);
+  CLUE_LAUNCH_CHECK("map_offsets");
+  CLUE_HIP(hipMemcpyAsync(indexes_dev, perm, N * 4, hipMemcpyDeviceToDevice, st));
+  return CLUE_OK;
+}
+
+clue_status clue_cluster_centroids(clue_plan_t plan, void* stream, int32_t ndim, const float* const* coords_dev,
+                                   const int32_t* offsets_dev, const int32_t* indexes_dev, const int32_t* counts_dev,
+                                   int32_t max_clusters, float* centroids_dev) {
+  if (!plan || !coords_dev || !offsets_dev || !indexes_dev || !counts_dev || !centroids_dev)
+    return fail(CLUE_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (ndim < 1 || ndim > CLUE_MAX_DIM) return fail(CLUE_ERR_INVALID_ARGUMENT, "Ndim must be in [1, 10]");
+  if (max_clusters <= 0) return CLUE_OK;
+  for (int d = 0; d < ndim; ++d)
+    if (!coords_dev[d]) return fail(CLUE_ERR_INVALID_ARGUMENT, "NULL coordinate pointer");
+  CLUE_HIP(hipSetDevice(plan->device));
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  Coords c{};
+  for (int d = 0; d < ndim; ++d) c.p[d] = coords_dev[d];
+  const int64_t threads = static_cast<int64_t>(max_clusters) * ndim;
+  k_centroids<<<div_up(threads, kThreads), kThreads, 0, st>>>(c, ndim, offsets_dev, indexes_dev, counts_dev,
+                                                              max_clusters, centroids_dev);
+  CLUE_LAUNCH_CHECK("centroids");
+  return CLUE_OK;
+}
+
+clue_status clue_selftest_sqrt(uint32_t first_bits, uint64_t count, uint64_t* mismatches) {
+  if (!mismatches) return fail(CLUE_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (count == 0 || first_bits + (count - 1) < first_bits) return fail(CLUE_ERR_INVALID_ARGUMENT, "invalid range");
+  unsigned long long* d = nullptr;
+  CLUE_HIP(hipMalloc(&d, sizeof(*d)));
+  hipError_t e = hipMemset(d, 0, sizeof(*d));
+  if (e == hipSuccess) {
+    k_selftest_sqrt<<<4096, kThreads>>>(first_bits, count, d);
+    e = hipGetLastError();
+  }
+  unsigned long long h = 0;
+  if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof(h), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(CLUE_ERR_RUNTIME, hipGetErrorString(e));
+  *mismatches = h;
+  return CLUE_OK;
 }
 
 clue_status clue_selftest_math(int32_t op, const float* a_dev, const float* b_dev, float* out_dev,

@@ -302,6 +302,26 @@ clue_status clue_main_run(float dc, float rhoc, float dm, float seed_dc, int32_t
                           const clue_kernel_desc* kernel, int32_t ndim, int32_t n,
                           int32_t block_size, int32_t device);
 
+/* ---- post-processing on the device: the cluster -> points map of
+ *      get_clusters / make_associator (utils/detail/get_clusters.hpp:21-46,
+ *      internal/MakeAssociator.hpp:13-42) with ClusterProperties' n_clusters
+ *      and cluster sizes (ClusterProperties.hpp:19-51), and cluster_centroids
+ *      (utils/detail/cluster_centroid.hpp:38-65).  Stream-ordered, nothing is
+ *      synchronised; they use the plan's scratch, not its clustering state. */
+/* n labels (negative: outlier).  offsets_dev: n+1 entries, indexes_dev: n
+ * entries.  On completion counts_dev[0] = n_clusters (largest label + 1),
+ * counts_dev[1] = clustered points; offsets[0..n_clusters] delimit each
+ * cluster's points, listed in ascending order in indexes[0..counts[1]) (the
+ * outliers follow, also ascending); cluster sizes are offset differences. */
+clue_status clue_cluster_map(clue_plan_t plan, void* stream, int32_t n, const int32_t* cluster_dev,
+                             int32_t* offsets_dev, int32_t* indexes_dev, int32_t* counts_dev);
+/* Row c of centroids_dev (ndim floats) for c < min(max_clusters, n_clusters):
+ * each coordinate of the cluster's points summed from 0.f in ascending point
+ * order, divided by (float)size -- the reference's float arithmetic. */
+clue_status clue_cluster_centroids(clue_plan_t plan, void* stream, int32_t ndim, const float* const* coords_dev,
+                                   const int32_t* offsets_dev, const int32_t* indexes_dev, const int32_t* counts_dev,
+                                   int32_t max_clusters, float* centroids_dev);
+
 /* ---- introspection of the last call (synchronise the plan's last stream) -- */
 clue_status clue_plan_last_stats(clue_plan_t plan, clue_stats* stats);
 clue_status clue_plan_last_grid(clue_plan_t plan, clue_grid_info* grid);
@@ -321,9 +341,15 @@ clue_status clue_plan_stage_times(clue_plan_t plan, float* ms, int32_t capacity,
 const char* clue_stage_name(int32_t stage);
 
 /* ---- diagnostics used by the parity tests: the engine's device arithmetic
- *      on device arrays (expf as the kernels use it; sqrtf; division). ------- */
+ *      on device arrays (op 0 expf as the kernels use it, 1 sqrtf, 2 division,
+ *      3 sqrt bound, 4 quotient through a double reciprocal, 5 expf of
+ *      non-positive arguments). ------------------------------------------- */
 clue_status clue_selftest_math(int32_t op, const float* a_dev, const float* b_dev, float* out_dev,
                                int64_t n, void* stream);
+/* Number of floats with bit patterns in [first_bits, first_bits + count) on
+ * which the kernels' fast correctly rounded sqrt differs from IEEE sqrtf
+ * (synchronous). */
+clue_status clue_selftest_sqrt(uint32_t first_bits, uint64_t count, uint64_t* mismatches);
 
 #ifdef __cplusplus
 }

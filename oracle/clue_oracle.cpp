@@ -719,4 +719,43 @@ int64_t oracle_read_csv(const char* path, int32_t ndim, float* coords, float* we
 
 void oracle_set_mutation(int32_t mutation) { g_mutation = mutation; }
 
+// Cluster -> points map of a label array: internal/MakeAssociator.hpp:36-42
+// (bins = max label + 1) filled as the host AssociationMap fill does, a counting
+// sort in ascending point order (data_structures/AssociationMap.hpp).  offsets:
+// n+1 entries, indexes: n.  Returns the number of clusters.
+int32_t oracle_cluster_map(int32_t n, const int32_t* cluster, int32_t* offsets, int32_t* indexes,
+                           int32_t* clustered) {
+  int32_t mx = -1, cnt = 0;
+  for (int32_t i = 0; i < n; ++i) {
+    if (cluster[i] > mx) mx = cluster[i];
+    cnt += cluster[i] > -1;  // non_negative, utils/detail/get_clusters.hpp:14-17
+  }
+  const int32_t bins = mx + 1;
+  std::vector<int32_t> sizes(static_cast<size_t>(bins), 0);
+  for (int32_t i = 0; i < n; ++i)
+    if (cluster[i] >= 0) ++sizes[cluster[i]];
+  offsets[0] = 0;
+  for (int32_t b = 0; b < bins; ++b) offsets[b + 1] = offsets[b] + sizes[b];
+  std::vector<int32_t> cursor(offsets, offsets + bins);
+  for (int32_t i = 0; i < n; ++i)
+    if (cluster[i] >= 0) indexes[cursor[cluster[i]]++] = i;
+  *clustered = cnt;
+  return bins;
+}
+
+// cluster_centroids (utils/detail/cluster_centroid.hpp:38-65): per dimension,
+// centroids[cluster][dim] += coord over the points in index order (from 0.f),
+// then divided by (float)count(cluster).  centroids: n_clusters * ndim.
+void oracle_cluster_centroids(int32_t ndim, int32_t n, const float* coords, const int32_t* cluster,
+                              int32_t n_clusters, const int32_t* offsets, float* centroids) {
+  for (int32_t c = 0; c < n_clusters * ndim; ++c) centroids[c] = 0.f;
+  for (int32_t dim = 0; dim < ndim; ++dim) {
+    const float* x = coords + static_cast<size_t>(dim) * n;
+    for (int32_t i = 0; i < n; ++i)
+      if (cluster[i] >= 0) centroids[static_cast<size_t>(cluster[i]) * ndim + dim] += x[i];
+    for (int32_t c = 0; c < n_clusters; ++c)
+      centroids[static_cast<size_t>(c) * ndim + dim] /= static_cast<float>(offsets[c + 1] - offsets[c]);
+  }
+}
+
 }  // extern "C"

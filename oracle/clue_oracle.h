@@ -130,6 +130,14 @@ float oracle_distance(int32_t ndim, const float* a, const float* b, const oracle
 int64_t oracle_read_csv(const char* path, int32_t ndim, float* coords, float* weights,
                         int64_t capacity);
 
+/* Post-processing: cluster -> points map (returns n_clusters; offsets n+1,
+ * indexes n) and centroids (n_clusters rows of ndim) with the reference's
+ * serial arithmetic. */
+int32_t oracle_cluster_map(int32_t n, const int32_t* cluster, int32_t* offsets, int32_t* indexes,
+                           int32_t* clustered);
+void oracle_cluster_centroids(int32_t ndim, int32_t n, const float* coords, const int32_t* cluster,
+                              int32_t n_clusters, const int32_t* offsets, float* centroids);
+
 /* Plant a known mistake (tests only; 0 = none).  See clue_oracle.cpp. */
 void oracle_set_mutation(int32_t mutation);
 

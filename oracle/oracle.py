@@ -111,6 +111,9 @@ def lib():
         L.oracle_read_csv.argtypes = [ctypes.c_char_p, ctypes.c_int32, f32p, f32p, ctypes.c_int64]
         L.oracle_read_csv.restype = ctypes.c_int64
         L.oracle_set_mutation.argtypes = [ctypes.c_int32]
+        L.oracle_cluster_map.argtypes = [ctypes.c_int32, i32p, i32p, i32p, i32p]
+        L.oracle_cluster_map.restype = ctypes.c_int32
+        L.oracle_cluster_centroids.argtypes = [ctypes.c_int32, ctypes.c_int32, f32p, i32p, ctypes.c_int32, i32p, f32p]
         _lib = L
     return _lib
 
@@ -278,6 +281,28 @@ def read_csv(path, ndim):
     w = np.empty(n, np.float32)
     lib().oracle_read_csv(str(path).encode(), ndim, _fp(coords), _fp(w), n)
     return coords, w
+
+
+def cluster_map(cluster):
+    """(offsets[n_clusters+1], indexes[clustered]) of the host association map."""
+    c = np.ascontiguousarray(cluster, dtype=np.int32)
+    n = c.size
+    off = np.zeros(max(n, int(c.max(initial=-1)) + 1) + 1, np.int32)
+    idx = np.zeros(max(n, 1), np.int32)
+    cnt = ctypes.c_int32(0)
+    nc = lib().oracle_cluster_map(n, _ip(c), _ip(off), _ip(idx), ctypes.byref(cnt))
+    return off[: nc + 1].copy(), idx[: cnt.value].copy()
+
+
+def cluster_centroids(coords, cluster):
+    """(n_clusters, ndim) centroids with the reference's serial float arithmetic."""
+    x = _soa(coords)
+    c = np.ascontiguousarray(cluster, dtype=np.int32)
+    off, _ = cluster_map(c)
+    nc = off.size - 1
+    out = np.zeros((max(nc, 1), x.shape[0]), np.float32)
+    lib().oracle_cluster_centroids(x.shape[0], x.shape[1], _fp(x), _ip(c), nc, _ip(off), _fp(out))
+    return out[:nc]
 
 
 class mutation:

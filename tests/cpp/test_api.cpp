@@ -137,6 +137,26 @@ static int cluster(const std::string& csv, float dc, float rhoc, float dm, const
                                         queue.handle()));
     clue::wait(queue);
     write_labels(out, labels);
+    // device post-processing against the host versions
+    auto dmap = clue::get_clusters(queue, d_points);
+    auto hmap = clue::detail::get_clusters(std::span<const int>(labels));
+    CHECK(dmap.size() == hmap.size());
+    const auto back = dmap.toHost();
+    CHECK(back.offsets() == hmap.offsets());
+    CHECK(back.indexes() == hmap.indexes());
+    for (std::size_t c = 0; c < hmap.size(); ++c) CHECK(dmap.count(static_cast<int32_t>(c)) == hmap.count(static_cast<int32_t>(c)));
+    h_points.mark_clustered();
+    std::copy(labels.begin(), labels.end(), h_points.clusterIndexes().begin());
+    h_points.mark_clustered();  // drop properties cached before the copy
+    const auto hcent = clue::cluster_centroids(h_points);
+    const auto dcent = clue::cluster_centroids(queue, d_points);
+    CHECK(hcent.size() == dcent.size());
+    for (std::size_t c = 0; c < hcent.size() && c < dcent.size(); ++c)
+      CHECK(std::memcmp(hcent[c].data(), dcent[c].data(), sizeof(float) * N) == 0);
+    if (!hcent.empty()) {
+      const auto one = clue::cluster_centroid(h_points, 0);
+      CHECK(std::memcmp(one.data(), hcent[0].data(), sizeof(float) * N) == 0);
+    }
     std::printf("clusters: %zu\n", d_points.n_clusters());
     return g_fail ? 1 : 0;
   } else {

@@ -55,6 +55,13 @@ def assert_same(got, ref, tiles, grid, stats, delta):
     assert stats["n_dropped_seeds"] == ref.stats["n_dropped_seeds"]
 
 
+def test_fast_sqrt_exhaustive():
+    # every float in [2^-96, FLT_MAX]: the kernels' fast sqrt equals IEEE sqrtf
+    first = int(np.float32(2.0 ** -96).view(np.uint32))
+    last = int(np.finfo(np.float32).max.view(np.uint32))
+    assert clue.selftest_sqrt(first, last - first + 1) == 0
+
+
 def test_device_math_matches_host():
     # expf: dense sweep of the whole range the kernels can meet, plus specials
     x = np.concatenate([np.linspace(-111.0, 90.0, 400001, dtype=np.float32),
@@ -76,6 +83,13 @@ def test_device_math_matches_host():
     for k in range(0, bits.size, 1 << 23):
         xs = bits[k:k + (1 << 23)].view(np.float32)
         np.testing.assert_array_equal(clue.selftest_math(0, xs).view(np.int32),
+                                      oracle.expf_engine(xs).view(np.int32))
+    # the non-positive-argument variant on every 61st non-positive float (-0 to -inf)
+    neg = np.arange(0x80000000, 0xFF800001, 61, dtype=np.uint64).astype(np.uint32)
+    neg = np.concatenate([neg, np.array([0x80000000, 0xFF800000], np.uint32)])
+    for k in range(0, neg.size, 1 << 23):
+        xs = neg[k:k + (1 << 23)].view(np.float32)
+        np.testing.assert_array_equal(clue.selftest_math(5, xs).view(np.int32),
                                       oracle.expf_engine(xs).view(np.int32))
     # sqrtf and division are correctly rounded on both sides
     rng = np.random.default_rng(1)
