@@ -476,6 +476,9 @@ struct PassArgs {
   int32_t* pending_count;
   int32_t* pending_flag;     // per Morton position: 1 if unresolved after the first phase
   float thr1;                // first-phase search radius threshold
+  float lvl[8];              // second phase: squared radius thresholds of its levels, the last one dm's
+  int nlvl;
+  float lvl_prev0;           // threshold every first-phase candidate was examined within
   int32_t mask_min_points;   // density: tiles at least this large use the bitmask walk (0: never)
   float* rn_out;             // density: Morton order
   const float* rn;           // nearest-higher input: Morton order
@@ -898,6 +901,22 @@ __device__ __forceinline__ float lower_bound2(const float* q, const float* lo, c
   return s;
 }
 
+// Squared distance from q to the farthest point of a box, with the fold order of
+// squared_distance: an upper bound of squared_distance(q, c) for every c in the box.
+template <int D>
+__device__ __forceinline__ float upper_bound2(const float* q, const float* lo, const float* hi) {
+  float s = 0.f;
+#pragma unroll
+  for (int d = D - 1; d >= 0; --d) {
+    const float g1 = q[d] - lo[d];
+    const float g2 = hi[d] - q[d];
+    const float g = g1 > g2 ? g1 : g2;
+    const float t = g * g;
+    s = (d == D - 1) ? t : t + s;
+  }
+  return s;
+}
+
 // Largest float s with sqrtf(s) <= delta.
 __device__ __forceinline__ float sqrt_le_bound_dev(float delta) {
   if (!(delta >= 0.f)) return -1.f;
@@ -1136,83 +1155,152 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
   }
 }
 
-// Second nearest-higher phase: one wave per unresolved point (pending[0..npend),
-// positions in the Morton layout).  The 64 lanes split the candidates of the
-// point's search box (tiles pruned by their boxes against the wave's best
-// bound), each keeps its best (distance, tile id, point id), and a wave
-// reduction takes the least: the serial backend's first strict minimum on a
-// grid without wrapping.  The grid is persistent: every wave strides over the
-// pending list and leaves when it is exhausted.
-template <int D>
+// Second nearest-higher phase: a group of G lanes per unresolved point
+// (pending[0..npend), positions in the Morton layout), kWave / G points per
+// wave.  The group's lanes split the candidates of the point's search box
+// (tiles, then their 32-point Morton blocks, pruned by their boxes against the
+// group's best bound), each keeps its best (distance, tile id, point id), and a
+// group reduction takes the least:
+// the serial backend's first strict minimum on a grid without wrapping.  The
+// grid is persistent: every wave strides over the pending list and leaves when
+// it is exhausted.
+template <int G>
+__device__ __forceinline__ float group_min_f32(float v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) {
+    const float o = __shfl_xor(v, off);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+template <int D, int G>
 __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
+  static_assert(kWave % G == 0, "groups must tile the wave");
   const int npend = uniform(*a.pending_count);
   const int lane = __lane_id();
-  const int nwaves = gridDim.x * (kThreads / kWave);
-  cptr<int32_t> PEND = as_const(a.pending);
-  cptr<int32_t> OFF = as_const(a.offsets);
-  cptr<int32_t> PEV = as_const(a.pev);
-  for (int w = uniform(static_cast<int>((blockIdx.x * kThreads + threadIdx.x) / kWave)); w < npend;
-       w += nwaves) {
-    const int pos = PEND[w];
+  const int gl = lane % G;
+  const int wave = static_cast<int>((blockIdx.x * kThreads + threadIdx.x) / kWave);
+  const int per_wave = kWave / G;
+  const int stride = gridDim.x * (kThreads / kWave) * per_wave;
+  const int grp_shift = (lane / G) * G;  // the group's bits in a wave ballot
+  for (int base = uniform(wave * per_wave); base < npend; base += stride) {
+    const int w = base + lane / G;
+    const bool active = w < npend;
+    const int pos = a.pending[active ? w : npend - 1];
     float q[D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) q[d] = as_const(a.xn[d])[pos];
-    const float rho_i = as_const(a.rn)[pos];
-    const int32_t id_i = as_const(a.pn)[pos];
-    const int e = PEV[id_i];
-    const EvGrid<D> g = load_event<D>(a.ev, e, a.wrap);
+    for (int d = 0; d < D; ++d) q[d] = a.xn[d][pos];
+    const float rho_i = a.rn[pos];
+    const int32_t id_i = a.pn[pos];
+    const int e = a.pev[id_i];
+    const EventDesc* ev = a.ev + e;
+    EvGrid<D> g;
+    g.npd = ev->npd;
+    g.tile_base = ev->tile_base;
+    g.ntiles = ev->ntiles;
+    g.begin = ev->begin;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      g.stride[d] = ev->stride[d];
+      g.mn[d] = ev->mn[d];
+      g.mx[d] = ev->mx[d];
+      g.ts[d] = ev->ts[d];
+      g.wrapped[d] = a.wrap.w[d] != 0;
+    }
     int lo[D], hi[D];
     search_box<D>(q, a.radius, g, lo, hi);
     float delta = FLT_MAX;
     int64_t best = INT64_MAX;
     float bound = a.thr;  // candidates need s2 <= min(thr(dm), max{s : sqrtf(s) <= delta})
-    float wave_bound = a.thr;
-    int b[D];
+    float group_bound = a.thr;
+    // Levels of growing radius: a level examines the candidates within its
+    // radius that lie outside the previous one (blocks entirely inside were
+    // examined before and hold no higher point); the first level that finds a
+    // higher point is the last, since everything nearer has then been examined.
+    float prev = a.lvl_prev0;
+    bool found = false;
+    for (int lv = 0; lv < a.nlvl; ++lv) {
+      const float cap = a.lvl[lv];
+      int b[D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) b[d] = lo[d];
-    while (true) {
-      const int t = tile_index<D>(b, g);
-      float tl[D], th[D];
+      for (int d = 0; d < D; ++d) b[d] = lo[d];
+      bool more = active && !found;
+      while (__ballot(more) != 0) {
+        if (more) {  // uniform within a group
+          const int t = tile_index<D>(b, g);
+          float tl[D], th[D];
 #pragma unroll
-      for (int d = 0; d < D; ++d) {
-        tl[d] = as_const(a.tmin[d])[t];
-        th[d] = as_const(a.tmax[d])[t];
-      }
-      if (lower_bound2<D>(q, tl, th) <= wave_bound) {
-        const int end = OFF[t + 1];
-        for (int j = OFF[t] + lane; j < end; j += kWave) {
-          float c[D];
+          for (int d = 0; d < D; ++d) {
+            tl[d] = a.tmin[d][t];
+            th[d] = a.tmax[d][t];
+          }
+          const float lim_t = cap < group_bound ? cap : group_bound;
+          if (lower_bound2<D>(q, tl, th) <= lim_t && upper_bound2<D>(q, tl, th) > prev) {
+            // the tile's Morton blocks: G at a time against the group's bound,
+            // then the points of each block that can still hold a better candidate
+            const int k0 = a.blk_off[t], k1 = a.blk_off[t + 1];
+            const int off0 = a.offsets[t], off1 = a.offsets[t + 1];
+            for (int kb = k0; kb < k1; kb += G) {
+              const int k = kb + gl;
+              const bool inb = k < k1;
+              float bl[D], bh[D];
 #pragma unroll
-          for (int d = 0; d < D; ++d) c[d] = a.xn[d][j];
-          const float rho_j = a.rn[j];
-          const int32_t id_j = a.pn[j];
-          const int rb = __float_as_int(rho_j);
-          const bool pos_rho = (rb > 0) && (rb <= 0x7f800000);  // rho_j > 0.f
-          const bool higher = (rho_j > rho_i) || ((rho_j == rho_i) && pos_rho && (id_j > id_i));
-          const float s2 = squared_distance<D>(q, c);
-          if (higher && s2 <= bound) {
-            const float dist = __builtin_sqrtf(s2);
-            const int64_t key = (static_cast<int64_t>(t) << 32) | static_cast<uint32_t>(id_j);
-            if (dist < delta || key < best) {
-              delta = dist;
-              best = key;
-              const float lim = sqrt_le_bound_dev(dist);
-              bound = lim < a.thr ? lim : a.thr;
+              for (int d = 0; d < D; ++d) {
+                bl[d] = inb ? a.bmin[d][k] : __builtin_inff();
+                bh[d] = inb ? a.bmax[d][k] : -__builtin_inff();
+              }
+              const float lim_b = cap < group_bound ? cap : group_bound;
+              const bool pass = inb && lower_bound2<D>(q, bl, bh) <= lim_b && upper_bound2<D>(q, bl, bh) > prev;
+              uint64_t gm = __ballot(pass) >> grp_shift;
+              if constexpr (G < kWave) gm &= (uint64_t{1} << G) - 1;
+              while (gm) {
+                const int kl = __builtin_ctzll(gm);
+                gm &= gm - 1;
+                const int start = off0 + (kb + kl - k0) * kBlock;
+                const int end = min(start + kBlock, off1);
+                const float lane_cap = bound < cap ? bound : cap;
+                for (int j = start + gl; j < end; j += G) {
+                  float c[D];
+#pragma unroll
+                  for (int d = 0; d < D; ++d) c[d] = a.xn[d][j];
+                  const float rho_j = a.rn[j];
+                  const int32_t id_j = a.pn[j];
+                  const int rb = __float_as_int(rho_j);
+                  const bool pos_rho = (rb > 0) && (rb <= 0x7f800000);  // rho_j > 0.f
+                  const bool higher = (rho_j > rho_i) || ((rho_j == rho_i) && pos_rho && (id_j > id_i));
+                  const float s2 = squared_distance<D>(q, c);
+                  if (higher && s2 <= lane_cap && s2 <= bound) {
+                    const float dist = __builtin_sqrtf(s2);
+                    const int64_t key = (static_cast<int64_t>(t) << 32) | static_cast<uint32_t>(id_j);
+                    if (dist < delta || key < best) {
+                      delta = dist;
+                      best = key;
+                      const float lim = sqrt_le_bound_dev(dist);
+                      bound = lim < a.thr ? lim : a.thr;
+                    }
+                  }
+                }
+              }
+              group_bound = group_min_f32<G>(bound);  // the point's best so far over its lanes
             }
           }
+          int d = D - 1;
+          for (; d >= 0; --d) {
+            if (++b[d] <= hi[d]) break;
+            b[d] = lo[d];
+          }
+          more = d >= 0;
         }
-        wave_bound = wave_min_f32(bound);  // the point's best so far over all lanes
       }
-      int d = D - 1;
-      for (; d >= 0; --d) {
-        if (++b[d] <= hi[d]) break;
-        b[d] = lo[d];
-      }
-      if (d < 0) break;
+      uint64_t fm = __ballot(best != INT64_MAX) >> grp_shift;
+      if constexpr (G < kWave) fm &= (uint64_t{1} << G) - 1;
+      found = fm != 0;
+      prev = cap;
     }
-    // lexicographic (distance, key) minimum over the wave
+    // lexicographic (distance, key) minimum over the group
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
+    for (int off = G / 2; off > 0; off >>= 1) {
       const float od = __shfl_xor(delta, off);
       const int32_t klo = __shfl_xor(static_cast<int32_t>(best & 0xffffffff), off);
       const int32_t khi = __shfl_xor(static_cast<int32_t>(best >> 32), off);
@@ -1222,7 +1310,7 @@ __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
         best = ok;
       }
     }
-    if (lane == 0) {
+    if (active && gl == 0) {
       const int32_t nh = best == INT64_MAX ? -1 : static_cast<int32_t>(best & 0xffffffff);
       a.nh_out[id_i] = nh;
       a.delta_out[id_i] = nh == -1 ? FLT_MAX : delta;
@@ -1489,7 +1577,7 @@ struct clue_plan_s {
   DevBuf pending, pending_flag;
   // cluster -> points map (post-processing): sort keys and values
   DevBuf mkey, mkey2, mval, mval2;
-  clue_tuning tuning{0.15f, 0, 128, -1, 3, 0};
+  clue_tuning tuning{0.15f, 0, 128, -1, 3, 0, 32};
   int64_t ldb = 0;  // leading dimension of the block boxes
   int64_t ld = 0;   // leading dimension of xs / xn
   // pinned staging ring of the event and chunk tables (a slot is reused once its
@@ -1506,7 +1594,7 @@ struct clue_plan_s {
   bool reset_capacity = false;
   // last call
   bool has_run = false;
-  hipStream_t last_stream = nullptr;
+  hipEvent_t done = nullptr;  // recorded after each call's last kernel (introspection waits on it)
   int32_t last_n = 0, last_d = 0, last_T = 0, last_events = 0;
   int32_t* last_perm = nullptr;
   // optional per-stage HIP event timing of the last call
@@ -1670,6 +1758,18 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
   if constexpr (METRIC == MEUCLID && D <= 3) {
     if (blocks) {
       a.thr1 = sqrt_le_bound(dm * p->tuning.nh_first_radius);
+      // second phase levels: twice the radius each time, dm last
+      a.nlvl = 0;
+      a.lvl_prev0 = std::min(a.thr1, a.thr);
+      float r = dm * p->tuning.nh_first_radius;
+      if (r < dm) {
+        while (a.nlvl < 7) {
+          r *= 2.f;
+          if (!(r < dm)) break;
+          a.lvl[a.nlvl++] = sqrt_le_bound(r);
+        }
+        a.lvl[a.nlvl++] = a.thr;
+      }
       k_nearest_higher_blocks<D, 1><<<grid, kThreads, 0, st>>>(a);
       CLUE_LAUNCH_CHECK("nearest_higher_blocks");
       rc = scan_exclusive(a.pending_flag, a.pending_flag, a.n, p->part.as<int32_t>(), st);
@@ -1680,7 +1780,12 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
       // enough waves to fill the GPU; surplus waves find the pending list exhausted
       const int pblocks = p->tuning.pending_blocks > 0 ? p->tuning.pending_blocks : 2048;
       const int nblk = static_cast<int>(std::min<int64_t>(pblocks, div_up(a.n, kThreads / kWave)));
-      k_nh_pending<D><<<nblk, kThreads, 0, st>>>(a);
+      switch (p->tuning.pending_group) {
+        case 8: k_nh_pending<D, 8><<<nblk, kThreads, 0, st>>>(a); break;
+        case 16: k_nh_pending<D, 16><<<nblk, kThreads, 0, st>>>(a); break;
+        case 32: k_nh_pending<D, 32><<<nblk, kThreads, 0, st>>>(a); break;
+        default: k_nh_pending<D, 64><<<nblk, kThreads, 0, st>>>(a); break;
+      }
       CLUE_LAUNCH_CHECK("nearest_higher_pending");
       return CLUE_OK;
     }
@@ -1955,7 +2060,13 @@ clue_status cluster_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
   CLUE_HIP(hipMemcpyAsync(p->chunks.p, static_cast<EventDesc*>(slot.host) + bc.n_events,
                           static_cast<size_t>(bc.n_chunks) * sizeof(Chunk), hipMemcpyHostToDevice, st));
   CLUE_HIP(hipEventRecord(slot.done, st));
-  return enqueue_batch(p, st, prm, bc);
+  rc = enqueue_batch(p, st, prm, bc);
+  if (rc != CLUE_OK) return rc;
+  // the introspection calls wait on this event, never on the caller's stream,
+  // which may be gone by then
+  if (!p->done) CLUE_HIP(hipEventCreateWithFlags(&p->done, hipEventDisableTiming));
+  CLUE_HIP(hipEventRecord(p->done, st));
+  return CLUE_OK;
 }
 
 // The kernels of one call; the event and chunk tables are already on the device.
@@ -2149,7 +2260,6 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
 
   CLUE_MARK(8);
   p->has_run = true;
-  p->last_stream = st;
   p->last_n = n;
   p->last_d = D;
   p->last_T = Ttot;
@@ -2283,7 +2393,10 @@ clue_status clue_plan_create(int32_t device, clue_plan_t* plan) {
 clue_status clue_plan_destroy(clue_plan_t plan) {
   if (!plan) return CLUE_OK;
   (void)hipSetDevice(plan->device);
-  if (plan->has_run && plan->last_stream) (void)hipStreamSynchronize(plan->last_stream);
+  if (plan->done) {
+    (void)hipEventSynchronize(plan->done);
+    (void)hipEventDestroy(plan->done);
+  }
   for (int k = 0; DevBuf* b = plan->all_buffers(k); ++k) b->release();
   for (auto& slot : plan->stage) {
     if (slot.done) (void)hipEventSynchronize(slot.done);
@@ -2386,9 +2499,9 @@ clue_status clue_main_run(float dc, float rhoc, float dm, float seed_dc, int32_t
 
 clue_status clue_plan_last_stats(clue_plan_t plan, clue_stats* stats) {
   if (!plan || !stats) return fail(CLUE_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (!plan->has_run) return fail(CLUE_ERR_INVALID_ARGUMENT, "the plan has not run");
+  if (!plan->has_run || !plan->done) return fail(CLUE_ERR_INVALID_ARGUMENT, "the plan has not run");
   CLUE_HIP(hipSetDevice(plan->device));
-  CLUE_HIP(hipStreamSynchronize(plan->last_stream));
+  CLUE_HIP(hipEventSynchronize(plan->done));
   int32_t c[16];
   CLUE_HIP(hipMemcpy(c, plan->counters.p, sizeof(c), hipMemcpyDeviceToHost));
   stats->seed_candidates = c[0];
@@ -2401,9 +2514,9 @@ clue_status clue_plan_last_stats(clue_plan_t plan, clue_stats* stats) {
 
 clue_status clue_plan_last_grid(clue_plan_t plan, clue_grid_info* grid) {
   if (!plan || !grid) return fail(CLUE_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (!plan->has_run) return fail(CLUE_ERR_INVALID_ARGUMENT, "the plan has not run");
+  if (!plan->has_run || !plan->done) return fail(CLUE_ERR_INVALID_ARGUMENT, "the plan has not run");
   CLUE_HIP(hipSetDevice(plan->device));
-  CLUE_HIP(hipStreamSynchronize(plan->last_stream));
+  CLUE_HIP(hipEventSynchronize(plan->done));
   EventDesc g;  // the first event of the last call (the only one of a plain call)
   CLUE_HIP(hipMemcpy(&g, plan->events.p, sizeof(g), hipMemcpyDeviceToHost));
   std::memset(grid, 0, sizeof(*grid));
@@ -2421,9 +2534,9 @@ clue_status clue_plan_last_grid(clue_plan_t plan, clue_grid_info* grid) {
 
 clue_status clue_plan_copy_tiles(clue_plan_t plan, int32_t* offsets, int32_t* indexes) {
   if (!plan || !offsets || !indexes) return fail(CLUE_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (!plan->has_run) return fail(CLUE_ERR_INVALID_ARGUMENT, "the plan has not run");
+  if (!plan->has_run || !plan->done) return fail(CLUE_ERR_INVALID_ARGUMENT, "the plan has not run");
   CLUE_HIP(hipSetDevice(plan->device));
-  CLUE_HIP(hipStreamSynchronize(plan->last_stream));
+  CLUE_HIP(hipEventSynchronize(plan->done));
   CLUE_HIP(hipMemcpy(offsets, plan->offsets.p, (static_cast<size_t>(plan->last_T) + 1) * 4,
                      hipMemcpyDeviceToHost));
   CLUE_HIP(hipMemcpy(indexes, plan->last_perm, static_cast<size_t>(plan->last_n) * 4,
@@ -2433,9 +2546,9 @@ clue_status clue_plan_copy_tiles(clue_plan_t plan, int32_t* offsets, int32_t* in
 
 clue_status clue_plan_copy_delta(clue_plan_t plan, float* delta) {
   if (!plan || !delta) return fail(CLUE_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (!plan->has_run) return fail(CLUE_ERR_INVALID_ARGUMENT, "the plan has not run");
+  if (!plan->has_run || !plan->done) return fail(CLUE_ERR_INVALID_ARGUMENT, "the plan has not run");
   CLUE_HIP(hipSetDevice(plan->device));
-  CLUE_HIP(hipStreamSynchronize(plan->last_stream));
+  CLUE_HIP(hipEventSynchronize(plan->done));
   CLUE_HIP(hipMemcpy(delta, plan->delta.p, static_cast<size_t>(plan->last_n) * 4,
                      hipMemcpyDeviceToHost));
   return CLUE_OK;
@@ -2444,7 +2557,9 @@ clue_status clue_plan_copy_delta(clue_plan_t plan, float* delta) {
 clue_status clue_plan_set_tuning(clue_plan_t plan, const clue_tuning* tuning) {
   if (!plan || !tuning) return fail(CLUE_ERR_INVALID_ARGUMENT, "NULL argument");
   if (!(tuning->nh_first_radius > 0.f) || tuning->bitmask_min_points < 0 || tuning->morton_bits > 10 ||
-      tuning->xcd_remap < 0 || tuning->xcd_remap > 3 || tuning->pending_blocks < 0)
+      tuning->xcd_remap < 0 || tuning->xcd_remap > 3 || tuning->pending_blocks < 0 ||
+      (tuning->pending_group != 8 && tuning->pending_group != 16 && tuning->pending_group != 32 &&
+       tuning->pending_group != 64))
     return fail(CLUE_ERR_INVALID_ARGUMENT, "invalid tuning values");
   plan->tuning = *tuning;
   return CLUE_OK;
@@ -2632,7 +2747,6 @@ clue_status clue_batch_graph_launch(clue_batch_graph_t graph, void* stream) {
   auto* bg = reinterpret_cast<BatchGraph*>(graph);
   CLUE_HIP(hipSetDevice(bg->plan->device));
   CLUE_HIP(hipGraphLaunch(bg->exec, static_cast<hipStream_t>(stream)));
-  bg->plan->last_stream = static_cast<hipStream_t>(stream);
   return CLUE_OK;
 }
 
@@ -2730,9 +2844,11 @@ clue_status clue_autotune_batched(clue_plan_t plan, void* stream, clue_params* p
   auto set_radius = [](clue_tuning& t, float v) { t.nh_first_radius = v; };
   auto set_xcd = [](clue_tuning& t, float v) { t.xcd_remap = static_cast<int32_t>(v); };
   auto set_mask = [](clue_tuning& t, float v) { t.density_bitmask = static_cast<int32_t>(v); };
+  auto set_group = [](clue_tuning& t, float v) { t.pending_group = static_cast<int32_t>(v); };
   try_field(set_radius, {0.08f, 0.12f, 0.18f, 0.25f, 0.35f, 0.5f});
   try_field(set_xcd, {0.f, 1.f, 2.f, 3.f});
   try_field(set_mask, {0.f, 1.f});
+  try_field(set_group, {8.f, 16.f, 32.f, 64.f});
   try_field(set_radius, {cur.nh_first_radius * 0.8f, cur.nh_first_radius * 1.25f});
   (void)hipStreamSynchronize(st);
   (void)hipFree(labels);

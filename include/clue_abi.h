@@ -105,8 +105,8 @@ typedef struct {
  *  xcd_remap           bit 0: density, bit 1: nearest-higher -- give each
  *                      XCD a contiguous range of the pass's query blocks, so
  *                      neighbouring queries share one L2
- *  pending_blocks      workgroups of the second nearest-higher phase (one
- *                      wave per unresolved point), 0 auto */
+ *  pending_blocks      workgroups of the second nearest-higher phase, 0 auto
+ *  pending_group       lanes per unresolved point in that phase: 8, 16, 32, 64 */
 typedef struct {
   float nh_first_radius;
   int32_t density_bitmask;
@@ -114,6 +114,7 @@ typedef struct {
   int32_t morton_bits;
   int32_t xcd_remap;
   int32_t pending_blocks;
+  int32_t pending_group;
 } clue_tuning;
 
 typedef struct clue_plan_s* clue_plan_t;
