@@ -371,10 +371,15 @@ def main():
     dom = max(per_pt, key=lambda k: stages.get(k, 0.0))
     t_dom = stages[dom] * 1e-3
     achieved = N * per_pt[dom] / t_dom / 1e9
+    # HBM bytes per launch of that kernel from the committed rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE summary of this workload (calibrated by a copy of known size)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get(dom)
+        kern = json.load(open(tpath)).get("kernels", {})
+        names = {"density": ["k_density"], "nearest_higher": ["k_nearest_higher_blocks", "k_nh_pending"]}[dom]
+        if all(k in kern for k in names):
+            traffic = sum(kern[k]["traffic_bytes"] for k in names)
     b_plain = N * (24 * D + 92) + 12 * (plan.grid()["ntiles"] + 1)
     result = {
         "metric": METRIC,
@@ -395,7 +400,9 @@ def main():
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
                      "algorithmic_bytes_per_launch": N * per_pt[dom], "avg_launch_ms": stages[dom],
-                     "note": "the neighbour loops are VALU-bound; see pairs_per_s"},
+                     "timing": "HIP events on the launch stream around the kernel, mean of the timed steps",
+                     "note": "the neighbour loops are VALU-bound (kernel evaluations per in-radius pair); "
+                             "see pairs_per_s"},
         "pipeline_hbm": {"bytes": b_plain, "frac": round(b_plain / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 6)},
         "stages_ms": stages,
         "stats": stats,

@@ -55,6 +55,26 @@ def build_cpp_tests():
     return out
 
 
+def build_bindings():
+    """The reference's Python modules (CLUE_Convolutional_Kernels, CLUE_GPU_HIP) as
+    pybind11 extensions over the C++ drop-in, in cluestering_alpaka3_amd/pybind/."""
+    import sysconfig
+
+    import pybind11
+
+    outdir = os.path.join(HERE, "pybind")
+    os.makedirs(outdir, exist_ok=True)
+    suffix = sysconfig.get_config_var("EXT_SUFFIX")
+    common = ["g++", "-std=c++20", "-O2", "-shared", "-fPIC", "-fvisibility=hidden",
+              "-I", os.path.join(ROOT, "include"), "-I", pybind11.get_include(),
+              "-I", sysconfig.get_paths()["include"]]
+    for mod, src in (("CLUE_Convolutional_Kernels", "binding_kernels.cpp"), ("CLUE_GPU_HIP", "binding_hip.cpp")):
+        out = os.path.join(outdir, mod + suffix)
+        run([*common, os.path.join(HERE, "csrc", src), "-L", HERE, "-lclue_hip",
+             "-Wl,-rpath,$ORIGIN/..", "-o", out + ".tmp"])
+        os.replace(out + ".tmp", out)
+
+
 def build_oracle():
     run(["make", "-C", os.path.join(ROOT, "oracle"), "-s"])
 
@@ -63,9 +83,11 @@ def build_all():
     build_oracle()
     build_engine()
     build_cpp_tests()
+    build_bindings()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "engine": build_engine, "oracle": build_oracle, "cpp": build_cpp_tests}[w]()
+        {"all": build_all, "engine": build_engine, "oracle": build_oracle, "cpp": build_cpp_tests,
+         "bindings": build_bindings}[w]()

@@ -354,7 +354,8 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const int32_t* keys_
 // Tile-ordered SoA copy: xs[d][slot] = x[d][perm[slot]] (the coalesced tile array).
 __global__ __launch_bounds__(kThreads) void k_gather(Coords c, const float* w, const int32_t* perm,
                                                       int32_t n, int D, int64_t ld, float* xs,
-                                                      float* ws, float* hw, float h) {
+                                                      float* ws, float* hw, float h,
+                                                      const int32_t* key_nh, int32_t* key_nh_s) {
   const int s = blockIdx.x * kThreads + threadIdx.x;
   if (s >= n) return;
   const int32_t id = perm[s];
@@ -362,27 +363,24 @@ __global__ __launch_bounds__(kThreads) void k_gather(Coords c, const float* w, c
   const float wv = w[id];
   ws[s] = wv;
   hw[s] = h * wv;  // FlatKernel term k*w with k = h, precomputed once per point
+  if (key_nh) key_nh_s[s] = key_nh[id];  // (tile, Morton code) key, in tile order
 }
 
 // Morton-ordered SoA copy (tile-major, Morton code inside the tile): the query
 // order of both neighbour passes and the candidate layout of the nearest-higher
-// search.  Also the weight and the point's slot in the tile-ordered layout.
-__global__ __launch_bounds__(kThreads) void k_gather_morton(Coords c, const float* w,
-                                                             const int32_t* pn,
-                                                             const int32_t* inv_ref, int32_t n,
-                                                             int D, int64_t ld, float* xn,
-                                                             float* wn, int32_t* sn) {
+// search.  slot_of[p] is the tile-order slot of Morton position p (a permutation
+// inside each tile, so every read stays within one tile's range); mid[p] its
+// point id.
+__global__ __launch_bounds__(kThreads) void k_gather_morton(const float* xs, const float* ws,
+                                                             const int32_t* perm, const int32_t* slot_of,
+                                                             int32_t n, int D, int64_t ld, float* xn,
+                                                             float* wn, int32_t* mid) {
   const int p = blockIdx.x * kThreads + threadIdx.x;
   if (p >= n) return;
-  const int32_t id = pn[p];
-  for (int d = 0; d < D; ++d) xn[d * ld + p] = c.p[d][id];
-  wn[p] = w[id];
-  sn[p] = inv_ref[id];
-}
-
-__global__ void k_inverse(const int32_t* perm, int32_t n, int32_t* inv) {
-  const int s = blockIdx.x * kThreads + threadIdx.x;
-  if (s < n) inv[perm[s]] = s;
+  const int32_t s = slot_of[p];
+  for (int d = 0; d < D; ++d) xn[d * ld + p] = xs[d * ld + s];
+  wn[p] = ws[s];
+  mid[p] = perm[s];
 }
 
 constexpr int kBlock = 32;       // points per block of the Morton layout
@@ -1573,7 +1571,7 @@ struct clue_plan_s {
   // events of the call: descriptors, chunks, point -> event, per-event counters
   DevBuf events, chunks, pev, sc, cap_used, rank_base;
   // Morton-ordered layout and the block boxes of the nearest-higher search
-  DevBuf kn, kn2, pn, pn2, inv, xn, wn, sn, rn, bcnt, blk_off, bbox_lo, bbox_hi, tbox_lo, tbox_hi;
+  DevBuf kn, kn2, pn, pn2, mid, xn, wn, rn, bcnt, blk_off, bbox_lo, bbox_hi, tbox_lo, tbox_hi;
   DevBuf pending, pending_flag;
   // cluster -> points map (post-processing): sort keys and values
   DevBuf mkey, mkey2, mval, mval2;
@@ -1605,8 +1603,8 @@ struct clue_plan_s {
     DevBuf* b[] = {&key,     &key2,   &perm,     &perm2,   &count,   &offsets,  &hist,    &part,
                    &xs,      &ws,     &hw,       &rho_s,   &delta,   &flag,     &root,    &nh_tmp,
                    &rho_tmp, &seed_tmp, &bbox_part, &counters, &events, &chunks, &pev,    &sc,
-                   &cap_used, &rank_base, &kn,   &kn2,     &pn,      &pn2,      &inv,     &xn,
-                   &wn,      &sn,     &rn,       &bcnt,    &blk_off, &bbox_lo,  &bbox_hi, &tbox_lo,
+                   &cap_used, &rank_base, &kn,   &kn2,     &pn,      &pn2,      &mid,     &xn,
+                   &wn,      &rn,       &bcnt,    &blk_off, &bbox_lo,  &bbox_hi, &tbox_lo,
                    &tbox_hi, &pending, &pending_flag, &mkey,  &mkey2,  &mval,    &mval2};
     return k < static_cast<int>(sizeof(b) / sizeof(b[0])) ? b[k] : nullptr;
   }
@@ -1677,10 +1675,9 @@ clue_status reserve(clue_plan_s* p, int32_t D, int32_t n, int32_t T, int32_t n_e
   CLUE_HIP(p->kn2.ensure(N * 4));
   CLUE_HIP(p->pn.ensure(N * 4));
   CLUE_HIP(p->pn2.ensure(N * 4));
-  CLUE_HIP(p->inv.ensure(N * 4));
+  CLUE_HIP(p->mid.ensure(N * 4));
   CLUE_HIP(p->xn.ensure(static_cast<size_t>(ld) * D * 4));
   CLUE_HIP(p->wn.ensure(N * 4));
-  CLUE_HIP(p->sn.ensure(N * 4));
   CLUE_HIP(p->rn.ensure(N * 4));
   CLUE_HIP(p->bcnt.ensure(TT * 4));
   CLUE_HIP(p->blk_off.ensure(TT * 4));
@@ -2119,14 +2116,6 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
                   p->perm2.as<int32_t>(), false, &sorted_keys, &perm);
   if (rc != CLUE_OK) return rc;
   p->last_perm = perm;
-  // Morton layout: sort by (tile id, Morton code inside the tile)
-  int32_t* mkeys = nullptr;
-  int32_t* pn = nullptr;
-  if (blocks) {
-    rc = radix_sort(p, st, n, tbits + sub_bits * D, p->kn.as<int32_t>(), p->kn2.as<int32_t>(),
-                    p->pn.as<int32_t>(), p->pn2.as<int32_t>(), true, &mkeys, &pn);
-    if (rc != CLUE_OK) return rc;
-  }
 
   // ---- tile offsets
   CLUE_MARK(3);
@@ -2137,14 +2126,19 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
   const float h = kd.type == CLUE_KERNEL_FLAT ? kd.params[0] : 0.f;
   k_gather<<<div_up(n, kThreads), kThreads, 0, st>>>(c, weights, perm, n, D, p->ld,
                                                      p->xs.as<float>(), p->ws.as<float>(),
-                                                     p->hw.as<float>(), h);
+                                                     p->hw.as<float>(), h, blocks ? p->kn.as<int32_t>() : nullptr,
+                                                     p->kn2.as<int32_t>());
   CLUE_LAUNCH_CHECK("gather");
+  int32_t* mkeys = nullptr;
+  int32_t* slot_of = nullptr;  // Morton position -> tile-order slot
   if (blocks) {
-    k_inverse<<<div_up(n, kThreads), kThreads, 0, st>>>(perm, n, p->inv.as<int32_t>());
-    CLUE_LAUNCH_CHECK("inverse");
-    k_gather_morton<<<div_up(n, kThreads), kThreads, 0, st>>>(c, weights, pn, p->inv.as<int32_t>(), n, D,
-                                                              p->ld, p->xn.as<float>(), p->wn.as<float>(),
-                                                              p->sn.as<int32_t>());
+    // Morton layout: the tile-ordered slots sorted by (tile id, Morton code inside the tile)
+    rc = radix_sort(p, st, n, tbits + sub_bits * D, p->kn2.as<int32_t>(), p->kn.as<int32_t>(),
+                    p->pn.as<int32_t>(), p->pn2.as<int32_t>(), true, &mkeys, &slot_of);
+    if (rc != CLUE_OK) return rc;
+    k_gather_morton<<<div_up(n, kThreads), kThreads, 0, st>>>(p->xs.as<float>(), p->ws.as<float>(), perm, slot_of,
+                                                              n, D, p->ld, p->xn.as<float>(), p->wn.as<float>(),
+                                                              p->mid.as<int32_t>());
     CLUE_LAUNCH_CHECK("gather_morton");
     k_block_count<<<div_up(Ttot, kThreads), kThreads, 0, st>>>(p->offsets.as<int32_t>(), Ttot,
                                                                p->bcnt.as<int32_t>());
@@ -2193,8 +2187,8 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
       a.tmax[d] = p->tbox_hi.as<float>() + static_cast<int64_t>(d) * Ttot;
     }
     a.wn = p->wn.as<float>();
-    a.sn = p->sn.as<int32_t>();
-    a.pn = pn;
+    a.sn = slot_of;
+    a.pn = p->mid.as<int32_t>();
     a.kn = mkeys;
     a.mshift = sub_bits * D;
     a.blk_off = p->blk_off.as<int32_t>();

@@ -1,0 +1,114 @@
+"""Summaries of rocprofv3 output for the engine's kernels.
+
+    python -m cluestering_alpaka3_amd.profiling stats <kernel_stats.csv>
+        per-kernel call count and average duration (a --kernel-trace --stats run)
+
+    python -m cluestering_alpaka3_amd.profiling traffic <fetch.csv> <write.csv> [--copy-mib 64] [--json OUT]
+        HBM bytes per dispatch from two --pmc runs (FETCH_SIZE, then WRITE_SIZE),
+        each scaled by what the same counter reads for the device-to-device copy
+        of known size that bench.py makes before timing (--calibration-copy-mib):
+        the copy moves copy_mib MiB in and copy_mib MiB out, so
+        factor = copy bytes / counter bytes of that copy dispatch.
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import json
+import re
+from collections import defaultdict
+
+COPY_KERNEL = "__amd_rocclr_copyBuffer"
+
+
+def short_name(kernel_name):
+    """'void clue_hip::k_density<2, 0, 1, true, true>(clue_hip::PassArgs)' -> 'k_density'."""
+    base = kernel_name.split("(")[0]
+    base = re.sub(r"<.*>", "", base)
+    return base.split("::")[-1].replace("void ", "").strip()
+
+
+def kernel_stats(path):
+    rows = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            rows.append((r["Name"], int(r["Calls"]), float(r["AverageNs"]), float(r["Percentage"])))
+    rows.sort(key=lambda x: -x[3])
+    return rows
+
+
+def _per_dispatch(path, counter):
+    vals = defaultdict(list)
+    grids = defaultdict(list)
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] != counter:
+                continue
+            vals[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+            grids[r["Kernel_Name"]].append(int(r["Grid_Size"]))
+    return vals, grids
+
+
+def traffic(fetch_csv, write_csv, copy_mib=64):
+    fetch, fgrid = _per_dispatch(fetch_csv, "FETCH_SIZE")
+    write, wgrid = _per_dispatch(write_csv, "WRITE_SIZE")
+    copy_kib = copy_mib * 1024.0
+
+    def calib(vals, grids):
+        # the calibration copy is the largest copy dispatch of the run
+        cands = [(g, v) for name in vals if name.startswith(COPY_KERNEL)
+                 for g, v in zip(grids[name], vals[name])]
+        g, v = max(cands)
+        return copy_kib / v, v
+
+    ff, fraw = calib(fetch, fgrid)
+    fw, wraw = calib(write, wgrid)
+    out = {"calibration": {"copy_mib": copy_mib, "fetch_counter_kib": fraw, "write_counter_kib": wraw,
+                           "fetch_factor": ff, "write_factor": fw},
+           "kernels": {}}
+    for name in fetch:
+        if name.startswith(COPY_KERNEL) or name.startswith("__amd"):
+            continue
+        fk = sum(fetch[name]) / len(fetch[name])
+        wk = sum(write.get(name, [0.0])) / max(1, len(write.get(name, [])))
+        key = short_name(name)
+        prev = out["kernels"].get(key)
+        fb, wb = fk * 1024 * ff, wk * 1024 * fw
+        if prev is None or fb + wb > prev["traffic_bytes"]:
+            out["kernels"][key] = {"name": name, "dispatches": len(fetch[name]), "fetch_bytes": round(fb),
+                                   "write_bytes": round(wb), "traffic_bytes": round(fb + wb)}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("csv")
+    s.add_argument("--top", type=int, default=25)
+    t = sub.add_parser("traffic")
+    t.add_argument("fetch_csv")
+    t.add_argument("write_csv")
+    t.add_argument("--copy-mib", type=int, default=64)
+    t.add_argument("--json")
+    a = ap.parse_args()
+    if a.cmd == "stats":
+        print(f"{'kernel':100s} {'calls':>6s} {'avg_us':>10s} {'share%':>7s}")
+        for name, calls, avg, pct in kernel_stats(a.csv)[: a.top]:
+            print(f"{name[:100]:100s} {calls:6d} {avg / 1e3:10.2f} {pct:7.2f}")
+    else:
+        res = traffic(a.fetch_csv, a.write_csv, a.copy_mib)
+        c = res["calibration"]
+        print(f"calibration: {c['copy_mib']} MiB copy read as FETCH {c['fetch_counter_kib']:.1f} KiB "
+              f"(x{c['fetch_factor']:.4f}), WRITE {c['write_counter_kib']:.1f} KiB (x{c['write_factor']:.4f})")
+        print(f"{'kernel':28s} {'dispatches':>10s} {'fetch_MB':>10s} {'write_MB':>10s} {'total_MB':>10s}")
+        for k, v in sorted(res["kernels"].items(), key=lambda kv: -kv[1]["traffic_bytes"]):
+            print(f"{k:28s} {v['dispatches']:10d} {v['fetch_bytes'] / 1e6:10.2f} {v['write_bytes'] / 1e6:10.2f} "
+                  f"{v['traffic_bytes'] / 1e6:10.2f}")
+        if a.json:
+            with open(a.json, "w") as f:
+                json.dump(res, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()
