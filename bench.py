@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--calibration-copy-mib", type=int, default=64,
                     help="size of one device-to-device copy made before timing; its rocprofv3 FETCH_SIZE / "
                          "WRITE_SIZE calibrate the HBM byte counters (0 disables)")
-    ap.add_argument("--window-fraction", type=float, default=0.25,
+    ap.add_argument("--window-fraction", type=float, default=0.5,
                     help="side of the serial CPU sample window, as a fraction of the box side")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-events measurement")
     ap.add_argument("--batch-events", type=int, default=4096)
