@@ -438,6 +438,63 @@ __global__ __launch_bounds__(kThreads) void k_tile_boxes(const float* bmin, cons
   }
 }
 
+// Estimated cost of each block of kThreads Morton-ordered queries: the sum of
+// the sizes of the queries' tiles (the candidates grow with the local density).
+__global__ __launch_bounds__(kThreads) void k_query_block_cost(const int32_t* kn, int mshift,
+                                                                const int32_t* offsets, int32_t n,
+                                                                int32_t* cost) {
+  __shared__ int32_t part[kThreads / kWave];
+  const int q = blockIdx.x * kThreads + threadIdx.x;
+  int32_t c = 0;
+  if (q < n) {
+    const int t = kn[q] >> mshift;
+    c = offsets[t + 1] - offsets[t];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  if (__lane_id() == 0) part[threadIdx.x / kWave] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t s = 0;
+    for (int w = 0; w < kThreads / kWave; ++w) s += part[w];
+    cost[blockIdx.x] = s;
+  }
+}
+
+constexpr int kOrderMax = 16384;  // query blocks the in-LDS ordering handles
+
+// Query blocks in decreasing cost (ties by index), one workgroup bitonic sort in
+// LDS: heavy blocks start first, so the light ones fill in behind them and the
+// pass does not end on a few late heavy blocks.
+__global__ __launch_bounds__(1024) void k_query_block_order(const int32_t* cost, int32_t nb, int32_t* order) {
+  __shared__ uint64_t key[kOrderMax];
+  int m = 1;
+  while (m < nb) m <<= 1;
+  for (int i = threadIdx.x; i < m; i += 1024) {
+    // larger cost first: sort ascending on (~cost, index)
+    const uint32_t c = i < nb ? static_cast<uint32_t>(cost[i]) : 0u;
+    key[i] = i < nb ? ((static_cast<uint64_t>(~c) << 32) | static_cast<uint32_t>(i)) : ~uint64_t{0};
+  }
+  __syncthreads();
+  for (int k = 2; k <= m; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < m; i += 1024) {
+        const int l = i ^ j;
+        if (l > i) {
+          const uint64_t a = key[i], b = key[l];
+          const bool up = (i & k) == 0;
+          if ((a > b) == up) {
+            key[i] = b;
+            key[l] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < nb; i += 1024) order[i] = static_cast<int32_t>(key[i] & 0xffffffffu);
+}
+
 // ===========================================================================
 // Density (K4) and nearest-higher (K5).
 // ===========================================================================
@@ -486,6 +543,7 @@ struct PassArgs {
   float* delta_out;          // nearest-higher distance: original order
   int xcd;                   // bit 0: XCD-aware block order of density, bit 1: of nearest-higher
   int exp_nonpos;            // the kernel's exp argument is always <= 0 and never NaN
+  const int32_t* qorder;     // density: query blocks in decreasing estimated cost (or null)
 };
 
 // Workgroups are dispatched to the 8 XCDs round robin (block b to XCD b % 8).
@@ -614,7 +672,8 @@ __device__ __forceinline__ float readlane_f32(float v, int lane) {
 // so the lanes of a wave share most of their search box); otherwise in tile order.
 template <int D, int METRIC, int KERNEL, bool UNION, bool QNH>
 __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
-  const int pos = logical_block(a.xcd & 1) * kThreads + threadIdx.x;
+  const int qb = a.qorder ? a.qorder[blockIdx.x] : logical_block(a.xcd & 1);
+  const int pos = qb * kThreads + threadIdx.x;
   const bool valid = pos < a.n;
   const int pq = valid ? pos : a.n - 1;
   const int lane = __lane_id();
@@ -1572,10 +1631,10 @@ struct clue_plan_s {
   DevBuf events, chunks, pev, sc, cap_used, rank_base;
   // Morton-ordered layout and the block boxes of the nearest-higher search
   DevBuf kn, kn2, pn, pn2, mid, xn, wn, rn, bcnt, blk_off, bbox_lo, bbox_hi, tbox_lo, tbox_hi;
-  DevBuf pending, pending_flag;
+  DevBuf pending, pending_flag, qcost, qorder;
   // cluster -> points map (post-processing): sort keys and values
   DevBuf mkey, mkey2, mval, mval2;
-  clue_tuning tuning{0.15f, 0, 128, -1, 3, 0, 32};
+  clue_tuning tuning{0.15f, 0, 128, -1, 3, 0, 32, -1};
   int64_t ldb = 0;  // leading dimension of the block boxes
   int64_t ld = 0;   // leading dimension of xs / xn
   // pinned staging ring of the event and chunk tables (a slot is reused once its
@@ -1605,7 +1664,7 @@ struct clue_plan_s {
                    &rho_tmp, &seed_tmp, &bbox_part, &counters, &events, &chunks, &pev,    &sc,
                    &cap_used, &rank_base, &kn,   &kn2,     &pn,      &pn2,      &mid,     &xn,
                    &wn,      &rn,       &bcnt,    &blk_off, &bbox_lo,  &bbox_hi, &tbox_lo,
-                   &tbox_hi, &pending, &pending_flag, &mkey,  &mkey2,  &mval,    &mval2};
+                   &tbox_hi, &pending, &pending_flag, &qcost, &qorder, &mkey,  &mkey2,  &mval,    &mval2};
     return k < static_cast<int>(sizeof(b) / sizeof(b[0])) ? b[k] : nullptr;
   }
 };
@@ -1689,6 +1748,8 @@ clue_status reserve(clue_plan_s* p, int32_t D, int32_t n, int32_t T, int32_t n_e
   CLUE_HIP(p->tbox_hi.ensure(static_cast<size_t>(T) * D * 4));
   CLUE_HIP(p->pending.ensure((N + 64) * 4));
   CLUE_HIP(p->pending_flag.ensure((N + 1) * 4));
+  CLUE_HIP(p->qcost.ensure(static_cast<size_t>(div_up(n, kThreads)) * 4));
+  CLUE_HIP(p->qorder.ensure(static_cast<size_t>(div_up(n, kThreads)) * 4));
   return CLUE_OK;
 }
 
@@ -1732,6 +1793,17 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
   a.thr = sqrt_le_bound(dc);
   clue_status rc = mark(p, 4, st);
   if (rc != CLUE_OK) return rc;
+  a.qorder = nullptr;
+  // heaviest-first pays in 2D (a shorter tail); in 3D the lost spatial locality
+  // of concurrent blocks costs more than it saves (measured), hence the default
+  const bool order = p->tuning.density_order > 0 || (p->tuning.density_order < 0 && D <= 2);
+  if (blocks && order && grid <= kOrderMax) {
+    k_query_block_cost<<<grid, kThreads, 0, st>>>(a.kn, a.mshift, a.offsets, a.n, p->qcost.as<int32_t>());
+    CLUE_LAUNCH_CHECK("query_block_cost");
+    k_query_block_order<<<1, 1024, 0, st>>>(p->qcost.as<int32_t>(), grid, p->qorder.as<int32_t>());
+    CLUE_LAUNCH_CHECK("query_block_order");
+    a.qorder = p->qorder.as<int32_t>();
+  }
   if (blocks) {
     switch (kernel_type) {
       case KFLAT: k_density<D, METRIC, KFLAT, U, true><<<grid, kThreads, 0, st>>>(a); break;
@@ -2553,7 +2625,8 @@ clue_status clue_plan_set_tuning(clue_plan_t plan, const clue_tuning* tuning) {
   if (!(tuning->nh_first_radius > 0.f) || tuning->bitmask_min_points < 0 || tuning->morton_bits > 10 ||
       tuning->xcd_remap < 0 || tuning->xcd_remap > 3 || tuning->pending_blocks < 0 ||
       (tuning->pending_group != 8 && tuning->pending_group != 16 && tuning->pending_group != 32 &&
-       tuning->pending_group != 64))
+       tuning->pending_group != 64) ||
+      tuning->density_order < -1 || tuning->density_order > 1)
     return fail(CLUE_ERR_INVALID_ARGUMENT, "invalid tuning values");
   plan->tuning = *tuning;
   return CLUE_OK;
@@ -2843,6 +2916,7 @@ clue_status clue_autotune_batched(clue_plan_t plan, void* stream, clue_params* p
   try_field(set_xcd, {0.f, 1.f, 2.f, 3.f});
   try_field(set_mask, {0.f, 1.f});
   try_field(set_group, {8.f, 16.f, 32.f, 64.f});
+  try_field([](clue_tuning& t, float v) { t.density_order = static_cast<int32_t>(v); }, {0.f, 1.f});
   try_field(set_radius, {cur.nh_first_radius * 0.8f, cur.nh_first_radius * 1.25f});
   (void)hipStreamSynchronize(st);
   (void)hipFree(labels);

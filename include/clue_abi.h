@@ -106,7 +106,10 @@ typedef struct {
  *                      XCD a contiguous range of the pass's query blocks, so
  *                      neighbouring queries share one L2
  *  pending_blocks      workgroups of the second nearest-higher phase, 0 auto
- *  pending_group       lanes per unresolved point in that phase: 8, 16, 32, 64 */
+ *  pending_group       lanes per unresolved point in that phase: 8, 16, 32, 64
+ *  density_order       1: launch the density pass's query blocks heaviest first
+ *                      (estimated from their tiles' sizes), 0: in Morton order,
+ *                      -1: heaviest first in 1D and 2D only */
 typedef struct {
   float nh_first_radius;
   int32_t density_bitmask;
@@ -115,6 +118,7 @@ typedef struct {
   int32_t xcd_remap;
   int32_t pending_blocks;
   int32_t pending_group;
+  int32_t density_order;
 } clue_tuning;
 
 typedef struct clue_plan_s* clue_plan_t;

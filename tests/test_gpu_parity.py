@@ -130,7 +130,8 @@ TUNINGS = [dict(nh_first_radius=0.05, density_bitmask=1, bitmask_min_points=1, m
            dict(nh_first_radius=0.5, density_bitmask=1, bitmask_min_points=64, morton_bits=-1),
            dict(xcd_remap=0, pending_blocks=1, pending_group=8),
            dict(xcd_remap=3, pending_blocks=7, density_bitmask=0, pending_group=64),
-           dict(xcd_remap=2, pending_blocks=0, nh_first_radius=0.01, pending_group=32)]
+           dict(xcd_remap=2, pending_blocks=0, nh_first_radius=0.01, pending_group=32, density_order=0),
+           dict(density_order=1, density_bitmask=1, xcd_remap=0)]
 
 
 @pytest.mark.parametrize("tuning", TUNINGS)
