@@ -239,7 +239,7 @@ __device__ __forceinline__ EvGrid<D> load_event(const EventDesc* events, int e, 
 template <int D>
 __global__ __launch_bounds__(kThreads) void k_bin(Coords c, const Chunk* chunks,
                                                    const EventDesc* events, Wrap wrap, int32_t* key,
-                                                   int32_t* count, int32_t* key_nh, int sub_bits,
+                                                   int32_t* key_nh, int sub_bits,
                                                    int32_t* pev) {
   const Chunk ch = chunks[blockIdx.x];
   const EvGrid<D> g = load_event<D>(events, ch.event, wrap);
@@ -261,7 +261,6 @@ __global__ __launch_bounds__(kThreads) void k_bin(Coords c, const Chunk* chunks,
     }
     key[i] = gb;
     pev[i] = ch.event;
-    atomicAdd(&count[gb], 1);
     if (key_nh) key_nh[i] = static_cast<int32_t>((static_cast<uint32_t>(gb) << (sub_bits * D)) | code);
   }
 }
@@ -381,6 +380,20 @@ __global__ __launch_bounds__(kThreads) void k_gather_morton(const float* xs, con
   for (int d = 0; d < D; ++d) xn[d * ld + p] = xs[d * ld + s];
   wn[p] = ws[s];
   mid[p] = perm[s];
+}
+
+// offsets[t] = number of points whose tile id is below t (lower bound of t in
+// the sorted tile ids), t in [0, T]: the CSR offsets of the tiles.
+__global__ void k_tile_offsets(const int32_t* skey, int32_t n, int32_t T, int32_t* offsets) {
+  const int t = blockIdx.x * kThreads + threadIdx.x;
+  if (t > T) return;
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = lo + ((hi - lo) >> 1);
+    if (skey[mid] < t) lo = mid + 1;
+    else hi = mid;
+  }
+  offsets[t] = lo;
 }
 
 constexpr int kBlock = 32;       // points per block of the Morton layout
@@ -1625,7 +1638,7 @@ using namespace clue_hip;
 
 struct clue_plan_s {
   int device = 0;
-  DevBuf key, key2, perm, perm2, count, offsets, hist, part, xs, ws, hw, rho_s, delta, flag, root,
+  DevBuf key, key2, perm, perm2, offsets, hist, part, xs, ws, hw, rho_s, delta, flag, root,
       nh_tmp, rho_tmp, seed_tmp, bbox_part, counters;
   // events of the call: descriptors, chunks, point -> event, per-event counters
   DevBuf events, chunks, pev, sc, cap_used, rank_base;
@@ -1659,7 +1672,7 @@ struct clue_plan_s {
   hipEvent_t ev[kStages + 1] = {};
 
   DevBuf* all_buffers(int k) {
-    DevBuf* b[] = {&key,     &key2,   &perm,     &perm2,   &count,   &offsets,  &hist,    &part,
+    DevBuf* b[] = {&key,     &key2,   &perm,     &perm2,   &offsets,  &hist,    &part,
                    &xs,      &ws,     &hw,       &rho_s,   &delta,   &flag,     &root,    &nh_tmp,
                    &rho_tmp, &seed_tmp, &bbox_part, &counters, &events, &chunks, &pev,    &sc,
                    &cap_used, &rank_base, &kn,   &kn2,     &pn,      &pn2,      &mid,     &xn,
@@ -1704,7 +1717,6 @@ clue_status reserve(clue_plan_s* p, int32_t D, int32_t n, int32_t T, int32_t n_e
   CLUE_HIP(p->key2.ensure(N * 4));
   CLUE_HIP(p->perm.ensure(N * 4));
   CLUE_HIP(p->perm2.ensure(N * 4));
-  CLUE_HIP(p->count.ensure(TT * 4));
   CLUE_HIP(p->offsets.ensure(TT * 4));
   CLUE_HIP(p->hist.ensure(hist_entries * 4));
   CLUE_HIP(p->part.ensure((div_up(scan_max, kScanTile) + 1) * 4));
@@ -1775,7 +1787,7 @@ clue_status launch_bin(const Coords& c, int32_t n_chunks, clue_plan_s* p, const 
                        int sub_bits, hipStream_t st) {
   k_bin<D><<<n_chunks, kThreads, 0, st>>>(
       c, p->chunks.as<Chunk>(), p->events.as<EventDesc>(), wrap, p->key.as<int32_t>(),
-      p->count.as<int32_t>(), sub_bits >= 0 ? p->kn.as<int32_t>() : nullptr, sub_bits < 0 ? 0 : sub_bits,
+      sub_bits >= 0 ? p->kn.as<int32_t>() : nullptr, sub_bits < 0 ? 0 : sub_bits,
       p->pev.as<int32_t>());
   CLUE_LAUNCH_CHECK("bin");
   return CLUE_OK;
@@ -2173,8 +2185,6 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
 
   // ---- tile ids + histogram (+ Morton keys, point -> event)
   CLUE_MARK(1);
-  k_fill_i32<<<div_up(Ttot + 1, kThreads), kThreads, 0, st>>>(p->count.as<int32_t>(), 0, Ttot + 1);
-  CLUE_LAUNCH_CHECK("fill");
   rc = dispatch_dim(D, [&](auto dc_) {
     return launch_bin<decltype(dc_)::value>(c, n_chunks, p, wrap, sub_bits, st);
   });
@@ -2185,14 +2195,14 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
   int32_t* sorted_keys = nullptr;
   int32_t* perm = nullptr;  // slot -> point id
   rc = radix_sort(p, st, n, tbits, p->key.as<int32_t>(), p->key2.as<int32_t>(), p->perm.as<int32_t>(),
-                  p->perm2.as<int32_t>(), false, &sorted_keys, &perm);
+                  p->perm2.as<int32_t>(), true, &sorted_keys, &perm);
   if (rc != CLUE_OK) return rc;
   p->last_perm = perm;
 
   // ---- tile offsets
   CLUE_MARK(3);
-  rc = scan_exclusive(p->count.as<int32_t>(), p->offsets.as<int32_t>(), Ttot, p->part.as<int32_t>(), st);
-  if (rc != CLUE_OK) return rc;
+  k_tile_offsets<<<div_up(Ttot + 1, kThreads), kThreads, 0, st>>>(sorted_keys, n, Ttot, p->offsets.as<int32_t>());
+  CLUE_LAUNCH_CHECK("tile_offsets");
 
   // ---- tile-ordered SoA (+ Morton-ordered SoA and block boxes)
   const float h = kd.type == CLUE_KERNEL_FLAT ? kd.params[0] : 0.f;
