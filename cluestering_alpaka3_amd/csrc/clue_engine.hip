@@ -544,6 +544,7 @@ struct PassArgs {
   int32_t* pending_count;
   int32_t* pending_flag;     // per Morton position: 1 if unresolved after the first phase
   float thr1;                // first-phase search radius threshold
+  float box1;                // first-phase search box radius (<= dm)
   float lvl[8];              // second phase: squared radius thresholds of its levels, the last one dm's
   int nlvl;
   float lvl_prev0;           // threshold every first-phase candidate was examined within
@@ -1105,7 +1106,20 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
   for_each_event(valid, my_e, [&](int e, bool act) {
     const EvGrid<D> g = load_event<D>(a.ev, e, a.wrap);
     int lo[D], hi[D];
-    search_box<D>(q, a.radius, g, lo, hi);
+    if constexpr (PHASE == 1) {
+      // phase 1 only looks within thr1: a box of a slightly larger radius (1/1024
+      // of it, plus 2^-20 of the coordinate for the rounding of q -+ r) holds
+      // every such candidate and lies inside the reference's box of dm
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        float r = a.box1 + __builtin_fabsf(q[d]) * 0x1p-20f;
+        r = r < a.radius ? r : a.radius;
+        lo[d] = tile_bin(q[d] - r, g.mn[d], g.mx[d], g.ts[d], g.npd, false);
+        hi[d] = tile_bin(q[d] + r, g.mn[d], g.mx[d], g.ts[d], g.npd, false);
+      }
+    } else {
+      search_box<D>(q, a.radius, g, lo, hi);
+    }
     auto in_box = [&](int t) {
       const int tl = t - g.tile_base;
       bool in = act && tl >= 0 && tl < g.ntiles;
@@ -1839,6 +1853,7 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
   if constexpr (METRIC == MEUCLID && D <= 3) {
     if (blocks) {
       a.thr1 = sqrt_le_bound(dm * p->tuning.nh_first_radius);
+      a.box1 = dm * p->tuning.nh_first_radius * (1.f + 1.f / 1024.f);
       // second phase levels: twice the radius each time, dm last
       a.nlvl = 0;
       a.lvl_prev0 = std::min(a.thr1, a.thr);
