@@ -68,8 +68,7 @@ class ClueStats(ctypes.Structure):
 
 
 class ClueTuning(ctypes.Structure):
-    _fields_ = [("nh_first_radius", ctypes.c_float), ("density_bitmask", ctypes.c_int32),
-                ("bitmask_min_points", ctypes.c_int32), ("morton_bits", ctypes.c_int32),
+    _fields_ = [("nh_first_radius", ctypes.c_float), ("morton_bits", ctypes.c_int32),
                 ("xcd_remap", ctypes.c_int32), ("pending_blocks", ctypes.c_int32),
                 ("pending_group", ctypes.c_int32), ("density_order", ctypes.c_int32)]
 

@@ -397,7 +397,6 @@ __global__ void k_tile_offsets(const int32_t* skey, int32_t n, int32_t T, int32_
 }
 
 constexpr int kBlock = 32;       // points per block of the Morton layout
-constexpr int kMaskWords = 256;  // per-wave LDS bitmask: tiles up to 8192 points
 
 __global__ void k_block_count(const int32_t* offsets, int32_t T, int32_t* cnt) {
   const int t = blockIdx.x * kThreads + threadIdx.x;
@@ -548,7 +547,6 @@ struct PassArgs {
   float lvl[8];              // second phase: squared radius thresholds of its levels, the last one dm's
   int nlvl;
   float lvl_prev0;           // threshold every first-phase candidate was examined within
-  int32_t mask_min_points;   // density: tiles at least this large use the bitmask walk (0: never)
   float* rn_out;             // density: Morton order
   const float* rn;           // nearest-higher input: Morton order
   float* rho_s_out;          // density: tile order
@@ -780,71 +778,6 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
       if (need) eval(self_c, j, c, wj, hwj);
     }
   };
-  // Bitmask walk (Morton queries, Euclidean, no wrapping): the blocks of the
-  // Morton layout that can hold a point within dc of some lane mark their
-  // points' tile-order slots in a per-wave LDS bitmask; walking the set bits in
-  // ascending order visits exactly the candidates that can contribute, in the
-  // reference order.  Points of unmarked blocks are beyond dc for every lane
-  // (box lower bounds are exact, see lower_bound2), so their terms are +0.
-  constexpr bool kMasked = QNH && UNION && METRIC == MEUCLID;
-  __shared__ uint32_t s_mask[kMasked ? kThreads / kWave : 1][kMasked ? kMaskWords : 1];
-  float qlo[D], qhi[D];
-  if constexpr (kMasked) {
-    for (int w = lane; w < kMaskWords; w += kWave) s_mask[wv][w] = 0u;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      qlo[d] = wave_min_f32(valid ? q[d] : __builtin_inff());
-      qhi[d] = wave_max_f32(valid ? q[d] : -__builtin_inff());
-    }
-  }
-  auto run_masked = [&](auto self_c, int t, int beg, int end, bool need) {
-    uint32_t* mask = s_mask[wv];
-    const int k0 = uniform(a.blk_off[t]), k1 = uniform(a.blk_off[t + 1]);
-    for (int kb = k0; kb < k1; kb += kWave) {
-      const int k = kb + lane;
-      const bool inb = k < k1;
-      float lbw = 0.f;
-#pragma unroll
-      for (int d = D - 1; d >= 0; --d) {
-        const float bl = inb ? a.bmin[d][k] : __builtin_inff();
-        const float bh = inb ? a.bmax[d][k] : -__builtin_inff();
-        float g = bl - qhi[d];
-        const float g2 = qlo[d] - bh;
-        g = g2 > g ? g2 : g;
-        g = g > 0.f ? g : 0.f;
-        const float tt = g * g;
-        lbw = (d == D - 1) ? tt : tt + lbw;
-      }
-      uint64_t m = __ballot(inb && lbw <= a.thr);
-      while (m) {
-        const int kl = __builtin_ctzll(m);
-        m &= m - 1;
-        const int start = beg + (kb + kl - k0) * kBlock;
-        const int cnt = min(kBlock, end - start);
-        if (lane < cnt) {
-          const int bit = a.sn[start + lane] - beg;
-          atomicOr(&mask[bit >> 5], 1u << (bit & 31));
-        }
-      }
-    }
-    const int nwords = (end - beg + 31) >> 5;
-    for (int w = 0; w < nwords; ++w) {
-      uint32_t word = static_cast<uint32_t>(uniform(static_cast<int>(mask[w])));
-      if (word == 0u) continue;
-      if (lane == 0) mask[w] = 0u;
-      while (word) {
-        const int j = beg + w * 32 + __builtin_ctz(word);
-        word &= word - 1u;
-        float c[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) c[d] = X[d][j];
-        float wj = 0.f, hwj = 0.f;
-        if constexpr (kUsesHW) hwj = HW[j];
-        else wj = W[j];
-        if (need) eval(self_c, j, c, wj, hwj);
-      }
-    }
-  };
   for_each_event(valid, my_e, [&](int e, bool act) {
     const EvGrid<D> g = load_event<D>(a.ev, e, a.wrap);
     int lo[D], hi[D];
@@ -852,13 +785,6 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
     visit_box<D, UNION>(act, lo, hi, g, as_const(a.offsets), [&](int t, int beg, int end, bool need) {
       // Only the tile holding a lane's own point needs the j == i test.
       const bool self = !UNION || __ballot(s >= beg && s < end) != 0;
-      if constexpr (kMasked) {
-        if (a.mask_min_points > 0 && end - beg >= a.mask_min_points && end - beg <= kMaskWords * 32) {
-          if (self) run_masked(std::true_type{}, t, beg, end, need);
-          else run_masked(std::false_type{}, t, beg, end, need);
-          return;
-        }
-      }
       if (self) run(std::true_type{}, beg, end, need);
       else run(std::false_type{}, beg, end, need);
     });
@@ -1661,7 +1587,7 @@ struct clue_plan_s {
   DevBuf pending, pending_flag, qcost, qorder;
   // cluster -> points map (post-processing): sort keys and values
   DevBuf mkey, mkey2, mval, mval2;
-  clue_tuning tuning{0.15f, 0, 128, -1, 3, 0, 32, -1};
+  clue_tuning tuning{0.15f, -1, 3, 0, 32, -1};
   int64_t ldb = 0;  // leading dimension of the block boxes
   int64_t ld = 0;   // leading dimension of xs / xn
   // pinned staging ring of the event and chunk tables (a slot is reused once its
@@ -2293,7 +2219,6 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
     a.rn = p->rn.as<float>();
     a.pending = p->pending.as<int32_t>();
     a.pending_flag = p->pending_flag.as<int32_t>();
-    a.mask_min_points = p->tuning.density_bitmask ? std::max(p->tuning.bitmask_min_points, 1) : 0;
   }
   a.xcd = p->tuning.xcd_remap;
   a.rho_s_out = p->rho_s.as<float>();
@@ -2647,7 +2572,7 @@ clue_status clue_plan_copy_delta(clue_plan_t plan, float* delta) {
 
 clue_status clue_plan_set_tuning(clue_plan_t plan, const clue_tuning* tuning) {
   if (!plan || !tuning) return fail(CLUE_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (!(tuning->nh_first_radius > 0.f) || tuning->bitmask_min_points < 0 || tuning->morton_bits > 10 ||
+  if (!(tuning->nh_first_radius > 0.f) || tuning->morton_bits > 10 ||
       tuning->xcd_remap < 0 || tuning->xcd_remap > 3 || tuning->pending_blocks < 0 ||
       (tuning->pending_group != 8 && tuning->pending_group != 16 && tuning->pending_group != 32 &&
        tuning->pending_group != 64) ||
@@ -2935,11 +2860,9 @@ clue_status clue_autotune_batched(clue_plan_t plan, void* stream, clue_params* p
   };
   auto set_radius = [](clue_tuning& t, float v) { t.nh_first_radius = v; };
   auto set_xcd = [](clue_tuning& t, float v) { t.xcd_remap = static_cast<int32_t>(v); };
-  auto set_mask = [](clue_tuning& t, float v) { t.density_bitmask = static_cast<int32_t>(v); };
   auto set_group = [](clue_tuning& t, float v) { t.pending_group = static_cast<int32_t>(v); };
   try_field(set_radius, {0.08f, 0.12f, 0.18f, 0.25f, 0.35f, 0.5f});
   try_field(set_xcd, {0.f, 1.f, 2.f, 3.f});
-  try_field(set_mask, {0.f, 1.f});
   try_field(set_group, {8.f, 16.f, 32.f, 64.f});
   try_field([](clue_tuning& t, float v) { t.density_order = static_cast<int32_t>(v); }, {0.f, 1.f});
   try_field(set_radius, {cur.nh_first_radius * 0.8f, cur.nh_first_radius * 1.25f});

@@ -98,9 +98,6 @@ typedef struct {
 /* Launch tuning of a plan.  Every value leaves the results unchanged (they only
  * steer work order and pruning); the autotuner picks them by timing.
  *  nh_first_radius     first nearest-higher phase radius, as a fraction of dm
- *  density_bitmask     1: large tiles walk only the candidates near the wave
- *                      through an LDS bitmask (Euclidean, no wrapping)
- *  bitmask_min_points  smallest tile that takes the bitmask walk
  *  morton_bits         bits per dimension of the in-tile Morton order, -1 auto
  *  xcd_remap           bit 0: density, bit 1: nearest-higher -- give each
  *                      XCD a contiguous range of the pass's query blocks, so
@@ -112,8 +109,6 @@ typedef struct {
  *                      -1: heaviest first in 1D and 2D only */
 typedef struct {
   float nh_first_radius;
-  int32_t density_bitmask;
-  int32_t bitmask_min_points;
   int32_t morton_bits;
   int32_t xcd_remap;
   int32_t pending_blocks;

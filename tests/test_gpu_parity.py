@@ -125,13 +125,13 @@ def test_workload_parity(name, n):
     assert_same(*run_both(ps.coords, ps.weights, prm["dc"], prm["rhoc"], prm["dm"], kernel=prm["kernel"]))
 
 
-TUNINGS = [dict(nh_first_radius=0.05, density_bitmask=1, bitmask_min_points=1, morton_bits=5),
-           dict(nh_first_radius=1.0, density_bitmask=0, bitmask_min_points=0, morton_bits=1),
-           dict(nh_first_radius=0.5, density_bitmask=1, bitmask_min_points=64, morton_bits=-1),
+TUNINGS = [dict(nh_first_radius=0.05, morton_bits=5),
+           dict(nh_first_radius=1.0, morton_bits=1),
+           dict(nh_first_radius=0.5, morton_bits=-1),
            dict(xcd_remap=0, pending_blocks=1, pending_group=8),
-           dict(xcd_remap=3, pending_blocks=7, density_bitmask=0, pending_group=64),
+           dict(xcd_remap=3, pending_blocks=7, pending_group=64),
            dict(xcd_remap=2, pending_blocks=0, nh_first_radius=0.01, pending_group=32, density_order=0),
-           dict(density_order=1, density_bitmask=1, xcd_remap=0)]
+           dict(density_order=1, xcd_remap=0, pending_group=16)]
 
 
 @pytest.mark.parametrize("tuning", TUNINGS)
