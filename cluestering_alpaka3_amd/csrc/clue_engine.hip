@@ -437,16 +437,27 @@ __global__ __launch_bounds__(kThreads) void k_tile_boxes(const float* bmin, cons
                                                           int64_t ldb, int D,
                                                           const int32_t* blk_off, int32_t T,
                                                           float* tmin, float* tmax) {
-  const int t = blockIdx.x * kThreads + threadIdx.x;
-  if (t >= T) return;
+  // one wave per tile: the lanes split its blocks, a wave reduction combines
+  const int t = static_cast<int>((static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x) / kWave);
+  if (t >= T) return;  // wave-uniform
+  const int lane = __lane_id();
+  const int b0 = blk_off[t], b1 = blk_off[t + 1];
   for (int d = 0; d < D; ++d) {
     float mn = __builtin_inff(), mx = -__builtin_inff();
-    for (int b = blk_off[t]; b < blk_off[t + 1]; ++b) {
+    for (int b = b0 + lane; b < b1; b += kWave) {
       mn = bmin[d * ldb + b] < mn ? bmin[d * ldb + b] : mn;
       mx = mx < bmax[d * ldb + b] ? bmax[d * ldb + b] : mx;
     }
-    tmin[d * T + t] = mn;
-    tmax[d * T + t] = mx;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const float omn = __shfl_xor(mn, off), omx = __shfl_xor(mx, off);
+      mn = omn < mn ? omn : mn;
+      mx = mx < omx ? omx : mx;
+    }
+    if (lane == 0) {
+      tmin[d * T + t] = mn;
+      tmax[d * T + t] = mx;
+    }
   }
 }
 
@@ -1407,6 +1418,14 @@ __global__ void k_capacity(const EventDesc* ev, int n_events, const int32_t* sc,
   atomicAdd(&counters[5], ns - stored);
 }
 
+// One synchronous pointer-jumping step, parent <- parent of parent: halves the
+// distance of every node to its root (other threads' concurrent updates only
+// replace a parent by one of its ancestors).
+__global__ __launch_bounds__(kThreads) void k_jump(int32_t* root, int32_t n) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) root[i] = root[root[i]];
+}
+
 // Pointer jumping: every hop follows the current parent array, which other
 // threads only ever shorten; `hops` per round bounds the work of a round.
 // The final round writes the label -- the position of the root in its event's
@@ -2172,9 +2191,9 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
         p->xn.as<float>(), p->ld, D, p->offsets.as<int32_t>(), p->blk_off.as<int32_t>(), Ttot,
         p->bbox_lo.as<float>(), p->bbox_hi.as<float>(), p->ldb);
     CLUE_LAUNCH_CHECK("block_boxes");
-    k_tile_boxes<<<div_up(Ttot, kThreads), kThreads, 0, st>>>(p->bbox_lo.as<float>(), p->bbox_hi.as<float>(),
-                                                              p->ldb, D, p->blk_off.as<int32_t>(), Ttot,
-                                                              p->tbox_lo.as<float>(), p->tbox_hi.as<float>());
+    k_tile_boxes<<<div_up(static_cast<int64_t>(Ttot) * kWave, kThreads), kThreads, 0, st>>>(
+        p->bbox_lo.as<float>(), p->bbox_hi.as<float>(), p->ldb, D, p->blk_off.as<int32_t>(), Ttot,
+        p->tbox_lo.as<float>(), p->tbox_hi.as<float>());
     CLUE_LAUNCH_CHECK("tile_boxes");
   }
 
@@ -2262,7 +2281,12 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
       persistent_capacity ? 1 : 0);
   CLUE_LAUNCH_CHECK("capacity");
   CLUE_MARK(7);
-  // Rounds with 4096 hops each cover any forest depth below 4096^rounds.
+  // Two synchronous halvings first shorten the chains each thread then follows;
+  // rounds with 4096 hops each cover any forest depth below 4096^rounds.
+  for (int r = 0; r < 2; ++r) {
+    k_jump<<<div_up(n, kThreads), kThreads, 0, st>>>(p->root.as<int32_t>(), n);
+    CLUE_LAUNCH_CHECK("jump");
+  }
   const int hops = 4096;
   int rounds = 1;
   for (int64_t span = hops; span < n; span *= hops) ++rounds;
