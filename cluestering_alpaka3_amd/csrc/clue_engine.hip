@@ -501,15 +501,15 @@ __global__ __launch_bounds__(1024) void k_query_block_order(const int32_t* cost,
   __syncthreads();
   for (int k = 2; k <= m; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < m; i += 1024) {
-        const int l = i ^ j;
-        if (l > i) {
-          const uint64_t a = key[i], b = key[l];
-          const bool up = (i & k) == 0;
-          if ((a > b) == up) {
-            key[i] = b;
-            key[l] = a;
-          }
+      // one compare-exchange per pair (i, i + j), i with bit j clear
+      for (int pr = threadIdx.x; pr < (m >> 1); pr += 1024) {
+        const int i = ((pr & ~(j - 1)) << 1) | (pr & (j - 1));
+        const int l = i + j;
+        const uint64_t a = key[i], b = key[l];
+        const bool up = (i & k) == 0;
+        if ((a > b) == up) {
+          key[i] = b;
+          key[l] = a;
         }
       }
       __syncthreads();
@@ -1241,27 +1241,78 @@ __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
     // higher point is the last, since everything nearer has then been examined.
     float prev = a.lvl_prev0;
     bool found = false;
+    // The box's tiles, one per lane of the group when they fit (their boxes and
+    // ranges are then loaded once, together, and shuffled to the group at each
+    // level); otherwise each level walks them and loads them one by one.
+    int dims[D];
+    int nt = 1;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      dims[d] = hi[d] - lo[d] + 1;
+      nt *= dims[d];
+    }
+    const bool cached = nt <= G;
+    auto tile_of = [&](int ti) {
+      int bb[D];
+      int rem = ti;
+#pragma unroll
+      for (int d = D - 1; d >= 0; --d) {
+        bb[d] = lo[d] + rem % dims[d];
+        rem /= dims[d];
+      }
+      return tile_index<D>(bb, g);
+    };
+    int my_t = 0, my_k0 = 0, my_k1 = 0, my_o0 = 0, my_o1 = 0;
+    float my_tl[D], my_th[D];
+    if (cached) {
+      const bool tv = active && gl < nt;
+      my_t = tv ? tile_of(gl) : 0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        my_tl[d] = tv ? a.tmin[d][my_t] : __builtin_inff();
+        my_th[d] = tv ? a.tmax[d][my_t] : -__builtin_inff();
+      }
+      my_k0 = tv ? a.blk_off[my_t] : 0;
+      my_k1 = tv ? a.blk_off[my_t + 1] : 0;
+      my_o0 = tv ? a.offsets[my_t] : 0;
+      my_o1 = tv ? a.offsets[my_t + 1] : 0;
+    }
     for (int lv = 0; lv < a.nlvl; ++lv) {
       const float cap = a.lvl[lv];
-      int b[D];
-#pragma unroll
-      for (int d = 0; d < D; ++d) b[d] = lo[d];
+      int ti = 0;
       bool more = active && !found;
       while (__ballot(more) != 0) {
         if (more) {  // uniform within a group
-          const int t = tile_index<D>(b, g);
+          int t, k0, k1, off0, off1;
           float tl[D], th[D];
+          if (cached) {
+            const int src = grp_shift + ti;
+            t = __shfl(my_t, src);
+            k0 = __shfl(my_k0, src);
+            k1 = __shfl(my_k1, src);
+            off0 = __shfl(my_o0, src);
+            off1 = __shfl(my_o1, src);
 #pragma unroll
-          for (int d = 0; d < D; ++d) {
-            tl[d] = a.tmin[d][t];
-            th[d] = a.tmax[d][t];
+            for (int d = 0; d < D; ++d) {
+              tl[d] = __shfl(my_tl[d], src);
+              th[d] = __shfl(my_th[d], src);
+            }
+          } else {
+            t = tile_of(ti);
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+              tl[d] = a.tmin[d][t];
+              th[d] = a.tmax[d][t];
+            }
+            k0 = a.blk_off[t];
+            k1 = a.blk_off[t + 1];
+            off0 = a.offsets[t];
+            off1 = a.offsets[t + 1];
           }
           const float lim_t = cap < group_bound ? cap : group_bound;
           if (lower_bound2<D>(q, tl, th) <= lim_t && upper_bound2<D>(q, tl, th) > prev) {
             // the tile's Morton blocks: G at a time against the group's bound,
             // then the points of each block that can still hold a better candidate
-            const int k0 = a.blk_off[t], k1 = a.blk_off[t + 1];
-            const int off0 = a.offsets[t], off1 = a.offsets[t + 1];
             for (int kb = k0; kb < k1; kb += G) {
               const int k = kb + gl;
               const bool inb = k < k1;
@@ -1306,12 +1357,7 @@ __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
               group_bound = group_min_f32<G>(bound);  // the point's best so far over its lanes
             }
           }
-          int d = D - 1;
-          for (; d >= 0; --d) {
-            if (++b[d] <= hi[d]) break;
-            b[d] = lo[d];
-          }
-          more = d >= 0;
+          more = ++ti < nt;
         }
       }
       uint64_t fm = __ballot(best != INT64_MAX) >> grp_shift;
