@@ -150,7 +150,8 @@ __device__ __forceinline__ float engine_expf(float x) {
 // engine_expf for arguments known to be <= 0 and not NaN (the Gaussian and
 // exponential kernels with finite positive width / decay): the same sequence
 // without the upper clamp and the NaN pass-through, hence the same bits.
-__device__ __forceinline__ float engine_expf_nonpos(float x) {
+// table: kExp2Table or a copy of it (in LDS for the density pass).
+__device__ __forceinline__ float engine_expf_nonpos(float x, const double* table = kExp2Table) {
   const double xd = static_cast<double>(__builtin_fmaxf(x, -110.0f));
   const double kd = __builtin_rint(xd * 0x1.71547652b82fep+6);
   const int k = static_cast<int>(kd);
@@ -162,7 +163,7 @@ __device__ __forceinline__ float engine_expf_nonpos(float x) {
   p = __builtin_fma(r * r, p, r);
   const int j = k & 63;
   const int m = (k - j) / 64;
-  const double t = kExp2Table[j];
+  const double t = table[j];
   return static_cast<float>(__builtin_ldexp(__builtin_fma(t, p, t), m));
 }
 
@@ -194,17 +195,18 @@ enum { KFLAT = 0, KGAUSS = 1, KEXP = 2 };
 // rcp_den: 1 / ((2*sigma)*sigma) rounded to double (Gaussian only).
 // NONPOS: the exp argument is known to be <= 0 and not NaN (checked on the host).
 template <int KERNEL, bool NONPOS = false>
-__device__ __forceinline__ float kernel_value(float dist, float p0, float p1, float p2, double rcp_den) {
+__device__ __forceinline__ float kernel_value(float dist, float p0, float p1, float p2, double rcp_den,
+                                              const double* table = kExp2Table) {
   if constexpr (KERNEL == KFLAT) {
     return p0;
   } else if constexpr (KERNEL == KGAUSS) {
     const float a = dist - p0;
     const float num = (-a) * a;  // -(d-mu)*(d-mu), divided by (2*sigma)*sigma
     const float x = quotient_via_rcp(num, rcp_den);
-    return p2 * (NONPOS ? engine_expf_nonpos(x) : engine_expf(x));
+    return p2 * (NONPOS ? engine_expf_nonpos(x, table) : engine_expf(x));
   } else {
     const float x = (-p0) * dist;
-    return p1 * (NONPOS ? engine_expf_nonpos(x) : engine_expf(x));
+    return p1 * (NONPOS ? engine_expf_nonpos(x, table) : engine_expf(x));
   }
 }
 

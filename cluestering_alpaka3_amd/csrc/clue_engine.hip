@@ -695,6 +695,12 @@ __device__ __forceinline__ float readlane_f32(float v, int lane) {
 // so the lanes of a wave share most of their search box); otherwise in tile order.
 template <int D, int METRIC, int KERNEL, bool UNION, bool QNH>
 __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
+  // the exponential's 2^(j/64) table, read through LDS by the kernel evaluations
+  __shared__ double s_exp2[KERNEL == KFLAT ? 1 : 64];
+  if constexpr (KERNEL != KFLAT) {
+    if (threadIdx.x < 64) s_exp2[threadIdx.x] = kExp2Table[threadIdx.x];
+    __syncthreads();
+  }
   const int qb = a.qorder ? a.qorder[blockIdx.x] : logical_block(a.xcd & 1);
   const int pos = qb * kThreads + threadIdx.x;
   const bool valid = pos < a.n;
@@ -744,7 +750,8 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
           float k = 1.f;
           if (j != s) {
             if (a.exp_nonpos && s2 >= 0x1p-96f) {
-              k = kernel_value<KERNEL, true>(sqrt_rn_normal(s2), a.kp[0], a.kp[1], a.kp[2], a.kden_rcp);
+              k = kernel_value<KERNEL, true>(sqrt_rn_normal(s2), a.kp[0], a.kp[1], a.kp[2], a.kden_rcp,
+                                             s_exp2);
             } else {
               k = kernel_value<KERNEL>(__builtin_sqrtf(s2), a.kp[0], a.kp[1], a.kp[2], a.kden_rcp);
             }
