@@ -283,6 +283,50 @@ __device__ __forceinline__ float sqrt_lt_bound(float delta) {
   return t;
 }
 
+// The floats s >= 0 with sqrtf(s) == d, for d the sqrtf of a finite float s2 >= 0,
+// form an interval [lo, hi].  sqrtf rounds to nearest and the square root of a
+// float is never half-way between two floats, so s lies in it exactly when
+// (d - g_lo/2)^2 < s < (d + g_hi/2)^2, with g_lo and g_hi the gaps from d to the
+// floats beside it.  Both midpoints have at most 25 significant bits and their
+// squares at most 50, so the squares are exact in double precision and are not
+// floats themselves.
+__device__ __forceinline__ void sqrt_class(float d, float& lo, float& hi) {
+  const double dd = d;
+  const double mh = 0.5 * (dd + static_cast<double>(__uint_as_float(__float_as_uint(d) + 1u)));
+  const double th = mh * mh;
+  float h = static_cast<float>(th);
+  if (static_cast<double>(h) > th) h = __uint_as_float(__float_as_uint(h) - 1u);
+  hi = h;
+  float l = 0.f;
+  if (d > 0.f) {
+    const double ml = 0.5 * (dd + static_cast<double>(__uint_as_float(__float_as_uint(d) - 1u)));
+    const double tl = ml * ml;
+    l = static_cast<float>(tl);
+    if (static_cast<double>(l) < tl) l = __uint_as_float(__float_as_uint(l) + 1u);
+  }
+  lo = l;
+}
+
+// Order key of the nearest-higher relation (ClusteringKernels.hpp:132-134): point j
+// is higher than point i when rho_j > rho_i, or rho_j == rho_i > 0 and j > i.  With
+//   rank_key(rho, id) = ord(rho) << 32 | id,
+// ord monotone in rho, equal for -0 and +0, and 0 (below every number) for NaN,
+// j is higher than i exactly when rank_key(rho_j, j) > higher_threshold(rank_key(rho_i, i)).
+__device__ __forceinline__ uint64_t rank_key(float rho, int32_t id) {
+  uint32_t b = __float_as_uint(rho);
+  if (b == 0x80000000u) b = 0u;
+  uint32_t ord = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+  if (rho != rho) ord = 0u;
+  return (static_cast<uint64_t>(ord) << 32) | static_cast<uint32_t>(id);
+}
+// rho > 0: the key itself (equal densities are ordered by id); rho <= 0: above
+// every key of that density (no tie-break); NaN: above every key.
+__device__ __forceinline__ uint64_t higher_threshold(uint64_t key) {
+  const uint32_t ord = static_cast<uint32_t>(key >> 32);
+  if (ord == 0u) return ~uint64_t{0};
+  return ord > 0x80000000u ? key : (key | 0xffffffffull);
+}
+
 __device__ __forceinline__ int wave_min_i32(int v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {

@@ -558,8 +558,8 @@ struct PassArgs {
   float lvl[8];              // second phase: squared radius thresholds of its levels, the last one dm's
   int nlvl;
   float lvl_prev0;           // threshold every first-phase candidate was examined within
-  float* rn_out;             // density: Morton order
-  const float* rn;           // nearest-higher input: Morton order
+  uint64_t* hk_out;          // density: rank_key of each Morton position
+  const uint64_t* hk;        // nearest-higher input: rank_key, Morton order
   float* rho_s_out;          // density: tile order
   float* rho_out;            // density: original order
   int32_t* nh_out;           // nearest-higher: original order
@@ -810,8 +810,9 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
   if (valid) {
     a.rho_s_out[s] = rho;
     if constexpr (QNH) {
-      a.rn_out[pos] = rho;
-      a.rho_out[a.pn[pos]] = rho;
+      const int32_t id = a.pn[pos];
+      a.hk_out[pos] = rank_key(rho, id);
+      a.rho_out[id] = rho;
     } else {
       a.rho_out[a.perm[s]] = rho;
     }
@@ -932,21 +933,6 @@ __device__ __forceinline__ float upper_bound2(const float* q, const float* lo, c
   return s;
 }
 
-// Largest float s with sqrtf(s) <= delta.
-__device__ __forceinline__ float sqrt_le_bound_dev(float delta) {
-  if (!(delta >= 0.f)) return -1.f;
-  float t = delta * delta;
-  if (!(t <= FLT_MAX)) t = FLT_MAX;
-  while (t > 0.f && !(__builtin_sqrtf(t) <= delta)) t = __int_as_float(__float_as_int(t) - 1);
-  if (!(__builtin_sqrtf(t) <= delta)) return -1.f;
-  while (t < FLT_MAX) {
-    const float nt = __int_as_float(__float_as_int(t) + 1);
-    if (!(__builtin_sqrtf(nt) <= delta)) break;
-    t = nt;
-  }
-  return t;
-}
-
 // Nearest-higher (K5) as a block search over the Morton-ordered layout
 // (Euclidean metric, no wrapped dimension).  The result is the serial one: the
 // candidate minimising (distance, visit position) where the visit position is
@@ -976,8 +962,9 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
   float q[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) q[d] = a.xn[d][pos];
-  const float rho_i = a.rn[pos];
-  const int32_t id_i = a.pn[pos];
+  const uint64_t key_i = a.hk[pos];
+  const uint64_t thr_i = higher_threshold(key_i);
+  const int32_t id_i = static_cast<int32_t>(key_i & 0xffffffffu);
   const int my_e = a.pev[id_i];
   cptr<float> X[D], TMIN[D], TMAX[D];
 #pragma unroll
@@ -986,16 +973,19 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
     TMIN[d] = as_const(a.tmin[d]);
     TMAX[d] = as_const(a.tmax[d]);
   }
-  cptr<float> R = as_const(a.rn);
-  cptr<int32_t> P = as_const(a.pn);
+  cptr<uint64_t> HK = as_const(a.hk);
   cptr<int32_t> OFF = as_const(a.offsets);
   cptr<int32_t> BOFF = as_const(a.blk_off);
   float delta = FLT_MAX;
   int32_t nh = -1;
   int64_t best = INT64_MAX;
-  // candidates need s2 <= bound = min(search radius, max{s : sqrtf(s) <= delta})
+  // The best candidate so far is at distance delta; the squared distances with
+  // sqrtf equal to delta are [lo2, hi2].  A candidate replaces it when it is
+  // nearer (s2 < lo2) or as near and earlier in the visit order (key < best), so
+  // candidates need s2 <= bound = min(search radius, hi2).
   const float cap = (PHASE == 1 && a.thr1 < a.thr) ? a.thr1 : a.thr;
   float bound = cap;
+  float lo2 = __builtin_inff();
   // the wave's query box, for the block prefilter
   float qlo[D], qhi[D];
 #pragma unroll
@@ -1004,21 +994,16 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
     qhi[d] = wave_max_f32(valid ? q[d] : -__builtin_inff());
   }
 
-  auto cand = [&](int t, const float* c, float rho_j, int32_t id_j) {
-    const int rb = __float_as_int(rho_j);
-    const bool pos_rho = (rb > 0) && (rb <= 0x7f800000);
-    const bool higher = (rho_j > rho_i) || ((rho_j == rho_i) && pos_rho && (id_j > id_i));
+  auto cand = [&](int t, const float* c, uint64_t key_j) {
     const float s2 = squared_distance<D>(q, c);
-    if (higher && s2 <= bound) {
-      const float dist = __builtin_sqrtf(s2);
-      const int64_t key = (static_cast<int64_t>(t) << 32) | static_cast<uint32_t>(id_j);
-      if (dist < delta || key < best) {
-        delta = dist;
-        nh = id_j;
-        best = key;
-        const float lim = sqrt_le_bound_dev(dist);
-        bound = lim < cap ? lim : cap;
-      }
+    const int64_t key = (static_cast<int64_t>(t) << 32) | static_cast<uint32_t>(key_j);
+    if (key_j > thr_i && s2 <= bound && (s2 < lo2 || key < best)) {
+      delta = __builtin_sqrtf(s2);
+      nh = static_cast<int32_t>(key_j & 0xffffffffu);
+      best = key;
+      float hi2;
+      sqrt_class(delta, lo2, hi2);
+      bound = hi2 < cap ? hi2 : cap;
     }
   };
   // candidates [beg, end) of tile t, wave-uniform: eight at a time, all loads
@@ -1027,23 +1012,22 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
     constexpr int U = 8;
     int j = beg;
     for (; j + U <= end; j += U) {
-      float c[U][D], r[U];
-      int32_t id[U];
+      float c[U][D];
+      uint64_t k[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
 #pragma unroll
         for (int d = 0; d < D; ++d) c[u][d] = X[d][j + u];
-        r[u] = R[j + u];
-        id[u] = P[j + u];
+        k[u] = HK[j + u];
       }
 #pragma unroll
-      for (int u = 0; u < U; ++u) cand(t, c[u], r[u], id[u]);
+      for (int u = 0; u < U; ++u) cand(t, c[u], k[u]);
     }
     for (; j < end; ++j) {
       float c[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) c[d] = X[d][j];
-      cand(t, c, R[j], P[j]);
+      cand(t, c, HK[j]);
     }
   };
 
@@ -1219,8 +1203,9 @@ __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
     float q[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) q[d] = a.xn[d][pos];
-    const float rho_i = a.rn[pos];
-    const int32_t id_i = a.pn[pos];
+    const uint64_t key_i = a.hk[pos];
+    const uint64_t thr_i = higher_threshold(key_i);
+    const int32_t id_i = static_cast<int32_t>(key_i & 0xffffffffu);
     const int e = a.pev[id_i];
     const EventDesc* ev = a.ev + e;
     EvGrid<D> g;
@@ -1240,7 +1225,10 @@ __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
     search_box<D>(q, a.radius, g, lo, hi);
     float delta = FLT_MAX;
     int64_t best = INT64_MAX;
-    float bound = a.thr;  // candidates need s2 <= min(thr(dm), max{s : sqrtf(s) <= delta})
+    // candidates need s2 <= bound = min(thr(dm), hi2), and replace the lane's best
+    // when s2 < lo2 or, as near, key < best ([lo2, hi2]: the sqrtf class of delta)
+    float bound = a.thr;
+    float lo2 = __builtin_inff();
     float group_bound = a.thr;
     // Levels of growing radius: a level examines the candidates within its
     // radius that lie outside the previous one (blocks entirely inside were
@@ -1343,21 +1331,15 @@ __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
                   float c[D];
 #pragma unroll
                   for (int d = 0; d < D; ++d) c[d] = a.xn[d][j];
-                  const float rho_j = a.rn[j];
-                  const int32_t id_j = a.pn[j];
-                  const int rb = __float_as_int(rho_j);
-                  const bool pos_rho = (rb > 0) && (rb <= 0x7f800000);  // rho_j > 0.f
-                  const bool higher = (rho_j > rho_i) || ((rho_j == rho_i) && pos_rho && (id_j > id_i));
+                  const uint64_t key_j = a.hk[j];
                   const float s2 = squared_distance<D>(q, c);
-                  if (higher && s2 <= lane_cap && s2 <= bound) {
-                    const float dist = __builtin_sqrtf(s2);
-                    const int64_t key = (static_cast<int64_t>(t) << 32) | static_cast<uint32_t>(id_j);
-                    if (dist < delta || key < best) {
-                      delta = dist;
-                      best = key;
-                      const float lim = sqrt_le_bound_dev(dist);
-                      bound = lim < a.thr ? lim : a.thr;
-                    }
+                  const int64_t key = (static_cast<int64_t>(t) << 32) | static_cast<uint32_t>(key_j);
+                  if (key_j > thr_i && s2 <= lane_cap && s2 <= bound && (s2 < lo2 || key < best)) {
+                    delta = __builtin_sqrtf(s2);
+                    best = key;
+                    float hi2;
+                    sqrt_class(delta, lo2, hi2);
+                    bound = hi2 < a.thr ? hi2 : a.thr;
                   }
                 }
               }
@@ -1572,6 +1554,13 @@ __global__ void k_selftest(int op, const float* a, const float* b, float* out, i
     case 3: out[i] = sqrt_lt_bound(x); break;
     case 4: out[i] = quotient_via_rcp(x, 1.0 / static_cast<double>(b[i])); break;
     case 5: out[i] = engine_expf_nonpos(x); break;
+    case 6:
+    case 7: {
+      float lo, hi;
+      sqrt_class(__builtin_sqrtf(x), lo, hi);
+      out[i] = op == 6 ? hi : lo;
+      break;
+    }
     default: out[i] = 0.f;
   }
 }
@@ -1655,7 +1644,7 @@ struct clue_plan_s {
   // events of the call: descriptors, chunks, point -> event, per-event counters
   DevBuf events, chunks, pev, sc, cap_used, rank_base;
   // Morton-ordered layout and the block boxes of the nearest-higher search
-  DevBuf kn, kn2, pn, pn2, mid, xn, wn, rn, bcnt, blk_off, bbox_lo, bbox_hi, tbox_lo, tbox_hi;
+  DevBuf kn, kn2, pn, pn2, mid, xn, wn, hk, bcnt, blk_off, bbox_lo, bbox_hi, tbox_lo, tbox_hi;
   DevBuf pending, pending_flag, qcost, qorder;
   // cluster -> points map (post-processing): sort keys and values
   DevBuf mkey, mkey2, mval, mval2;
@@ -1688,7 +1677,7 @@ struct clue_plan_s {
                    &xs,      &ws,     &hw,       &rho_s,   &delta,   &flag,     &root,    &nh_tmp,
                    &rho_tmp, &seed_tmp, &bbox_part, &counters, &events, &chunks, &pev,    &sc,
                    &cap_used, &rank_base, &kn,   &kn2,     &pn,      &pn2,      &mid,     &xn,
-                   &wn,      &rn,       &bcnt,    &blk_off, &bbox_lo,  &bbox_hi, &tbox_lo,
+                   &wn,      &hk,       &bcnt,    &blk_off, &bbox_lo,  &bbox_hi, &tbox_lo,
                    &tbox_hi, &pending, &pending_flag, &qcost, &qorder, &mkey,  &mkey2,  &mval,    &mval2};
     return k < static_cast<int>(sizeof(b) / sizeof(b[0])) ? b[k] : nullptr;
   }
@@ -1761,7 +1750,7 @@ clue_status reserve(clue_plan_s* p, int32_t D, int32_t n, int32_t T, int32_t n_e
   CLUE_HIP(p->mid.ensure(N * 4));
   CLUE_HIP(p->xn.ensure(static_cast<size_t>(ld) * D * 4));
   CLUE_HIP(p->wn.ensure(N * 4));
-  CLUE_HIP(p->rn.ensure(N * 4));
+  CLUE_HIP(p->hk.ensure(N * 8));
   CLUE_HIP(p->bcnt.ensure(TT * 4));
   CLUE_HIP(p->blk_off.ensure(TT * 4));
   const int64_t ldb = div_up(n, kBlock) + T + 1;
@@ -2287,8 +2276,8 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
     a.kn = mkeys;
     a.mshift = sub_bits * D;
     a.blk_off = p->blk_off.as<int32_t>();
-    a.rn_out = p->rn.as<float>();
-    a.rn = p->rn.as<float>();
+    a.hk_out = p->hk.as<uint64_t>();
+    a.hk = p->hk.as<uint64_t>();
     a.pending = p->pending.as<int32_t>();
     a.pending_flag = p->pending_flag.as<int32_t>();
   }

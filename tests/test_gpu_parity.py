@@ -114,6 +114,29 @@ def test_device_math_matches_host():
     assert np.all(np.sqrt(nxt) >= delta)
 
 
+def test_sqrt_class_bounds():
+    # ops 6 / 7: the interval of squared distances whose sqrtf equals sqrtf(s), used
+    # by the nearest-higher searches to compare distances in the squared domain
+    f32 = np.float32
+    rng = np.random.default_rng(7)
+    bits = np.arange(0, int(f32(1e4).view(np.uint32)) + 1, 97, dtype=np.uint64).astype(np.uint32)
+    pw = np.array([f32(2.0) ** k for k in range(-140, 64)], f32).view(np.uint32)
+    near = (pw[:, None].astype(np.int64) + np.arange(-3, 4)[None, :]).ravel()
+    near = near[(near >= 0)].astype(np.uint32)
+    rnd = (rng.random(200000) * 64.0).astype(f32)
+    s = np.concatenate([bits.view(f32), near.view(f32), rnd, np.array([0.0, 1e-45, 3e-45, 1e-40], f32)])
+    d = np.sqrt(s)
+    for k in range(0, s.size, 1 << 23):
+        ss, dd = s[k:k + (1 << 23)], d[k:k + (1 << 23)]
+        hi = clue.selftest_math(6, ss)
+        lo = clue.selftest_math(7, ss)
+        assert np.all(lo <= ss) and np.all(ss <= hi)
+        assert np.array_equal(np.sqrt(hi), dd) and np.array_equal(np.sqrt(lo), dd)
+        assert np.all(np.sqrt(np.nextafter(hi, f32(np.inf))) > dd)
+        below = np.nextafter(lo, f32(-np.inf))
+        assert np.all((lo == 0) | (np.sqrt(np.maximum(below, f32(0))) < dd))
+
+
 WORKLOAD_CASES = [("2d_16k_flat", 16384), ("2d_1m_gauss", 65536), ("3d_4m_exp", 65536),
                   ("10d_256k_flat", 8192)]
 
