@@ -461,6 +461,47 @@ __global__ __launch_bounds__(kThreads) void k_tile_boxes(const float* bmin, cons
   }
 }
 
+// Largest rank key of each block and of each tile (the density filter of the
+// nearest-higher searches: a block whose largest key does not exceed a query's
+// threshold holds no point higher than it).  One lane per Morton position; runs
+// of equal block (tile) inside the wave are reduced by a segmented suffix
+// maximum and their first lane combines the run into the block (tile) with an
+// atomic maximum (a block spans at most two waves).  bmaxk and tmaxk start at 0.
+__global__ __launch_bounds__(kThreads) void k_block_maxkey(const uint64_t* hk, const int32_t* kn, int mshift,
+                                                            const int32_t* offsets, const int32_t* blk_off,
+                                                            int32_t n, unsigned long long* bmaxk,
+                                                            unsigned long long* tmaxk) {
+  const int p = blockIdx.x * kThreads + threadIdx.x;
+  const int lane = __lane_id();
+  const bool valid = p < n;
+  int t = -1, b = -1;
+  uint64_t v = 0;
+  if (valid) {
+    t = kn[p] >> mshift;
+    b = blk_off[t] + (p - offsets[t]) / kBlock;
+    v = hk[p];
+  }
+  uint64_t vt = v;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const int ob = __shfl_down(b, off);
+    const int ot = __shfl_down(t, off);
+    const uint32_t vlo = __shfl_down(static_cast<uint32_t>(v), off);
+    const uint32_t vhi = __shfl_down(static_cast<uint32_t>(v >> 32), off);
+    const uint32_t tlo = __shfl_down(static_cast<uint32_t>(vt), off);
+    const uint32_t thi = __shfl_down(static_cast<uint32_t>(vt >> 32), off);
+    const uint64_t ov = (static_cast<uint64_t>(vhi) << 32) | vlo;
+    const uint64_t otv = (static_cast<uint64_t>(thi) << 32) | tlo;
+    const bool in = lane + off < kWave;
+    if (in && ob == b && ov > v) v = ov;
+    if (in && ot == t && otv > vt) vt = otv;
+  }
+  const int pb = __shfl_up(b, 1);
+  const int pt = __shfl_up(t, 1);
+  if (valid && (lane == 0 || pb != b)) atomicMax(bmaxk + b, static_cast<unsigned long long>(v));
+  if (valid && (lane == 0 || pt != t)) atomicMax(tmaxk + t, static_cast<unsigned long long>(vt));
+}
+
 // Estimated cost of each block of kThreads Morton-ordered queries: the sum of
 // the sizes of the queries' tiles (the candidates grow with the local density).
 __global__ __launch_bounds__(kThreads) void k_query_block_cost(const int32_t* kn, int mshift,
@@ -560,6 +601,8 @@ struct PassArgs {
   float lvl_prev0;           // threshold every first-phase candidate was examined within
   uint64_t* hk_out;          // density: rank_key of each Morton position
   const uint64_t* hk;        // nearest-higher input: rank_key, Morton order
+  const uint64_t* bmaxk;     // largest rank_key of each block
+  const uint64_t* tmaxk;     // largest rank_key of each tile
   float* rho_s_out;          // density: tile order
   float* rho_out;            // density: original order
   int32_t* nh_out;           // nearest-higher: original order
@@ -689,6 +732,24 @@ __device__ __forceinline__ float wave_min_f32(float v) {
 }
 __device__ __forceinline__ float readlane_f32(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int lane) {
+  const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(v)), lane);
+  const uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(v >> 32)), lane);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int off) {
+  const uint32_t lo = __shfl_xor(static_cast<uint32_t>(v), off);
+  const uint32_t hi = __shfl_xor(static_cast<uint32_t>(v >> 32), off);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint64_t o = shfl_xor_u64(v, off);
+    v = o < v ? o : v;
+  }
+  return v;
 }
 
 // QNH: the lanes take their queries in Morton order (spatially coherent waves,
@@ -974,8 +1035,14 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
     TMAX[d] = as_const(a.tmax[d]);
   }
   cptr<uint64_t> HK = as_const(a.hk);
+  cptr<uint64_t> BMK = as_const(a.bmaxk);
+  cptr<uint64_t> TMK = as_const(a.tmaxk);
   cptr<int32_t> OFF = as_const(a.offsets);
   cptr<int32_t> BOFF = as_const(a.blk_off);
+  // a block or tile whose largest rank key is at most a lane's threshold holds
+  // no point higher than that lane's; at most the wave's least threshold, none
+  // higher than any of its lanes'
+  const uint64_t wthr = uniform(wave_min_u64(valid ? thr_i : ~uint64_t{0}));
   float delta = FLT_MAX;
   int32_t nh = -1;
   int64_t best = INT64_MAX;
@@ -1096,7 +1163,7 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
           tl[d] = TMIN[d][t];
           th[d] = TMAX[d][t];
         }
-        need = need && lower_bound2<D>(q, tl, th) <= bound;
+        need = need && TMK[t] > thr_i && lower_bound2<D>(q, tl, th) <= bound;
         const float maxb = wave_max_f32(need ? bound : -1.f);
         if (maxb >= 0.f) {
           const int k0 = BOFF[t], k1 = BOFF[t + 1];
@@ -1111,6 +1178,7 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
               bl[d] = inb ? a.bmin[d][k] : __builtin_inff();
               bh[d] = inb ? a.bmax[d][k] : -__builtin_inff();
             }
+            const uint64_t bk = inb ? BMK[k] : 0;
             float lbw = 0.f;
 #pragma unroll
             for (int d = D - 1; d >= 0; --d) {
@@ -1121,7 +1189,7 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
               const float tt = gg * gg;
               lbw = (d == D - 1) ? tt : tt + lbw;
             }
-            uint64_t m = __ballot(inb && lbw <= maxb);
+            uint64_t m = __ballot(inb && lbw <= maxb && bk > wthr);
             while (m) {
               const int kl = __builtin_ctzll(m);
               m &= m - 1;
@@ -1131,7 +1199,7 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
                 ul[d] = readlane_f32(bl[d], kl);
                 uh[d] = readlane_f32(bh[d], kl);
               }
-              const bool nk = need && lower_bound2<D>(q, ul, uh) <= bound;
+              const bool nk = need && readlane_u64(bk, kl) > thr_i && lower_bound2<D>(q, ul, uh) <= bound;
               if (__ballot(nk) != 0) {
                 const int start = off0 + (kb + kl - k0) * kBlock;
                 const int end = min(start + kBlock, off1);
@@ -1258,6 +1326,7 @@ __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
       return tile_index<D>(bb, g);
     };
     int my_t = 0, my_k0 = 0, my_k1 = 0, my_o0 = 0, my_o1 = 0;
+    uint64_t my_tk = 0;
     float my_tl[D], my_th[D];
     if (cached) {
       const bool tv = active && gl < nt;
@@ -1271,6 +1340,7 @@ __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
       my_k1 = tv ? a.blk_off[my_t + 1] : 0;
       my_o0 = tv ? a.offsets[my_t] : 0;
       my_o1 = tv ? a.offsets[my_t + 1] : 0;
+      my_tk = tv ? a.tmaxk[my_t] : 0;
     }
     for (int lv = 0; lv < a.nlvl; ++lv) {
       const float cap = a.lvl[lv];
@@ -1279,10 +1349,13 @@ __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
       while (__ballot(more) != 0) {
         if (more) {  // uniform within a group
           int t, k0, k1, off0, off1;
+          uint64_t tk;
           float tl[D], th[D];
           if (cached) {
             const int src = grp_shift + ti;
             t = __shfl(my_t, src);
+            tk = (static_cast<uint64_t>(__shfl(static_cast<uint32_t>(my_tk >> 32), src)) << 32) |
+                 __shfl(static_cast<uint32_t>(my_tk), src);
             k0 = __shfl(my_k0, src);
             k1 = __shfl(my_k1, src);
             off0 = __shfl(my_o0, src);
@@ -1303,9 +1376,11 @@ __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
             k1 = a.blk_off[t + 1];
             off0 = a.offsets[t];
             off1 = a.offsets[t + 1];
+            tk = a.tmaxk[t];
           }
           const float lim_t = cap < group_bound ? cap : group_bound;
-          if (lower_bound2<D>(q, tl, th) <= lim_t && upper_bound2<D>(q, tl, th) > prev) {
+          // tiles and blocks without a point higher than this one are skipped
+          if (tk > thr_i && lower_bound2<D>(q, tl, th) <= lim_t && upper_bound2<D>(q, tl, th) > prev) {
             // the tile's Morton blocks: G at a time against the group's bound,
             // then the points of each block that can still hold a better candidate
             for (int kb = k0; kb < k1; kb += G) {
@@ -1317,8 +1392,10 @@ __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
                 bl[d] = inb ? a.bmin[d][k] : __builtin_inff();
                 bh[d] = inb ? a.bmax[d][k] : -__builtin_inff();
               }
+              const uint64_t bk = inb ? a.bmaxk[k] : 0;
               const float lim_b = cap < group_bound ? cap : group_bound;
-              const bool pass = inb && lower_bound2<D>(q, bl, bh) <= lim_b && upper_bound2<D>(q, bl, bh) > prev;
+              const bool pass = inb && bk > thr_i && lower_bound2<D>(q, bl, bh) <= lim_b &&
+                                upper_bound2<D>(q, bl, bh) > prev;
               uint64_t gm = __ballot(pass) >> grp_shift;
               if constexpr (G < kWave) gm &= (uint64_t{1} << G) - 1;
               while (gm) {
@@ -1645,6 +1722,7 @@ struct clue_plan_s {
   DevBuf events, chunks, pev, sc, cap_used, rank_base;
   // Morton-ordered layout and the block boxes of the nearest-higher search
   DevBuf kn, kn2, pn, pn2, mid, xn, wn, hk, bcnt, blk_off, bbox_lo, bbox_hi, tbox_lo, tbox_hi;
+  DevBuf bmaxk, tmaxk;  // largest rank key of each block / tile
   DevBuf pending, pending_flag, qcost, qorder;
   // cluster -> points map (post-processing): sort keys and values
   DevBuf mkey, mkey2, mval, mval2;
@@ -1678,7 +1756,8 @@ struct clue_plan_s {
                    &rho_tmp, &seed_tmp, &bbox_part, &counters, &events, &chunks, &pev,    &sc,
                    &cap_used, &rank_base, &kn,   &kn2,     &pn,      &pn2,      &mid,     &xn,
                    &wn,      &hk,       &bcnt,    &blk_off, &bbox_lo,  &bbox_hi, &tbox_lo,
-                   &tbox_hi, &pending, &pending_flag, &qcost, &qorder, &mkey,  &mkey2,  &mval,    &mval2};
+                   &tbox_hi, &bmaxk, &tmaxk, &pending, &pending_flag, &qcost, &qorder, &mkey,  &mkey2,
+                   &mval,    &mval2};
     return k < static_cast<int>(sizeof(b) / sizeof(b[0])) ? b[k] : nullptr;
   }
 };
@@ -1759,6 +1838,8 @@ clue_status reserve(clue_plan_s* p, int32_t D, int32_t n, int32_t T, int32_t n_e
   p->ldb = ldb;
   CLUE_HIP(p->tbox_lo.ensure(static_cast<size_t>(T) * D * 4));
   CLUE_HIP(p->tbox_hi.ensure(static_cast<size_t>(T) * D * 4));
+  CLUE_HIP(p->bmaxk.ensure(static_cast<size_t>(ldb) * 8));
+  CLUE_HIP(p->tmaxk.ensure(static_cast<size_t>(T) * 8));
   CLUE_HIP(p->pending.ensure((N + 64) * 4));
   CLUE_HIP(p->pending_flag.ensure((N + 1) * 4));
   CLUE_HIP(p->qcost.ensure(static_cast<size_t>(div_up(n, kThreads)) * 4));
@@ -1853,6 +1934,10 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
         }
         a.lvl[a.nlvl++] = a.thr;
       }
+      k_block_maxkey<<<grid, kThreads, 0, st>>>(a.hk, a.kn, a.mshift, a.offsets, a.blk_off, a.n,
+                                                reinterpret_cast<unsigned long long*>(p->bmaxk.as<uint64_t>()),
+                                                reinterpret_cast<unsigned long long*>(p->tmaxk.as<uint64_t>()));
+      CLUE_LAUNCH_CHECK("block_maxkey");
       k_nearest_higher_blocks<D, 1><<<grid, kThreads, 0, st>>>(a);
       CLUE_LAUNCH_CHECK("nearest_higher_blocks");
       rc = scan_exclusive(a.pending_flag, a.pending_flag, a.n, p->part.as<int32_t>(), st);
@@ -2237,6 +2322,11 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
         p->bbox_lo.as<float>(), p->bbox_hi.as<float>(), p->ldb, D, p->blk_off.as<int32_t>(), Ttot,
         p->tbox_lo.as<float>(), p->tbox_hi.as<float>());
     CLUE_LAUNCH_CHECK("tile_boxes");
+    // the largest rank keys start at 0 (below every key) for the atomic maxima
+    k_fill_i32<<<div_up(2 * p->ldb, kThreads), kThreads, 0, st>>>(p->bmaxk.as<int32_t>(), 0, 2 * p->ldb);
+    k_fill_i32<<<div_up(2 * static_cast<int64_t>(Ttot), kThreads), kThreads, 0, st>>>(p->tmaxk.as<int32_t>(), 0,
+                                                                                      2 * static_cast<int64_t>(Ttot));
+    CLUE_LAUNCH_CHECK("fill");
   }
 
   // ---- density + nearest-higher
@@ -2278,6 +2368,8 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
     a.blk_off = p->blk_off.as<int32_t>();
     a.hk_out = p->hk.as<uint64_t>();
     a.hk = p->hk.as<uint64_t>();
+    a.bmaxk = p->bmaxk.as<uint64_t>();
+    a.tmaxk = p->tmaxk.as<uint64_t>();
     a.pending = p->pending.as<int32_t>();
     a.pending_flag = p->pending_flag.as<int32_t>();
   }
