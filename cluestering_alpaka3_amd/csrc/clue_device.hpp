@@ -345,5 +345,10 @@ __device__ __forceinline__ int wave_max_i32(int v) {
 }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uniform(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<uint32_t>(v)));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<uint32_t>(v >> 32)));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
 
 }  // namespace clue_hip

@@ -398,6 +398,16 @@ __global__ void k_tile_offsets(const int32_t* skey, int32_t n, int32_t T, int32_
 
 constexpr int kBlock = 32;       // points per block of the Morton layout
 
+// Tiles holding a weight that is infinite or NaN.  The reference adds
+// ((float)(d <= dc) * k) * w_j for every candidate of the search box
+// (ClusteringKernels.hpp:42); the density pass skips the out-of-radius terms,
+// which are +-0 -- except that 0 * w_j is NaN when w_j is infinite or NaN.  The
+// density pass applies those terms in the flagged tiles (flag starts at 0).
+__global__ void k_tile_nonfinite(const float* ws, const int32_t* skey, int32_t n, int32_t* flag) {
+  const int s = blockIdx.x * kThreads + threadIdx.x;
+  if (s < n && !__builtin_isfinite(ws[s])) flag[skey[s]] = 1;
+}
+
 __global__ void k_block_count(const int32_t* offsets, int32_t T, int32_t* cnt) {
   const int t = blockIdx.x * kThreads + threadIdx.x;
   if (t < T) cnt[t] = (offsets[t + 1] - offsets[t] + kBlock - 1) / kBlock;
@@ -610,6 +620,7 @@ struct PassArgs {
   int xcd;                   // bit 0: XCD-aware block order of density, bit 1: of nearest-higher
   int exp_nonpos;            // the kernel's exp argument is always <= 0 and never NaN
   const int32_t* qorder;     // density: query blocks in decreasing estimated cost (or null)
+  const int32_t* tnf;        // density: 1 for tiles holding an infinite or NaN weight
 };
 
 // Workgroups are dispatched to the 8 XCDs round robin (block b to XCD b % 8).
@@ -792,6 +803,7 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
   cptr<float> W = as_const(a.ws);
   cptr<float> HW = as_const(a.hw);
   float rho = 0.f;
+  bool poisoned = false;
   // The Euclidean flat-kernel loop reads the precomputed h*w_j; every other
   // combination reads w_j and evaluates the kernel.
   constexpr bool kUsesHW = (METRIC == MEUCLID) && (KERNEL == KFLAT);
@@ -866,8 +878,24 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
       const bool self = !UNION || __ballot(s >= beg && s < end) != 0;
       if (self) run(std::true_type{}, beg, end, need);
       else run(std::false_type{}, beg, end, need);
+      // out-of-radius candidates with an infinite or NaN weight add 0 * w = NaN
+      if (a.tnf[t] != 0) {
+        for (int j = beg; j < end; ++j) {
+          const float wj = W[j];
+          if (need && !__builtin_isfinite(wj)) {
+            float c[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) c[d] = X[d][j];
+            bool in;
+            if constexpr (METRIC == MEUCLID) in = squared_distance<D>(q, c) <= a.thr;
+            else in = metric_distance<D, METRIC>(q, c, a.mp) <= a.radius;
+            poisoned = poisoned || !in;
+          }
+        }
+      }
     });
   });
+  if (poisoned) rho = __builtin_nanf("");
   if (valid) {
     a.rho_s_out[s] = rho;
     if constexpr (QNH) {
@@ -1723,6 +1751,7 @@ struct clue_plan_s {
   // Morton-ordered layout and the block boxes of the nearest-higher search
   DevBuf kn, kn2, pn, pn2, mid, xn, wn, hk, bcnt, blk_off, bbox_lo, bbox_hi, tbox_lo, tbox_hi;
   DevBuf bmaxk, tmaxk;  // largest rank key of each block / tile
+  DevBuf tnf;           // tiles holding an infinite or NaN weight
   DevBuf pending, pending_flag, qcost, qorder;
   // cluster -> points map (post-processing): sort keys and values
   DevBuf mkey, mkey2, mval, mval2;
@@ -1756,7 +1785,7 @@ struct clue_plan_s {
                    &rho_tmp, &seed_tmp, &bbox_part, &counters, &events, &chunks, &pev,    &sc,
                    &cap_used, &rank_base, &kn,   &kn2,     &pn,      &pn2,      &mid,     &xn,
                    &wn,      &hk,       &bcnt,    &blk_off, &bbox_lo,  &bbox_hi, &tbox_lo,
-                   &tbox_hi, &bmaxk, &tmaxk, &pending, &pending_flag, &qcost, &qorder, &mkey,  &mkey2,
+                   &tbox_hi, &bmaxk, &tmaxk, &tnf, &pending, &pending_flag, &qcost, &qorder, &mkey,  &mkey2,
                    &mval,    &mval2};
     return k < static_cast<int>(sizeof(b) / sizeof(b[0])) ? b[k] : nullptr;
   }
@@ -1798,6 +1827,7 @@ clue_status reserve(clue_plan_s* p, int32_t D, int32_t n, int32_t T, int32_t n_e
   CLUE_HIP(p->perm.ensure(N * 4));
   CLUE_HIP(p->perm2.ensure(N * 4));
   CLUE_HIP(p->offsets.ensure(TT * 4));
+  CLUE_HIP(p->tnf.ensure(TT * 4));
   CLUE_HIP(p->hist.ensure(hist_entries * 4));
   CLUE_HIP(p->part.ensure((div_up(scan_max, kScanTile) + 1) * 4));
   CLUE_HIP(p->xs.ensure(static_cast<size_t>(ld) * D * 4));
@@ -2035,6 +2065,26 @@ clue_status validate(const clue_params* prm, int32_t ndim, int32_t n, const floa
     if (!coords[d]) return fail(CLUE_ERR_INVALID_ARGUMENT, "coordinate pointer is NULL");
   if (kernel && (kernel->type < 0 || kernel->type > 2))
     return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown convolutional kernel type");
+  if (kernel) {
+    // The reference adds ((float)(d <= dc) * k) * w for every candidate of the
+    // search box (ClusteringKernels.hpp:42); the engine skips the out-of-radius
+    // terms, which are +-0 while k is finite for every distance.  Parameters
+    // under which k can be infinite or NaN are refused.
+    const float* kp = kernel->params;
+    bool ok = true;
+    if (kernel->type == CLUE_KERNEL_FLAT) {
+      ok = std::isfinite(kp[0]);
+    } else if (kernel->type == CLUE_KERNEL_GAUSSIAN) {
+      const float den = (2.f * kp[1]) * kp[1];
+      ok = std::isfinite(kp[0]) && std::isfinite(kp[2]) && std::isfinite(den) && den > 0.f;
+    } else {
+      ok = std::isfinite(kp[0]) && kp[0] > 0.f && std::isfinite(kp[1]);
+    }
+    if (!ok)
+      return fail(CLUE_ERR_INVALID_ARGUMENT,
+                  "kernel parameters must be finite, with (2*std)*std finite and positive (gaussian) "
+                  "and a positive rate (exponential)");
+  }
   if (metric && (metric->type < 0 || metric->type > 5))
     return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown distance metric type");
   if (prm->seed_mode != CLUE_SEEDS_REFERENCE && prm->seed_mode != CLUE_SEEDS_UNBOUNDED)
@@ -2298,6 +2348,10 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
                                                      p->hw.as<float>(), h, blocks ? p->kn.as<int32_t>() : nullptr,
                                                      p->kn2.as<int32_t>());
   CLUE_LAUNCH_CHECK("gather");
+  k_fill_i32<<<div_up(Ttot, kThreads), kThreads, 0, st>>>(p->tnf.as<int32_t>(), 0, Ttot);
+  k_tile_nonfinite<<<div_up(n, kThreads), kThreads, 0, st>>>(p->ws.as<float>(), sorted_keys, n,
+                                                              p->tnf.as<int32_t>());
+  CLUE_LAUNCH_CHECK("tile_nonfinite");
   int32_t* mkeys = nullptr;
   int32_t* slot_of = nullptr;  // Morton position -> tile-order slot
   if (blocks) {
@@ -2374,6 +2428,7 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
     a.pending_flag = p->pending_flag.as<int32_t>();
   }
   a.xcd = p->tuning.xcd_remap;
+  a.tnf = p->tnf.as<int32_t>();
   a.rho_s_out = p->rho_s.as<float>();
   a.rho_out = rho_buf;
   a.nh_out = nh_buf;

@@ -273,6 +273,45 @@ def test_edge_cases():
     assert_same(*run_both(x, np.ones(3000, np.float32), 1.0, 2.0, 2.0, ppb=100000))
 
 
+def assert_same_special(got, ref, tiles, grid, stats, delta):
+    # densities may be NaN here: equal as values (NaN matches NaN), not as bits,
+    # since the payload of a NaN made by inf - inf differs between host and GPU
+    np.testing.assert_array_equal(tiles[1], ref.tile_indexes)
+    np.testing.assert_array_equal(got.rho, ref.rho)
+    np.testing.assert_array_equal(delta, ref.delta)
+    np.testing.assert_array_equal(got.nearest_higher, ref.nh)
+    np.testing.assert_array_equal(got.is_seed, ref.is_seed)
+    np.testing.assert_array_equal(got.cluster, ref.cluster)
+    assert stats["seed_candidates"] == ref.stats["seed_candidates"]
+
+
+@pytest.mark.parametrize("case", ["signed", "zero", "negzero", "nan", "inf", "signed_gauss"])
+def test_nonpositive_and_special_densities(case):
+    # the "higher" relation at rho <= 0 (no tie-break), at rho == +inf (tie-break
+    # by id) and with NaN densities (never higher, nothing higher than them)
+    rng = np.random.default_rng(11)
+    n = 3000
+    x = np.round(rng.random((2, n)) * 30).astype(np.float32) * np.float32(0.5)
+    kernel = ("flat", 0.5)
+    if case == "signed" or case == "signed_gauss":
+        w = rng.integers(-1, 2, n).astype(np.float32)
+        kernel = ("gaussian", 0.0, 1.0, 1.0) if case == "signed_gauss" else kernel
+    elif case == "zero":
+        w = np.zeros(n, np.float32)
+    elif case == "negzero":
+        w = np.where(rng.random(n) < 0.5, np.float32(-0.0), np.float32(0.0)).astype(np.float32)
+        w[::97] = 1.0
+    elif case == "nan":
+        w = np.ones(n, np.float32)
+        w[rng.choice(n, 20, replace=False)] = np.nan
+    else:
+        w = np.ones(n, np.float32)
+        w[rng.choice(n, 20, replace=False)] = np.inf
+        w[rng.choice(n, 5, replace=False)] = -np.inf
+    assert_same_special(*run_both(x, w, 1.0, 1.0, 2.0, kernel=kernel))
+    assert_same_special(*run_both(x, w, 1.0, 1.0, 2.0, kernel=kernel, tuning=dict(nh_first_radius=1.0)))
+
+
 def test_invalid_arguments():
     with pytest.raises(clue.InvalidArgument):
         clue.cluster_points(np.zeros((2, 4), np.float32), np.ones(4, np.float32), -1.0, 1.0)
