@@ -296,6 +296,12 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const int32_t* keys_
                                                              int32_t* keys_out, int32_t* vals_out) {
   __shared__ int32_t wcnt[kThreads / kWave][256];
   __shared__ int32_t gbase[256];
+  __shared__ int32_t lbase[256];
+  __shared__ int32_t scan_tmp[kThreads / kWave];
+  // the block's keys and values in digit order (a local stable sort), written
+  // out in runs of consecutive addresses per digit
+  __shared__ int32_t skey[kSortTile];
+  __shared__ int32_t sval[kSortTile];
   const int radix = 1 << bits;
   const int lane = __lane_id();
   const int w = threadIdx.x / kWave;
@@ -306,7 +312,8 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const int32_t* keys_
   }
   __syncthreads();
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
-  const int64_t base = static_cast<int64_t>(blockIdx.x) * kSortTile + w * (kSortItems * kWave);
+  const int64_t blk0 = static_cast<int64_t>(blockIdx.x) * kSortTile;
+  const int64_t base = blk0 + w * (kSortItems * kWave);
   int32_t kk[kSortItems], vv[kSortItems], loc[kSortItems];
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
@@ -330,23 +337,38 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const int32_t* keys_
     loc[r] = valid ? before + __popcll(lower) : -1;
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < radix; t += kThreads) {
-    int s = 0;
+  // per digit: offsets of the waves inside the block, and the digit's start in
+  // the block's digit order (an exclusive scan over the digits)
+  {
+    const int t = threadIdx.x;  // kThreads == 256 digits
+    int sum = 0;
 #pragma unroll
     for (int ww = 0; ww < kThreads / kWave; ++ww) {
-      const int cnt = wcnt[ww][t];
-      wcnt[ww][t] = s;
-      s += cnt;
+      const int cnt = t < radix ? wcnt[ww][t] : 0;
+      if (t < radix) wcnt[ww][t] = sum;
+      sum += cnt;
     }
+    int total;
+    const int incl = block_incl_scan<kThreads>(sum, scan_tmp, total);
+    lbase[t] = incl - sum;
   }
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
     if (loc[r] < 0) continue;
     const int digit = (kk[r] >> shift) & (radix - 1);
-    const int pos = gbase[digit] + wcnt[w][digit] + loc[r];
-    if (keys_out) keys_out[pos] = kk[r];
-    vals_out[pos] = vv[r];
+    const int li = lbase[digit] + wcnt[w][digit] + loc[r];
+    skey[li] = kk[r];
+    sval[li] = vv[r];
+  }
+  __syncthreads();
+  const int cnt = static_cast<int>(min(static_cast<int64_t>(kSortTile), static_cast<int64_t>(n) - blk0));
+  for (int li = threadIdx.x; li < cnt; li += kThreads) {
+    const int32_t key = skey[li];
+    const int digit = (key >> shift) & (radix - 1);
+    const int pos = gbase[digit] + (li - lbase[digit]);
+    if (keys_out) keys_out[pos] = key;
+    vals_out[pos] = sval[li];
   }
 }
 
@@ -3043,7 +3065,7 @@ clue_status clue_autotune_batched(clue_plan_t plan, void* stream, clue_params* p
   if (rc == CLUE_OK && (flags & CLUE_TUNE_POINTS_PER_TILE)) {
     const int32_t keep = prm.points_per_tile;
     int32_t best_ppb = keep;
-    for (int32_t ppb : {32, 64, 128, 256, 512}) {
+    for (int32_t ppb : {16, 32, 64, 128, 256, 512}) {
       if (ppb == keep || rc != CLUE_OK) continue;
       prm.points_per_tile = ppb;
       float ms;
@@ -3074,7 +3096,7 @@ clue_status clue_autotune_batched(clue_plan_t plan, void* stream, clue_params* p
   auto set_radius = [](clue_tuning& t, float v) { t.nh_first_radius = v; };
   auto set_xcd = [](clue_tuning& t, float v) { t.xcd_remap = static_cast<int32_t>(v); };
   auto set_group = [](clue_tuning& t, float v) { t.pending_group = static_cast<int32_t>(v); };
-  try_field(set_radius, {0.08f, 0.12f, 0.18f, 0.25f, 0.35f, 0.5f});
+  try_field(set_radius, {0.08f, 0.12f, 0.18f, 0.25f, 0.35f, 0.5f, 0.75f, 1.0f});
   try_field(set_xcd, {0.f, 1.f, 2.f, 3.f});
   try_field(set_group, {8.f, 16.f, 32.f, 64.f});
   try_field([](clue_tuning& t, float v) { t.density_order = static_cast<int32_t>(v); }, {0.f, 1.f});
