@@ -3,6 +3,12 @@
     python -m cluestering_alpaka3_amd.profiling stats <kernel_stats.csv>
         per-kernel call count and average duration (a --kernel-trace --stats run)
 
+    python -m cluestering_alpaka3_amd.profiling run-batched [--events 4096] [--steps 5] [--ppb 64]
+                                                            [--nh-first-radius 1.0]
+        the batched calorimeter workload (the events of bench.py's batched
+        measurement) clustered --steps times in one batched call each, with the
+        given points per tile and tuning: the program to run under rocprofv3
+
     python -m cluestering_alpaka3_amd.profiling traffic <fetch.csv> <write.csv> [--copy-mib 64] [--json OUT]
         HBM bytes per dispatch from two --pmc runs (FETCH_SIZE, then WRITE_SIZE),
         each scaled by what the same counter reads for the device-to-device copy
@@ -80,9 +86,44 @@ def traffic(fetch_csv, write_csv, copy_mib=64):
     return out
 
 
+def run_batched(events, steps, ppb, nh_first_radius):
+    import numpy as np
+
+    import cluestering_alpaka3_amd as clue
+    from cluestering_alpaka3_amd import datasets
+
+    sizes = datasets.event_sizes(events, 2024)
+    evs = [datasets.calorimeter_event(int(n), 7_000_000 + k) for k, n in enumerate(sizes)]
+    coords = np.ascontiguousarray(np.concatenate([e.coords for e in evs], axis=1))
+    weights = np.ascontiguousarray(np.concatenate([e.weights for e in evs]))
+    off = np.zeros(sizes.size + 1, np.int64)
+    off[1:] = np.cumsum(sizes)
+    n = coords.shape[1]
+    stream = clue.Stream(0)
+    plan = clue.Plan(0)
+    dcoords = clue.DeviceArray.from_host(coords)
+    dw = clue.DeviceArray.from_host(weights)
+    cp = [dcoords.offset(d * n) for d in range(2)]
+    outs = [clue.DeviceArray((n,), t) for t in (np.int32, np.int32, np.float32, np.int32)]
+    params = clue.make_params(1.3, 2.0, 2.6, ppb=ppb)
+    tuning = plan.get_tuning()
+    tuning["nh_first_radius"] = nh_first_radius
+    plan.set_tuning(**tuning)
+    kd, md = clue.kernel_desc(("flat", 0.5)), clue.metric_desc("euclidean")
+    for step in range(steps):
+        plan.make_clusters_batched_device(stream, params, 2, off, cp, dw.ptr, kd, md, None, *[o.ptr for o in outs])
+        stream.synchronize()
+        print(f"step {step + 1}/{steps}: {events} events, {n} points", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
+    b = sub.add_parser("run-batched")
+    b.add_argument("--events", type=int, default=4096)
+    b.add_argument("--steps", type=int, default=5)
+    b.add_argument("--ppb", type=int, default=64)
+    b.add_argument("--nh-first-radius", type=float, default=1.0)
     s = sub.add_parser("stats")
     s.add_argument("csv")
     s.add_argument("--top", type=int, default=25)
@@ -92,7 +133,9 @@ def main():
     t.add_argument("--copy-mib", type=int, default=64)
     t.add_argument("--json")
     a = ap.parse_args()
-    if a.cmd == "stats":
+    if a.cmd == "run-batched":
+        run_batched(a.events, a.steps, a.ppb, a.nh_first_radius)
+    elif a.cmd == "stats":
         print(f"{'kernel':100s} {'calls':>6s} {'avg_us':>10s} {'share%':>7s}")
         for name, calls, avg, pct in kernel_stats(a.csv)[: a.top]:
             print(f"{name[:100]:100s} {calls:6d} {avg / 1e3:10.2f} {pct:7.2f}")
