@@ -515,7 +515,11 @@ class Server:
         self._outstanding = {}  # ticket -> host arrays the server still reads or writes
 
     def submit(self, coords, weights, dc, rhoc, dm=None, seed_dc=None, ppb=128, kernel=("flat", 0.5),
-               metric="euclidean", metric_params=None, wrapped=None, seed_mode=SEEDS_REFERENCE, outputs="all"):
+               metric="euclidean", metric_params=None, wrapped=None, seed_mode=SEEDS_REFERENCE, outputs="all",
+               cluster_out=None):
+        """Queue one clustering request; returns a Ticket.  outputs="labels" returns the
+        labels only; cluster_out (int32, n elements, C-contiguous) receives them in place
+        of a new array."""
         c = np.ascontiguousarray(coords, dtype=np.float32)
         if c.ndim == 1:
             c = c[None, :]
@@ -529,7 +533,13 @@ class Server:
         r.metric = metric_desc(metric, metric_params)
         for d, v in enumerate(wrapped or []):
             r.wrapped[d] = int(v)
-        cl = np.empty(n, np.int32)
+        if cluster_out is not None:
+            if (not isinstance(cluster_out, np.ndarray) or cluster_out.dtype != np.int32 or cluster_out.size != n
+                    or not cluster_out.flags.c_contiguous or not cluster_out.flags.writeable):
+                raise InvalidArgument("cluster_out must be a writeable C-contiguous int32 array of n elements")
+            cl = cluster_out.reshape(-1)
+        else:
+            cl = np.empty(n, np.int32)
         full = outputs == "all"
         seed = np.empty(n, np.int32) if full else None
         rho = np.empty(n, np.float32) if full else None

@@ -2539,6 +2539,40 @@ struct BatchGraph {
   hipGraphExec_t exec = nullptr;
 };
 
+// Scratch for batched calls of up to npoints points in up to max_events events
+// of up to ndim dimensions at 128 points per tile or more, so that such calls
+// allocate nothing (the served engine's slots).  An event of nt tiles has at
+// most 2^D * nt tiles after the reference's rounding to npd^D; the bound is taken
+// with D at most 3 (larger grids of tiny high-dimensional events still grow the
+// buffers on their first call).  The pinned staging ring is sized for the tables.
+clue_status reserve_batches(clue_plan_t p, int32_t ndim, int32_t npoints, int32_t max_events) {
+  if (!p || ndim < 1 || ndim > CLUE_MAX_DIM || npoints <= 0 || max_events <= 0)
+    return fail(CLUE_ERR_INVALID_ARGUMENT, "invalid reserve size");
+  const int64_t e = std::min<int64_t>(max_events, npoints);
+  const int64_t tiles = (int64_t{1} << std::min(ndim, 3)) * (div_up(npoints, 128) + e);
+  const int64_t chunks = div_up(npoints, kChunk) + e;
+  if (tiles > INT32_MAX / 2) return fail(CLUE_ERR_INVALID_ARGUMENT, "reserve size too large");
+  CLUE_HIP(hipSetDevice(p->device));
+  clue_status rc = reserve(p, ndim, npoints, static_cast<int32_t>(tiles), static_cast<int32_t>(e),
+                           static_cast<int32_t>(chunks));
+  if (rc != CLUE_OK) return rc;
+  BatchCall bc;
+  bc.n_events = static_cast<int32_t>(e);
+  bc.n_chunks = static_cast<int32_t>(chunks);
+  const size_t tb = table_bytes(bc);
+  for (auto& slot : p->stage) {
+    if (slot.done) CLUE_HIP(hipEventSynchronize(slot.done));
+    if (slot.bytes < tb) {
+      if (slot.host) CLUE_HIP(hipHostFree(slot.host));
+      slot.host = nullptr;
+      slot.bytes = 0;
+      CLUE_HIP(hipHostMalloc(&slot.host, tb, hipHostMallocDefault));
+      slot.bytes = tb;
+    }
+  }
+  return CLUE_OK;
+}
+
 }  // namespace clue_hip
 
 // ===========================================================================

@@ -34,6 +34,8 @@
 
 namespace clue_hip {
 void set_error(const std::string& msg);  // the engine's clue_last_error_message
+// scratch of a plan for batched calls of up to npoints points in max_events events
+clue_status reserve_batches(clue_plan_t plan, int32_t ndim, int32_t npoints, int32_t max_events);
 }
 
 namespace {
@@ -279,7 +281,7 @@ clue_status clue_server_create(int32_t device, const clue_server_config* config,
   clue_server_config cfg{};
   if (config) cfg = *config;
   if (cfg.device_budget_bytes == 0) cfg.device_budget_bytes = size_t{32} << 30;
-  if (cfg.slots <= 0) cfg.slots = 3;
+  if (cfg.slots <= 0) cfg.slots = 4;
   if (cfg.max_batch_points <= 0) cfg.max_batch_points = 8 << 20;
   if (cfg.max_batch_events <= 0) cfg.max_batch_events = 4096;
   if (cfg.max_dim <= 0) cfg.max_dim = 3;
@@ -294,7 +296,7 @@ clue_status clue_server_create(int32_t device, const clue_server_config* config,
   clue_plan_t probe = nullptr;
   if (clue_plan_create(device, &probe) != CLUE_OK) return serr(CLUE_ERR_RUNTIME, clue_last_error_message());
   while (true) {
-    if (clue_plan_reserve(probe, cfg.max_dim, static_cast<int32_t>(cap)) != CLUE_OK) {
+    if (clue_hip::reserve_batches(probe, cfg.max_dim, static_cast<int32_t>(cap), cfg.max_batch_events) != CLUE_OK) {
       clue_plan_destroy(probe);
       return serr(CLUE_ERR_OUT_OF_MEMORY, clue_last_error_message());
     }
@@ -319,8 +321,14 @@ clue_status clue_server_create(int32_t device, const clue_server_config* config,
               hipHostMalloc(&sl->h_in, static_cast<size_t>(cap) * 4 * (cfg.max_dim + 1), hipHostMallocDefault) ==
                   hipSuccess &&
               hipHostMalloc(&sl->h_out, static_cast<size_t>(cap) * 16, hipHostMallocDefault) == hipSuccess;
+    if (ok) {
+      // touch the pinned staging once here, so that no request pays its first-touch faults
+      std::memset(sl->h_in, 0, static_cast<size_t>(cap) * 4 * (cfg.max_dim + 1));
+      std::memset(sl->h_out, 0, static_cast<size_t>(cap) * 16);
+    }
     ok = ok && clue_plan_create(device, &sl->plan) == CLUE_OK &&
-         clue_plan_reserve(sl->plan, cfg.max_dim, static_cast<int32_t>(cap)) == CLUE_OK;
+         clue_hip::reserve_batches(sl->plan, cfg.max_dim, static_cast<int32_t>(cap), cfg.max_batch_events) ==
+             CLUE_OK;
     if (ok) s->device_bytes += slot_bytes(sl->plan, cap, cfg.max_dim);
     s->slots.push_back(std::move(sl));
     if (!ok) {

@@ -240,7 +240,7 @@ typedef struct clue_server_s* clue_server_t;
 
 typedef struct {
   size_t device_budget_bytes; /* 0: 32 GiB */
-  int32_t slots;              /* batches in flight, 0: 3 */
+  int32_t slots;              /* batches in flight, 0: 4 */
   int32_t max_batch_points;   /* per slot before the budget cap, 0: 8 Mi */
   int32_t max_batch_events;   /* 0: 4096 */
   int32_t max_dim;            /* largest dimension served, 0: 3 */
