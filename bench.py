@@ -314,7 +314,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU per process; multi-GPU figures are not "
-                                   "produced (events shard across GPUs without communication, see DESIGN.md)",
+                                   "produced (batched and served events shard across GPUs without communication)",
                           "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
 
