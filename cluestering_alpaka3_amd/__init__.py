@@ -536,7 +536,7 @@ class Server:
         if cluster_out is not None:
             if (not isinstance(cluster_out, np.ndarray) or cluster_out.dtype != np.int32 or cluster_out.size != n
                     or not cluster_out.flags.c_contiguous or not cluster_out.flags.writeable):
-                raise InvalidArgument("cluster_out must be a writeable C-contiguous int32 array of n elements")
+                raise InvalidArgument(1, "cluster_out must be a writeable C-contiguous int32 array of n elements")
             cl = cluster_out.reshape(-1)
         else:
             cl = np.empty(n, np.int32)

@@ -78,6 +78,18 @@ __device__ __forceinline__ int block_incl_scan(int v, int* smem, int& total) {
   return incl;
 }
 
+// Workgroups are dispatched to the 8 XCDs round robin (block b to XCD b % 8).
+// With xcd set, XCD x takes the contiguous range x of the logical blocks, so
+// neighbouring query blocks -- which read the same tiles -- share one L2.
+__device__ __forceinline__ int logical_block(int xcd) {
+  const int b = blockIdx.x;
+  if (!xcd) return b;
+  const int nb = gridDim.x;
+  const int q = nb >> 3, r = nb & 7;
+  const int x = b & 7, i = b >> 3;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+}
+
 __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -377,7 +389,7 @@ __global__ __launch_bounds__(kThreads) void k_gather(Coords c, const float* w, c
                                                       int32_t n, int D, int64_t ld, float* xs,
                                                       float* ws, float* hw, float h,
                                                       const int32_t* key_nh, int32_t* key_nh_s) {
-  const int s = blockIdx.x * kThreads + threadIdx.x;
+  const int s = logical_block(1) * kThreads + threadIdx.x;
   if (s >= n) return;
   const int32_t id = perm[s];
   for (int d = 0; d < D; ++d) xs[d * ld + s] = c.p[d][id];
@@ -396,7 +408,7 @@ __global__ __launch_bounds__(kThreads) void k_gather_morton(const float* xs, con
                                                              const int32_t* perm, const int32_t* slot_of,
                                                              int32_t n, int D, int64_t ld, float* xn,
                                                              float* wn, int32_t* mid) {
-  const int p = blockIdx.x * kThreads + threadIdx.x;
+  const int p = logical_block(1) * kThreads + threadIdx.x;
   if (p >= n) return;
   const int32_t s = slot_of[p];
   for (int d = 0; d < D; ++d) xn[d * ld + p] = xs[d * ld + s];
@@ -441,7 +453,7 @@ __global__ __launch_bounds__(kThreads) void k_block_boxes(const float* xn, int64
                                                            const int32_t* blk_off, int32_t T,
                                                            float* bmin, float* bmax,
                                                            int64_t ldb) {
-  const int b = blockIdx.x * kThreads + threadIdx.x;
+  const int b = logical_block(1) * kThreads + threadIdx.x;
   const int nblk = blk_off[T];
   if (b >= nblk) return;
   int lo = 0, hi = T;  // last t with blk_off[t] <= b
@@ -470,7 +482,7 @@ __global__ __launch_bounds__(kThreads) void k_tile_boxes(const float* bmin, cons
                                                           const int32_t* blk_off, int32_t T,
                                                           float* tmin, float* tmax) {
   // one wave per tile: the lanes split its blocks, a wave reduction combines
-  const int t = static_cast<int>((static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x) / kWave);
+  const int t = static_cast<int>((static_cast<int64_t>(logical_block(1)) * kThreads + threadIdx.x) / kWave);
   if (t >= T) return;  // wave-uniform
   const int lane = __lane_id();
   const int b0 = blk_off[t], b1 = blk_off[t + 1];
@@ -503,7 +515,7 @@ __global__ __launch_bounds__(kThreads) void k_block_maxkey(const uint64_t* hk, c
                                                             const int32_t* offsets, const int32_t* blk_off,
                                                             int32_t n, unsigned long long* bmaxk,
                                                             unsigned long long* tmaxk) {
-  const int p = blockIdx.x * kThreads + threadIdx.x;
+  const int p = logical_block(1) * kThreads + threadIdx.x;
   const int lane = __lane_id();
   const bool valid = p < n;
   int t = -1, b = -1;
@@ -645,17 +657,6 @@ struct PassArgs {
   const int32_t* tnf;        // density: 1 for tiles holding an infinite or NaN weight
 };
 
-// Workgroups are dispatched to the 8 XCDs round robin (block b to XCD b % 8).
-// With xcd set, XCD x takes the contiguous range x of the logical blocks, so
-// neighbouring query blocks -- which read the same tiles -- share one L2.
-__device__ __forceinline__ int logical_block(int xcd) {
-  const int b = blockIdx.x;
-  if (!xcd) return b;
-  const int nb = gridDim.x;
-  const int q = nb >> 3, r = nb & 7;
-  const int x = b & 7, i = b >> 3;
-  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
-}
 
 // Search box of a point: [bin(x - r), bin(x + r)] per dimension, wrapped
 // dimensions unrolled past npd (TilesView::searchBox, TilesView.hpp:72-82).
@@ -1052,24 +1053,17 @@ __device__ __forceinline__ float upper_bound2(const float* q, const float* lo, c
 // grid without wrapping -- so candidates may be examined in any order, and a
 // block whose bounding box cannot beat a lane's current distance is skipped.
 //
-// PHASE 1 limits the search to the radius thr1 (the wave's own 64 points first,
-// then the blocks of its lanes' search boxes); a lane that finds a higher point
-// within it is final (every candidate at or below its distance was examined).
-// The others are queued for PHASE 2, the full search within dm.
-template <int D, int PHASE>
+// This first phase (one lane per query, a wave per 64 Morton-consecutive
+// queries) limits the search to the radius thr1: the wave's own 64 points
+// first, then the blocks of its lanes' search boxes.  A lane that finds a
+// higher point within it is final (every candidate at or below its distance was
+// examined); the others are queued for the second phase (k_nh_pending), the
+// full search within dm.
+template <int D>
 __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassArgs a) {
   const int gid = logical_block((a.xcd >> 1) & 1) * kThreads + threadIdx.x;
-  int pos;
-  bool valid;
-  if constexpr (PHASE == 1) {
-    valid = gid < a.n;
-    pos = valid ? gid : a.n - 1;
-  } else {
-    const int npend = uniform(*a.pending_count);
-    if (uniform(gid & ~(kWave - 1)) >= npend) return;  // wave-uniform exit
-    valid = gid < npend;
-    pos = a.pending[valid ? gid : npend - 1];
-  }
+  const bool valid = gid < a.n;
+  const int pos = valid ? gid : a.n - 1;
   float q[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) q[d] = a.xn[d][pos];
@@ -1100,7 +1094,7 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
   // sqrtf equal to delta are [lo2, hi2].  A candidate replaces it when it is
   // nearer (s2 < lo2) or as near and earlier in the visit order (key < best), so
   // candidates need s2 <= bound = min(search radius, hi2).
-  const float cap = (PHASE == 1 && a.thr1 < a.thr) ? a.thr1 : a.thr;
+  const float cap = a.thr1 < a.thr ? a.thr1 : a.thr;
   float bound = cap;
   float lo2 = __builtin_inff();
   // the wave's query box, for the block prefilter
@@ -1150,20 +1144,16 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
 
   for_each_event(valid, my_e, [&](int e, bool act) {
     const EvGrid<D> g = load_event<D>(a.ev, e, a.wrap);
+    // only candidates within thr1: a box of a slightly larger radius (1/1024 of
+    // it, plus 2^-20 of the coordinate for the rounding of q -+ r) holds every
+    // such candidate and lies inside the reference's box of dm
     int lo[D], hi[D];
-    if constexpr (PHASE == 1) {
-      // phase 1 only looks within thr1: a box of a slightly larger radius (1/1024
-      // of it, plus 2^-20 of the coordinate for the rounding of q -+ r) holds
-      // every such candidate and lies inside the reference's box of dm
 #pragma unroll
-      for (int d = 0; d < D; ++d) {
-        float r = a.box1 + __builtin_fabsf(q[d]) * 0x1p-20f;
-        r = r < a.radius ? r : a.radius;
-        lo[d] = tile_bin(q[d] - r, g.mn[d], g.mx[d], g.ts[d], g.npd, false);
-        hi[d] = tile_bin(q[d] + r, g.mn[d], g.mx[d], g.ts[d], g.npd, false);
-      }
-    } else {
-      search_box<D>(q, a.radius, g, lo, hi);
+    for (int d = 0; d < D; ++d) {
+      float r = a.box1 + __builtin_fabsf(q[d]) * 0x1p-20f;
+      r = r < a.radius ? r : a.radius;
+      lo[d] = tile_bin(q[d] - r, g.mn[d], g.mx[d], g.ts[d], g.npd, false);
+      hi[d] = tile_bin(q[d] + r, g.mn[d], g.mx[d], g.ts[d], g.npd, false);
     }
     auto in_box = [&](int t) {
       const int tl = t - g.tile_base;
@@ -1176,8 +1166,8 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
       return in;
     };
 
-    if constexpr (PHASE == 1) {
-      // the wave's own points: same tile, neighbouring Morton codes
+    // the wave's own points: same tile, neighbouring Morton codes
+    {
       const int base = uniform(gid & ~(kWave - 1));
       if (base < a.n) {
         const int t0 = uniform(as_const(a.kn)[base] >> a.mshift);
@@ -1268,20 +1258,12 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
     }
   });
 
-  if constexpr (PHASE == 1) {
-    // unresolved positions are compacted in Morton order (flag -> scan -> scatter)
-    // so that the full search runs on spatially coherent waves
-    if (valid) a.pending_flag[pos] = nh == -1 ? 1 : 0;
-    if (valid && nh != -1) {
-      a.nh_out[id_i] = nh;
-      a.delta_out[id_i] = delta;
-    }
-  } else {
-    if (valid) {
-      a.nh_out[id_i] = nh;
-      a.delta_out[id_i] = delta;
-      if (nh == -1) atomicAdd(&a.sc[my_e], 1);
-    }
+  // unresolved positions are compacted in Morton order (flag -> scan -> scatter)
+  // so that the full search runs on spatially coherent waves
+  if (valid) a.pending_flag[pos] = nh == -1 ? 1 : 0;
+  if (valid && nh != -1) {
+    a.nh_out[id_i] = nh;
+    a.delta_out[id_i] = delta;
   }
 }
 
@@ -1990,7 +1972,7 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
                                                 reinterpret_cast<unsigned long long*>(p->bmaxk.as<uint64_t>()),
                                                 reinterpret_cast<unsigned long long*>(p->tmaxk.as<uint64_t>()));
       CLUE_LAUNCH_CHECK("block_maxkey");
-      k_nearest_higher_blocks<D, 1><<<grid, kThreads, 0, st>>>(a);
+      k_nearest_higher_blocks<D><<<grid, kThreads, 0, st>>>(a);
       CLUE_LAUNCH_CHECK("nearest_higher_blocks");
       rc = scan_exclusive(a.pending_flag, a.pending_flag, a.n, p->part.as<int32_t>(), st);
       if (rc != CLUE_OK) return rc;

@@ -47,14 +47,19 @@ def test_requests_arriving_while_busy_join_later_batches():
     srv = clue.Server(0, slots=1)
     try:
         tickets = []
-        for ps, p in events:
+        outs = [np.full(ps.n, -7, np.int32) if k % 2 == 0 else None for k, (ps, _) in enumerate(events)]
+        for (ps, p), out in zip(events, outs):
             tickets.append(srv.submit(ps.coords, ps.weights, p["dc"], p["rhoc"], p["dm"], kernel=p["kernel"],
-                                      outputs="labels"))
+                                      outputs="labels", cluster_out=out))
         res = [t.result() for t in tickets]
+        with pytest.raises(clue.InvalidArgument):
+            srv.submit(events[0][0].coords, events[0][0].weights, 1.0, 1.0, cluster_out=np.empty(3, np.int32))
     finally:
         srv.close()
-    for (ps, p), got in zip(events, res):
+    for (ps, p), got, out in zip(events, res, outs):
         np.testing.assert_array_equal(got.cluster, alone(ps, p).cluster)
+        if out is not None:  # the caller's array received the labels
+            np.testing.assert_array_equal(out, got.cluster)
 
 
 def test_budget_caps_slot_capacity():
