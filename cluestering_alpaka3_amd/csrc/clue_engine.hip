@@ -866,7 +866,7 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
   auto run = [&](auto self_c, int beg, int end, bool need) {
     int j = beg;
     if constexpr (UNION) {
-      constexpr int U = D <= 2 ? 16 : 4;  // candidates loaded ahead of their use
+      constexpr int U = D <= 3 ? 16 : 8;  // candidates loaded ahead of their use
       for (; j + U <= end; j += U) {
         float c[U][D], wv4[U], hv[U];
 #pragma unroll
