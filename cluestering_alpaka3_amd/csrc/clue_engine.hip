@@ -1571,32 +1571,43 @@ __global__ __launch_bounds__(kThreads) void k_jump(int32_t* root, int32_t n) {
 }
 
 // Pointer jumping: every hop follows the current parent array, which other
-// threads only ever shorten; `hops` per round bounds the work of a round.
-// The final round writes the label -- the position of the root in its event's
-// serial seed array, -1 if the root is not a stored seed (KernelAssignClusters,
-// ClusteringKernels.hpp:252-279) -- and turns the nearest-higher into an index
-// local to the event.
+// threads only ever shorten; `hops` per round bounds the work of a round.  A
+// thread that reaches its root writes the label -- the position of the root in
+// its event's serial seed array, -1 if the root is not a stored seed
+// (KernelAssignClusters, ClusteringKernels.hpp:252-279); one that runs out of
+// hops raises the round's flag, and the next round (which returns at once when
+// the previous flag is clear) goes on from where it stopped.  The first round
+// also turns the nearest-higher into an index local to the event.
 __global__ __launch_bounds__(kThreads) void k_labels(int32_t* root, int32_t n, int hops,
                                                       const int32_t* is_seed, const int32_t* rank,
                                                       const int32_t* pev, const EventDesc* ev,
                                                       const int32_t* cap_used,
                                                       const int32_t* rank_base, int32_t* cluster,
-                                                      int32_t* nh_out, int final_round) {
+                                                      int32_t* nh_out, int32_t* flags, int round) {
+  if (round > 0 && flags[round - 1] == 0) return;  // every thread finished before
   const int i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
+  const int e = pev[i];
+  if (round == 0) {
+    const int32_t nhv = nh_out[i];
+    if (nhv >= 0) nh_out[i] = nhv - ev[e].begin;
+  }
   int32_t r = root[i];
+  bool at_root = false;
   for (int h = 0; h < hops; ++h) {
     const int32_t rr = root[r];
-    if (rr == r) break;
+    if (rr == r) {
+      at_root = true;
+      break;
+    }
     r = rr;
   }
   root[i] = r;
-  if (final_round) {
-    const int e = pev[i];
+  if (at_root) {
     const int32_t local = rank[r] - rank_base[e];
     cluster[i] = (is_seed[r] && local < cap_used[e]) ? local : -1;
-    const int32_t nhv = nh_out[i];
-    if (nhv >= 0) nh_out[i] = nhv - ev[e].begin;
+  } else {
+    flags[round] = 1;
   }
 }
 
@@ -2466,7 +2477,7 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
   // totals of this call; [1], the grow-only capacity, carries over unless reset
   k_fill_i32<<<1, 64, 0, st>>>(counters, 0, p->reset_capacity && persistent_capacity ? 2 : 1);
   if (persistent_capacity) p->reset_capacity = false;
-  k_fill_i32<<<1, 64, 0, st>>>(counters + 2, 0, 6);
+  k_fill_i32<<<1, 64, 0, st>>>(counters + 2, 0, 9);  // [2, 8) totals, [8, 11) label-round flags
   CLUE_LAUNCH_CHECK("fill");
   k_capacity<<<div_up(n_events, kThreads), kThreads, 0, st>>>(
       p->events.as<EventDesc>(), n_events, p->sc.as<int32_t>(), p->flag.as<int32_t>(), counters,
@@ -2487,7 +2498,7 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
     k_labels<<<div_up(n, kThreads), kThreads, 0, st>>>(
         p->root.as<int32_t>(), n, hops, seed_buf, p->flag.as<int32_t>(), p->pev.as<int32_t>(),
         p->events.as<EventDesc>(), p->cap_used.as<int32_t>(), p->rank_base.as<int32_t>(), cluster_out,
-        nh_buf, r == rounds - 1 ? 1 : 0);
+        nh_buf, counters + 8, r);
     CLUE_LAUNCH_CHECK("labels");
   }
 
