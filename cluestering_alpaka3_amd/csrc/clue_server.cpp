@@ -11,7 +11,8 @@
 // A scheduler thread forms a batch whenever a slot is free: the oldest pending
 // request fixes the batch's class (dimension, parameters, kernel, metric,
 // wrapping) and every pending request of that class is taken in arrival order
-// while the slot has room.  Requests that arrive while slots are busy wait in
+// while the batch holds at most half the slot's capacity (a single request may
+// fill the slot).  Requests that arrive while slots are busy wait in
 // the queue and join the next batch (continuous batching); the slots' streams
 // overlap one batch's copies with another's kernels.  A waiter thread per slot
 // blocks on the slot's completion event, scatters the outputs to the callers'
@@ -201,12 +202,16 @@ void scheduler_loop(clue_server_s* s) {
       return (slot && !s->pending.empty()) || (s->stop && s->pending.empty());
     });
     if (s->pending.empty()) break;  // stopping, nothing left to launch
-    // batch: the oldest request's class, in arrival order, while the slot has room
+    // batch: the oldest request's class, in arrival order, up to half a slot's
+    // capacity (a single request may fill the slot): under load, several
+    // batches then share the GPU, and one batch's copies overlap another's kernels
     const clue_request first = s->pending.front()->req;
+    const int64_t target = s->cap / 2;
     int64_t pts = 0;
     for (auto it = s->pending.begin(); it != s->pending.end();) {
       if (static_cast<int32_t>(slot->batch.size()) >= s->cfg.max_batch_events) break;
-      if (same_class((*it)->req, first) && pts + (*it)->req.n <= s->cap) {
+      const int64_t n = (*it)->req.n;
+      if (same_class((*it)->req, first) && pts + n <= s->cap && (slot->batch.empty() || pts + n <= target)) {
         pts += (*it)->req.n;
         slot->batch.push_back(*it);
         it = s->pending.erase(it);
