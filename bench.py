@@ -404,10 +404,10 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic: seeded 2D Gaussian blobs + uniform noise (splitmix64/Box-Muller generator)",
+        "data": f"synthetic: seeded {D}D Gaussian blobs + uniform noise (splitmix64/Box-Muller generator)",
         "config": {"workload": args.workload, "ndim": D, "n_points": N, "dc": prm["dc"], "rhoc": prm["rhoc"],
                    "dm": prm["dm"], "kernel": list(prm["kernel"]), "points_per_tile": 128,
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU", "input_sha256": ps.sha256()},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
                      "algorithmic_bytes_per_launch": N * per_pt[dom], "avg_launch_ms": stages[dom],
