@@ -9,6 +9,13 @@
         measurement) clustered --steps times in one batched call each, with the
         given points per tile and tuning: the program to run under rocprofv3
 
+    python -m cluestering_alpaka3_amd.profiling sq <pmc.csv> [--top 12]
+        per-kernel SQ counters of a --pmc run (SQ_WAVES, SQ_INSTS_*, SQ_BUSY_CYCLES):
+        instructions per wave, and the VALU instructions per second of the kernel
+        (with its mean duration from a --kernel-trace --stats summary, --stats-csv)
+        against one wave64 VALU instruction per 4 cycles on each of 1024 SIMDs at
+        2.4 GHz
+
     python -m cluestering_alpaka3_amd.profiling traffic <fetch.csv> <write.csv> [--copy-mib 64] [--json OUT]
         HBM bytes per dispatch from two --pmc runs (FETCH_SIZE, then WRITE_SIZE),
         each scaled by what the same counter reads for the device-to-device copy
@@ -86,6 +93,17 @@ def traffic(fetch_csv, write_csv, copy_mib=64):
     return out
 
 
+VALU_PEAK = 1024 * 2.4e9 / 4  # wave64 VALU instructions per second (256 CUs x 4 SIMDs)
+
+
+def sq_counters(path):
+    """Mean of each SQ counter per dispatch, per kernel (short name)."""
+    agg = defaultdict(lambda: defaultdict(list))
+    for r in csv.DictReader(open(path)):
+        agg[short_name(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    return {k: {c: sum(v) / len(v) for c, v in cs.items()} for k, cs in agg.items()}
+
+
 def run_batched(events, steps, ppb, nh_first_radius):
     import numpy as np
 
@@ -127,6 +145,10 @@ def main():
     s = sub.add_parser("stats")
     s.add_argument("csv")
     s.add_argument("--top", type=int, default=25)
+    q = sub.add_parser("sq")
+    q.add_argument("csv")
+    q.add_argument("--stats-csv")
+    q.add_argument("--top", type=int, default=12)
     t = sub.add_parser("traffic")
     t.add_argument("fetch_csv")
     t.add_argument("write_csv")
@@ -135,6 +157,24 @@ def main():
     a = ap.parse_args()
     if a.cmd == "run-batched":
         run_batched(a.events, a.steps, a.ppb, a.nh_first_radius)
+    elif a.cmd == "sq":
+        cnt = sq_counters(a.csv)
+        dur = {}
+        if a.stats_csv:
+            for name, calls, avg, pct in kernel_stats(a.stats_csv):
+                dur.setdefault(short_name(name), avg * 1e-9)
+        print(f"{'kernel':26s} {'waves':>8s} {'valu/w':>9s} {'salu/w':>8s} {'smem/w':>7s} {'lds/w':>7s} "
+              f"{'vmem/w':>7s} {'valu_instr/s':>13s} {'of_peak':>8s}")
+        rows = sorted(cnt.items(), key=lambda kv: -kv[1].get("SQ_INSTS_VALU", 0))
+        for k, c in rows[: a.top]:
+            w = max(c.get("SQ_WAVES", 1.0), 1.0)
+            vm = c.get("SQ_INSTS_VMEM_RD", 0.0) + c.get("SQ_INSTS_VMEM_WR", 0.0)
+            line = (f"{k:26s} {w:8.0f} {c.get('SQ_INSTS_VALU', 0) / w:9.1f} {c.get('SQ_INSTS_SALU', 0) / w:8.1f} "
+                    f"{c.get('SQ_INSTS_SMEM', 0) / w:7.1f} {c.get('SQ_INSTS_LDS', 0) / w:7.1f} {vm / w:7.1f}")
+            if k in dur:
+                rate = c.get("SQ_INSTS_VALU", 0) / dur[k]
+                line += f" {rate:13.3e} {rate / VALU_PEAK:8.3f}"
+            print(line)
     elif a.cmd == "stats":
         print(f"{'kernel':100s} {'calls':>6s} {'avg_us':>10s} {'share%':>7s}")
         for name, calls, avg, pct in kernel_stats(a.csv)[: a.top]:
