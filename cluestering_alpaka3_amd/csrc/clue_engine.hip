@@ -2746,31 +2746,52 @@ clue_status clue_main_run(float dc, float rhoc, float dm, float seed_dc, int32_t
   if (n <= 0)
     return fail(CLUE_ERR_INVALID_ARGUMENT,
                 "Number of points passed to PointsDevice constructor must be positive.");
-  clue_plan_t plan = nullptr;
-  clue_status rc = clue_plan_create(device, &plan);
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+    return fail(CLUE_ERR_RUNTIME, "no HIP device available");
+  if (device < 0 || device >= count) return fail(CLUE_ERR_INVALID_ARGUMENT, "invalid device id");
+  // One context per device for the whole process (like the reference's
+  // function-static queue): a plan, a stream and grow-only device buffers,
+  // used by one call at a time and kept until the process ends.
+  struct MainRunCtx {
+    std::mutex mu;
+    clue_plan_t plan = nullptr;
+    void* stream = nullptr;
+    DevBuf in, out;
+  };
+  static std::mutex ctx_mu;
+  static MainRunCtx* ctxs[64] = {};
+  if (device >= 64) return fail(CLUE_ERR_INVALID_ARGUMENT, "invalid device id");
+  MainRunCtx* ctx;
+  {
+    std::lock_guard<std::mutex> g(ctx_mu);
+    if (!ctxs[device]) ctxs[device] = new MainRunCtx();
+    ctx = ctxs[device];
+  }
+  std::lock_guard<std::mutex> g(ctx->mu);
+  clue_status rc = CLUE_OK;
+  if (!ctx->plan) rc = clue_plan_create(device, &ctx->plan);
+  if (rc == CLUE_OK && !ctx->stream) rc = clue_stream_create(device, &ctx->stream);
   if (rc != CLUE_OK) return rc;
-  void* stream = nullptr;
-  float* dbuf = nullptr;
-  int32_t* dout = nullptr;
-  clue_params prm{dc, rhoc, dm, seed_dc, points_per_tile, CLUE_SEEDS_REFERENCE};
+  CLUE_HIP(hipSetDevice(device));
   const size_t nn = static_cast<size_t>(n);
-  rc = clue_stream_create(device, &stream);
-  if (rc == CLUE_OK) rc = clue_malloc_device(device, (ndim + 1) * nn * 4, reinterpret_cast<void**>(&dbuf));
-  if (rc == CLUE_OK) rc = clue_malloc_device(device, nn * 4, reinterpret_cast<void**>(&dout));
-  if (rc == CLUE_OK) rc = clue_memcpy_h2d(dbuf, data, (ndim + 1) * nn * 4, stream);
+  CLUE_HIP(ctx->in.ensure((ndim + 1) * nn * 4));
+  CLUE_HIP(ctx->out.ensure(nn * 4));
+  float* dbuf = ctx->in.as<float>();
+  int32_t* dout = ctx->out.as<int32_t>();
+  clue_params prm{dc, rhoc, dm, seed_dc, points_per_tile, CLUE_SEEDS_REFERENCE};
+  rc = clue_plan_reset_capacity(ctx->plan);  // a fresh Clusterer per call
+  if (rc == CLUE_OK) rc = clue_memcpy_h2d(dbuf, data, (ndim + 1) * nn * 4, ctx->stream);
   if (rc == CLUE_OK) {
     const float* cp[CLUE_MAX_DIM];
     for (int d = 0; d < ndim; ++d) cp[d] = dbuf + d * nn;
-    rc = clue_make_clusters(plan, stream, &prm, ndim, n, cp, dbuf + ndim * nn, kernel, nullptr,
+    rc = clue_make_clusters(ctx->plan, ctx->stream, &prm, ndim, n, cp, dbuf + ndim * nn, kernel, nullptr,
                             wrapped, dout, nullptr, nullptr, nullptr);
   }
-  if (rc == CLUE_OK) rc = clue_memcpy_d2h(results, dout, nn * 4, stream);
-  if (rc == CLUE_OK) rc = clue_stream_synchronize(stream);
+  if (rc == CLUE_OK) rc = clue_memcpy_d2h(results, dout, nn * 4, ctx->stream);
   const std::string keep = g_error;
-  clue_plan_destroy(plan);
-  clue_free_device(dbuf);
-  clue_free_device(dout);
-  clue_stream_destroy(stream);
+  const clue_status rs = clue_stream_synchronize(ctx->stream);
+  if (rc == CLUE_OK) return rs;
   g_error = keep;
   return rc;
 }

@@ -295,8 +295,10 @@ clue_status clue_server_destroy(clue_server_t server);
  * mainRun(dc, rhoc, dm, seed_dc, pPBin, wrapped, data, results, kernel, Ndim,
  * n_points, block_size, device_id), CLUEstering/BindingModules/serial/
  * binding_cpu.cpp:35-52.  data: host float[(ndim+1)*n] SoA (coords then
- * weights); results: host int32[n].  Copies in, clusters with a fresh plan on
- * `device`, copies labels out, synchronises. */
+ * weights); results: host int32[n].  Copies in, clusters as a fresh Clusterer
+ * would on `device`, copies labels out, synchronises.  The device's plan, stream
+ * and buffers are kept for the process's later calls (one call at a time per
+ * device; concurrent callers wait). */
 clue_status clue_main_run(float dc, float rhoc, float dm, float seed_dc, int32_t points_per_tile,
                           const uint8_t* wrapped, const float* data, int32_t* results,
                           const clue_kernel_desc* kernel, int32_t ndim, int32_t n,
