@@ -1167,10 +1167,11 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
     };
 
     // the wave's own points: same tile, neighbouring Morton codes
+    int t0 = -1;
     {
       const int base = uniform(gid & ~(kWave - 1));
       if (base < a.n) {
-        const int t0 = uniform(as_const(a.kn)[base] >> a.mshift);
+        t0 = uniform(as_const(a.kn)[base] >> a.mshift);
         const int beg = uniform(max(base, OFF[t0]));
         const int end = uniform(min(min(base + kWave, OFF[t0 + 1]), a.n));
         const bool need0 = in_box(t0);
@@ -1187,67 +1188,79 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
       any = any && ulo[d] <= uhi[d];
     }
     if (!any) return;
+    const int lane = __lane_id();
+    // one tile of the union: the blocks whose box and largest key can still
+    // improve a lane that needs the tile (need: the tile is in the lane's box)
+    auto tile_pass = [&](int t, bool need) {
+      float tl[D], th[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        tl[d] = TMIN[d][t];
+        th[d] = TMAX[d][t];
+      }
+      need = need && TMK[t] > thr_i && lower_bound2<D>(q, tl, th) <= bound;
+      const float maxb = wave_max_f32(need ? bound : -1.f);
+      if (maxb >= 0.f) {
+        const int k0 = BOFF[t], k1 = BOFF[t + 1];
+        const int off0 = OFF[t], off1 = OFF[t + 1];
+        for (int kb = k0; kb < k1; kb += kWave) {
+          // one block per lane against the wave's query box
+          const int k = kb + lane;
+          const bool inb = k < k1;
+          float bl[D], bh[D];
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            bl[d] = inb ? a.bmin[d][k] : __builtin_inff();
+            bh[d] = inb ? a.bmax[d][k] : -__builtin_inff();
+          }
+          const uint64_t bk = inb ? BMK[k] : 0;
+          float lbw = 0.f;
+#pragma unroll
+          for (int d = D - 1; d >= 0; --d) {
+            float gg = bl[d] - qhi[d];
+            const float g2 = qlo[d] - bh[d];
+            gg = g2 > gg ? g2 : gg;
+            gg = gg > 0.f ? gg : 0.f;
+            const float tt = gg * gg;
+            lbw = (d == D - 1) ? tt : tt + lbw;
+          }
+          uint64_t m = __ballot(inb && lbw <= maxb && bk > wthr);
+          while (m) {
+            const int kl = __builtin_ctzll(m);
+            m &= m - 1;
+            float ul[D], uh[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+              ul[d] = readlane_f32(bl[d], kl);
+              uh[d] = readlane_f32(bh[d], kl);
+            }
+            const bool nk = need && readlane_u64(bk, kl) > thr_i && lower_bound2<D>(q, ul, uh) <= bound;
+            if (__ballot(nk) != 0) {
+              const int start = off0 + (kb + kl - k0) * kBlock;
+              const int end = min(start + kBlock, off1);
+              if (nk) scan(t, start, end);
+            }
+          }
+        }
+      }
+    };
+    // the wave's own tile first (its blocks hold the nearest candidates, so the
+    // lanes' bounds shrink before the neighbouring tiles are walked), then the
+    // rest of the union's tiles in lexicographic order
+    if (t0 >= 0) {
+      const bool need0 = in_box(t0);
+      if (__ballot(need0) != 0) tile_pass(t0, need0);
+    }
     int b[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) b[d] = ulo[d];
-    const int lane = __lane_id();
     while (true) {
       bool need = act;
 #pragma unroll
       for (int d = 0; d < D; ++d) need = need && b[d] >= lo[d] && b[d] <= hi[d];
       if (__ballot(need) != 0) {
         const int t = uniform(tile_index<D>(b, g));
-        float tl[D], th[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          tl[d] = TMIN[d][t];
-          th[d] = TMAX[d][t];
-        }
-        need = need && TMK[t] > thr_i && lower_bound2<D>(q, tl, th) <= bound;
-        const float maxb = wave_max_f32(need ? bound : -1.f);
-        if (maxb >= 0.f) {
-          const int k0 = BOFF[t], k1 = BOFF[t + 1];
-          const int off0 = OFF[t], off1 = OFF[t + 1];
-          for (int kb = k0; kb < k1; kb += kWave) {
-            // one block per lane against the wave's query box
-            const int k = kb + lane;
-            const bool inb = k < k1;
-            float bl[D], bh[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-              bl[d] = inb ? a.bmin[d][k] : __builtin_inff();
-              bh[d] = inb ? a.bmax[d][k] : -__builtin_inff();
-            }
-            const uint64_t bk = inb ? BMK[k] : 0;
-            float lbw = 0.f;
-#pragma unroll
-            for (int d = D - 1; d >= 0; --d) {
-              float gg = bl[d] - qhi[d];
-              const float g2 = qlo[d] - bh[d];
-              gg = g2 > gg ? g2 : gg;
-              gg = gg > 0.f ? gg : 0.f;
-              const float tt = gg * gg;
-              lbw = (d == D - 1) ? tt : tt + lbw;
-            }
-            uint64_t m = __ballot(inb && lbw <= maxb && bk > wthr);
-            while (m) {
-              const int kl = __builtin_ctzll(m);
-              m &= m - 1;
-              float ul[D], uh[D];
-#pragma unroll
-              for (int d = 0; d < D; ++d) {
-                ul[d] = readlane_f32(bl[d], kl);
-                uh[d] = readlane_f32(bh[d], kl);
-              }
-              const bool nk = need && readlane_u64(bk, kl) > thr_i && lower_bound2<D>(q, ul, uh) <= bound;
-              if (__ballot(nk) != 0) {
-                const int start = off0 + (kb + kl - k0) * kBlock;
-                const int end = min(start + kBlock, off1);
-                if (nk) scan(t, start, end);
-              }
-            }
-          }
-        }
+        if (t != t0) tile_pass(t, need);
       }
       int d = D - 1;
       for (; d >= 0; --d) {
