@@ -331,6 +331,24 @@ def test_main_run_matches(ref_data):
     np.testing.assert_array_equal(results, ref.cluster)
 
 
+def test_main_run_repeated_sizes_and_dims():
+    # the device context of mainRun is kept between calls: grow, shrink, change the
+    # dimension, and a fresh seed capacity every call
+    cases = [("2d_16k_flat", 4000, 2), ("3d_4m_exp", 30000, 3), ("2d_16k_flat", 1500, 2), ("10d_256k_flat", 3000, 10)]
+    for name, n, ndim in cases:
+        ps = datasets.workload(name, n=n)
+        prm = datasets.WORKLOADS[name]
+        data = np.concatenate([ps.coords.reshape(-1), ps.weights]).astype(np.float32)
+        results = np.full(n, -7, np.int32)
+        seed_dc = prm["dc"]
+        print(f"case: mainRun {name} n={n}", flush=True)
+        clue.main_run(prm["dc"], prm["rhoc"], prm["dm"], seed_dc, 128, [0] * ndim, data, results, prm["kernel"],
+                      ndim, n, 256, 0)
+        ref = oracle.make_clusters(ps.coords, ps.weights, oracle.make_params(prm["dc"], prm["rhoc"], prm["dm"],
+                                                                            kernel=prm["kernel"]))
+        np.testing.assert_array_equal(results, ref.cluster, err_msg=name)
+
+
 def test_deterministic_repeat():
     ps = datasets.workload("2d_1m_gauss", n=100000, seed=5)
     a = clue.cluster_points(ps.coords, ps.weights, 2.0, 8.0, 4.0, kernel=("gaussian", 0.0, 1.0, 1.0))
