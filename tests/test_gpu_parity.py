@@ -312,6 +312,21 @@ def test_nonpositive_and_special_densities(case):
     assert_same_special(*run_both(x, w, 1.0, 1.0, 2.0, kernel=kernel, tuning=dict(nh_first_radius=1.0)))
 
 
+@pytest.mark.parametrize("offset,scale", [(1.0e6, 1.0), (-3.0e5, 1.0), (0.0, 1.0e-3), (0.0, 250.0)])
+def test_far_and_scaled_coordinates(offset, scale):
+    # large coordinates (the first phase's box margin of 2^-20 |q|), tiny and large
+    # scales (squared distances far from 1), dm below dc, seed_dc above dm, one
+    # point per tile
+    rng = np.random.default_rng(5)
+    ps = datasets.workload("2d_16k_flat", n=6000)
+    x = (ps.coords.astype(np.float64) * scale + offset).astype(np.float32)
+    w = (rng.random(6000) + 0.5).astype(np.float32)
+    dc, dm = np.float32(1.5 * scale), np.float32(3.0 * scale)
+    assert_same(*run_both(x, w, float(dc), 3.0, float(dm)))
+    assert_same(*run_both(x, w, float(dc), 3.0, float(dm) * 0.5, seed_dc=float(dm) * 1.5))
+    assert_same(*run_both(x, w, float(dc), 3.0, float(dm), ppb=1, kernel=("gaussian", 0.0, float(scale), 1.0)))
+
+
 def test_invalid_arguments():
     with pytest.raises(clue.InvalidArgument):
         clue.cluster_points(np.zeros((2, 4), np.float32), np.ones(4, np.float32), -1.0, 1.0)
