@@ -276,7 +276,10 @@ def batched_bench(clue, datasets, args):
 
 def cpu_baseline(ps, prm, args, gpu_labels):
     """The oracle (CPU restatement of the reference's serial backend) on the box's
-    host cores: serial on a spatial window (same local density), OpenMP on all."""
+    host cores: serial on a spatial window (same local density), OpenMP on all.
+    Timed in its build with the reference's Release flags (fast math, vectorised);
+    the IEEE-strict build (the parity reference) is timed beside it and checks the
+    GPU's labels."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
@@ -287,24 +290,35 @@ def cpu_baseline(ps, prm, args, gpu_labels):
     mask = np.all((ps.coords >= lo[:, None]) & (ps.coords < (lo + side)[:, None]), axis=0)
     sub_c = np.ascontiguousarray(ps.coords[:, mask])
     sub_w = np.ascontiguousarray(ps.weights[mask])
-    t0 = time.perf_counter()
-    oracle.make_clusters(sub_c, sub_w, p, with_tiles=False)
-    t_serial = time.perf_counter() - t0
+    times = {}
+    for release in (True, False):
+        t0 = time.perf_counter()
+        oracle.make_clusters(sub_c, sub_w, p, with_tiles=False, release=release)
+        times[release] = time.perf_counter() - t0
+    t_serial = times[True]
     out = {
         "value": round(sub_c.shape[1] / t_serial / 1e6, 4),
         "unit": "Mpoints/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"serial oracle on the points of the same workload inside a "
+        "sample": f"serial oracle, reference Release flags (-O2 -funroll-loops -funsafe-math-optimizations "
+                  f"-ftree-vectorize, AVX2), on the points of the same workload inside a "
                   f"{args.window_fraction:.2f}x{args.window_fraction:.2f} corner window of the box "
                   f"({sub_c.shape[1]} points, same local density), {t_serial:.2f} s",
+        "strict_ieee_build": {"value": round(sub_c.shape[1] / times[False] / 1e6, 4), "unit": "Mpoints/s",
+                              "seconds": round(times[False], 2)},
     }
     threads = args.cpu_threads
     t0 = time.perf_counter()
-    r = oracle.make_clusters_omp(ps.coords, ps.weights, p, threads)
+    oracle.make_clusters_omp(ps.coords, ps.weights, p, threads, release=True)
     t_omp = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    r = oracle.make_clusters_omp(ps.coords, ps.weights, p, threads)
+    t_omp_strict = time.perf_counter() - t0
     out["omp"] = {"value": round(ps.n / t_omp / 1e6, 4), "unit": "Mpoints/s", "cores": threads,
-                  "sample": f"OpenMP oracle, full workload, {t_omp:.2f} s"}
+                  "sample": f"OpenMP oracle (Release flags), full workload, {t_omp:.2f} s",
+                  "strict_ieee_build": {"value": round(ps.n / t_omp_strict / 1e6, 4), "unit": "Mpoints/s",
+                                        "seconds": round(t_omp_strict, 2)}}
     parity = {"labels_equal_oracle": bool(np.array_equal(gpu_labels, r.cluster)),
               "n_clusters_oracle": int(r.stats["n_stored_seeds"]),
               "inradius_pairs_density": int(r.stats["inradius_pairs_density"]),

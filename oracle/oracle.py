@@ -15,6 +15,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# the same sources under the reference's Release-style flags (fast math): for
+# timing the CPU baseline only, never as the parity reference
+_RELEASE_PATH = os.path.join(_HERE, "liboracle_release.so")
 
 MAX_DIM = 10
 KERNELS = {"flat": 0, "gaussian": 1, "exponential": 2}
@@ -79,15 +82,15 @@ class OracleStats(ctypes.Structure):
         return {name: int(getattr(self, name)) for name, _ in self._fields_}
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
-            raise RuntimeError(f"oracle library not built: {_LIB_PATH} (run __graft_entry__.build())")
-        L = ctypes.CDLL(_LIB_PATH)
+def lib(release=False):
+    path = _RELEASE_PATH if release else _LIB_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle library not built: {path} (run __graft_entry__.build())")
+        L = ctypes.CDLL(path)
         P = ctypes.POINTER
         f32p, i32p, f64p = P(ctypes.c_float), P(ctypes.c_int32), P(ctypes.c_double)
         L.oracle_grid_setup.argtypes = [ctypes.c_int32, ctypes.c_int32, f32p, ctypes.c_int32, P(OracleGrid)]
@@ -114,8 +117,8 @@ def lib():
         L.oracle_cluster_map.argtypes = [ctypes.c_int32, i32p, i32p, i32p, i32p]
         L.oracle_cluster_map.restype = ctypes.c_int32
         L.oracle_cluster_centroids.argtypes = [ctypes.c_int32, ctypes.c_int32, f32p, i32p, ctypes.c_int32, i32p, f32p]
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _fp(a):
@@ -189,8 +192,9 @@ def grid(coords, ppb=128):
             "tile_size": np.array(g.tile_size[:d], np.float32)}
 
 
-def make_clusters(coords, weights, params, with_tiles=True) -> OracleResult:
-    """Serial pipeline: coords is (ndim, n) float32 SoA, weights (n,)."""
+def make_clusters(coords, weights, params, with_tiles=True, release=False) -> OracleResult:
+    """Serial pipeline: coords is (ndim, n) float32 SoA, weights (n,).  release=True runs
+    the fast-math build (timing only)."""
     c = _soa(coords)
     w = np.ascontiguousarray(weights, dtype=np.float32)
     ndim, n = c.shape
@@ -204,7 +208,7 @@ def make_clusters(coords, weights, params, with_tiles=True) -> OracleResult:
     gi = grid(c, params.points_per_tile)
     off = np.empty(gi["ntiles"] + 1, np.int32) if with_tiles else None
     idx = np.empty(n, np.int32) if with_tiles else None
-    rc = lib().oracle_make_clusters(ndim, n, _fp(c), _fp(w), ctypes.byref(params), _fp(rho), _ip(nh),
+    rc = lib(release).oracle_make_clusters(ndim, n, _fp(c), _fp(w), ctypes.byref(params), _fp(rho), _ip(nh),
                                     _fp(delta), _ip(is_seed), _ip(cluster), ctypes.byref(st),
                                     _ip(off), _ip(idx), ctypes.byref(g))
     if rc != 0:
@@ -212,7 +216,7 @@ def make_clusters(coords, weights, params, with_tiles=True) -> OracleResult:
     return OracleResult(rho, nh, delta, is_seed, cluster, st.as_dict(), off, idx, gi)
 
 
-def make_clusters_omp(coords, weights, params, nthreads=0):
+def make_clusters_omp(coords, weights, params, nthreads=0, release=False):
     c = _soa(coords)
     w = np.ascontiguousarray(weights, dtype=np.float32)
     ndim, n = c.shape
@@ -221,7 +225,7 @@ def make_clusters_omp(coords, weights, params, nthreads=0):
     is_seed = np.empty(n, np.int32)
     cluster = np.empty(n, np.int32)
     st = OracleStats()
-    rc = lib().oracle_make_clusters_omp(ndim, n, _fp(c), _fp(w), ctypes.byref(params), nthreads,
+    rc = lib(release).oracle_make_clusters_omp(ndim, n, _fp(c), _fp(w), ctypes.byref(params), nthreads,
                                         _fp(rho), _ip(nh), _ip(is_seed), _ip(cluster), ctypes.byref(st))
     if rc != 0:
         raise ValueError("oracle_make_clusters_omp: invalid parameters")
