@@ -655,6 +655,7 @@ struct PassArgs {
   int exp_nonpos;            // the kernel's exp argument is always <= 0 and never NaN
   const int32_t* qorder;     // density: query blocks in decreasing estimated cost (or null)
   const int32_t* tnf;        // density: 1 for tiles holding an infinite or NaN weight
+  const int32_t* qperm;      // tile-order walks: query slots by growing search-box size (or null)
 };
 
 
@@ -786,6 +787,32 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
   return v;
 }
 
+// Size class of each tile-ordered point's search box of radius r (ceil(log2) of
+// its tile count): the launch order of the tile-walk passes in more than three
+// dimensions, where one lane walks its own box and a wave lasts as long as its
+// largest box -- queries sorted by class keep a wave's boxes alike.
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_box_class(const PassArgs a, float r, int32_t* key) {
+  const int s = blockIdx.x * kThreads + threadIdx.x;
+  const bool valid = s < a.n;
+  const int sq = valid ? s : a.n - 1;
+  float q[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) q[d] = a.xs[d][sq];
+  const int my_e = a.pev[a.perm[sq]];
+  for_each_event(valid, my_e, [&](int e, bool act) {
+    const EvGrid<D> g = load_event<D>(a.ev, e, a.wrap);
+    int lo[D], hi[D];
+    search_box<D>(q, r, g, lo, hi);
+    int64_t count = 1;
+#pragma unroll
+    for (int d = 0; d < D; ++d) count *= (hi[d] >= lo[d] ? hi[d] - lo[d] + 1 : 0);
+    int c = 0;
+    while (c < 31 && (int64_t{1} << c) < count) ++c;
+    if (act) key[s] = c;
+  });
+}
+
 // QNH: the lanes take their queries in Morton order (spatially coherent waves,
 // so the lanes of a wave share most of their search box); otherwise in tile order.
 template <int D, int METRIC, int KERNEL, bool UNION, bool QNH>
@@ -813,11 +840,12 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
     s = valid ? a.sn[pq] : -1;
     pid = a.pn[pq];
   } else {
+    const int sq = a.qperm ? a.qperm[pq] : pq;
 #pragma unroll
-    for (int d = 0; d < D; ++d) q[d] = a.xs[d][pq];
-    w_self = a.ws[pq];
-    s = valid ? pq : -1;
-    pid = a.perm[pq];
+    for (int d = 0; d < D; ++d) q[d] = a.xs[d][sq];
+    w_self = a.ws[sq];
+    s = valid ? sq : -1;
+    pid = a.perm[sq];
   }
   const int my_e = a.pev[pid];
   cptr<float> X[D];
@@ -935,7 +963,7 @@ template <int D, int METRIC, bool UNION>
 __global__ __launch_bounds__(kThreads) void k_nearest_higher(const PassArgs a) {
   const int s = logical_block((a.xcd >> 1) & 1) * kThreads + threadIdx.x;
   const bool valid = s < a.n;
-  const int sq = valid ? s : a.n - 1;
+  const int sq = a.qperm ? a.qperm[valid ? s : a.n - 1] : (valid ? s : a.n - 1);
   float q[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) q[d] = a.xs[d][sq];
@@ -1779,6 +1807,7 @@ struct clue_plan_s {
   // Morton-ordered layout and the block boxes of the nearest-higher search
   DevBuf kn, kn2, pn, pn2, mid, xn, wn, hk, bcnt, blk_off, bbox_lo, bbox_hi, tbox_lo, tbox_hi;
   DevBuf bmaxk, tmaxk;  // largest rank key of each block / tile
+  DevBuf qk, qk2, qv, qv2;  // more than three dimensions: sort of the queries by box size
   DevBuf tnf;           // tiles holding an infinite or NaN weight
   DevBuf pending, pending_flag, qcost, qorder;
   // cluster -> points map (post-processing): sort keys and values
@@ -1813,7 +1842,7 @@ struct clue_plan_s {
                    &rho_tmp, &seed_tmp, &bbox_part, &counters, &events, &chunks, &pev,    &sc,
                    &cap_used, &rank_base, &kn,   &kn2,     &pn,      &pn2,      &mid,     &xn,
                    &wn,      &hk,       &bcnt,    &blk_off, &bbox_lo,  &bbox_hi, &tbox_lo,
-                   &tbox_hi, &bmaxk, &tmaxk, &tnf, &pending, &pending_flag, &qcost, &qorder, &mkey,  &mkey2,
+                   &tbox_hi, &bmaxk, &tmaxk, &tnf, &qk, &qk2, &qv, &qv2, &pending, &pending_flag, &qcost, &qorder, &mkey,  &mkey2,
                    &mval,    &mval2};
     return k < static_cast<int>(sizeof(b) / sizeof(b[0])) ? b[k] : nullptr;
   }
@@ -1856,6 +1885,12 @@ clue_status reserve(clue_plan_s* p, int32_t D, int32_t n, int32_t T, int32_t n_e
   CLUE_HIP(p->perm2.ensure(N * 4));
   CLUE_HIP(p->offsets.ensure(TT * 4));
   CLUE_HIP(p->tnf.ensure(TT * 4));
+  if (D > 3) {
+    CLUE_HIP(p->qk.ensure(N * 4));
+    CLUE_HIP(p->qk2.ensure(N * 4));
+    CLUE_HIP(p->qv.ensure(N * 4));
+    CLUE_HIP(p->qv2.ensure(N * 4));
+  }
   CLUE_HIP(p->hist.ensure(hist_entries * 4));
   CLUE_HIP(p->part.ensure((div_up(scan_max, kScanTile) + 1) * 4));
   CLUE_HIP(p->xs.ensure(static_cast<size_t>(ld) * D * 4));
@@ -1935,6 +1970,10 @@ clue_status launch_bin(const Coords& c, int32_t n_chunks, clue_plan_s* p, const 
 
 clue_status mark(clue_plan_s* p, int k, hipStream_t st);
 
+clue_status radix_sort(clue_plan_s* p, hipStream_t st, int32_t n, int total_bits, int32_t* k0,
+                       int32_t* k1, int32_t* v0, int32_t* v1, bool keep_keys,
+                       int32_t** keys_sorted, int32_t** vals_sorted);
+
 template <int D, int METRIC>
 clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, float dm,
                           hipStream_t st, clue_plan_s* p, bool blocks) {
@@ -1946,6 +1985,23 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
   clue_status rc = mark(p, 4, st);
   if (rc != CLUE_OK) return rc;
   a.qorder = nullptr;
+  a.qperm = nullptr;
+  // more than three dimensions: each lane walks its own box; queries sorted by
+  // box size keep a wave's boxes alike (sort of 5-bit classes, stable)
+  auto order_by_box = [&](float r) -> clue_status {
+    k_box_class<D><<<grid, kThreads, 0, st>>>(a, r, p->qk.as<int32_t>());
+    CLUE_LAUNCH_CHECK("box_class");
+    int32_t* keys = nullptr;
+    int32_t* vals = nullptr;
+    const clue_status r2 = radix_sort(p, st, a.n, 5, p->qk.as<int32_t>(), p->qk2.as<int32_t>(), p->qv.as<int32_t>(),
+                                      p->qv2.as<int32_t>(), false, &keys, &vals);
+    a.qperm = vals;
+    return r2;
+  };
+  if constexpr (!U) {
+    rc = order_by_box(dc);
+    if (rc != CLUE_OK) return rc;
+  }
   // heaviest-first pays in 2D (a shorter tail); in 3D the lost spatial locality
   // of concurrent blocks costs more than it saves (measured), hence the default
   const bool order = p->tuning.density_order > 0 || (p->tuning.density_order < 0 && D <= 2);
@@ -1973,6 +2029,10 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
   }
   CLUE_LAUNCH_CHECK("density");
   a.radius = dm;
+  if constexpr (!U) {
+    rc = order_by_box(dm);
+    if (rc != CLUE_OK) return rc;
+  }
   a.thr = sqrt_le_bound(dm);
   rc = mark(p, 5, st);
   if (rc != CLUE_OK) return rc;
