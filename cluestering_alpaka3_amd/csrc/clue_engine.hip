@@ -954,7 +954,9 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
       a.hk_out[pos] = rank_key(rho, id);
       a.rho_out[id] = rho;
     } else {
-      a.rho_out[a.perm[s]] = rho;
+      const int32_t id = a.perm[s];
+      a.rho_out[id] = rho;
+      a.hk_out[s] = rank_key(rho, id);  // tile order: the tile walk's candidates
     }
   }
 }
@@ -967,37 +969,39 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher(const PassArgs a) {
   float q[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) q[d] = a.xs[d][sq];
-  const float rho_i = a.rho_s[sq];
-  const int32_t id_i = a.perm[sq];
+  const uint64_t key_i = a.hk[sq];
+  const uint64_t thr_i = higher_threshold(key_i);
+  const int32_t id_i = static_cast<int32_t>(key_i & 0xffffffffu);
   const int my_e = a.pev[id_i];
   cptr<float> X[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) X[d] = as_const(a.xs[d]);
-  cptr<float> R = as_const(a.rho_s);
-  cptr<int32_t> P = as_const(a.perm);
+  cptr<uint64_t> HK = as_const(a.hk);
   float delta = FLT_MAX;
   int32_t nh = -1;
-  float bound = a.thr;  // updates need s2 <= min(thr(dm), max{s : sqrtf(s) < delta})
+  // updates need s2 <= bound = min(thr(dm), max{s : sqrtf(s) < delta}): the float
+  // below the smallest squared distance whose sqrtf is delta
+  float bound = a.thr;
   // One candidate (ClusteringKernels.hpp:129-144): j is "higher" if rho_j > rho_i,
-  // or equal, positive and with a larger id; the first strict minimum of the
-  // distance in the visit order wins.
-  auto eval = [&](const float* c, float rho_j, int32_t id_j) {
-    const int rb = __float_as_int(rho_j);
-    const bool pos = (rb > 0) && (rb <= 0x7f800000);  // rho_j > 0.f, on the uniform bits
-    const bool higher = (rho_j > rho_i) || ((rho_j == rho_i) && pos && (id_j > id_i));
+  // or equal, positive and with a larger id (one comparison of rank keys); the
+  // first strict minimum of the distance in the visit order wins.
+  auto eval = [&](const float* c, uint64_t key_j) {
+    const bool higher = key_j > thr_i;
     if constexpr (METRIC == MEUCLID) {
       const float s2 = squared_distance<D>(q, c);
       if (higher && s2 <= bound) {
         delta = __builtin_sqrtf(s2);
-        nh = id_j;
-        const float lim = sqrt_lt_bound(delta);
+        nh = static_cast<int32_t>(key_j & 0xffffffffu);
+        float lo2, hi2;
+        sqrt_class(delta, lo2, hi2);
+        const float lim = lo2 > 0.f ? __uint_as_float(__float_as_uint(lo2) - 1u) : -1.f;
         bound = lim < a.thr ? lim : a.thr;
       }
     } else {
       const float dist = metric_distance<D, METRIC>(q, c, a.mp);
       if (higher && dist <= a.radius && dist < delta) {
         delta = dist;
-        nh = id_j;
+        nh = static_cast<int32_t>(key_j & 0xffffffffu);
       }
     }
   };
@@ -1010,24 +1014,23 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher(const PassArgs a) {
     int j = beg;
     if constexpr (UNION) {
       for (; j + 4 <= end; j += 4) {
-        float c[4][D], rv[4];
-        int32_t iv[4];
+        float c[4][D];
+        uint64_t kv[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
 #pragma unroll
           for (int d = 0; d < D; ++d) c[u][d] = X[d][j + u];
-          rv[u] = R[j + u];
-          iv[u] = P[j + u];
+          kv[u] = HK[j + u];
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) eval(c[u], rv[u], iv[u]);
+        for (int u = 0; u < 4; ++u) eval(c[u], kv[u]);
       }
     }
     for (; j < end; ++j) {
       float c[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) c[d] = X[d][j];
-      eval(c, R[j], P[j]);
+      eval(c, HK[j]);
     }
   });
   });
@@ -2508,8 +2511,6 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
     a.kn = mkeys;
     a.mshift = sub_bits * D;
     a.blk_off = p->blk_off.as<int32_t>();
-    a.hk_out = p->hk.as<uint64_t>();
-    a.hk = p->hk.as<uint64_t>();
     a.bmaxk = p->bmaxk.as<uint64_t>();
     a.tmaxk = p->tmaxk.as<uint64_t>();
     a.pending = p->pending.as<int32_t>();
@@ -2517,6 +2518,9 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
   }
   a.xcd = p->tuning.xcd_remap;
   a.tnf = p->tnf.as<int32_t>();
+  // rank keys of the densities: Morton order for the block search, tile order for the tile walk
+  a.hk_out = p->hk.as<uint64_t>();
+  a.hk = p->hk.as<uint64_t>();
   a.rho_s_out = p->rho_s.as<float>();
   a.rho_out = rho_buf;
   a.nh_out = nh_buf;

@@ -310,6 +310,11 @@ def test_nonpositive_and_special_densities(case):
         w[rng.choice(n, 5, replace=False)] = -np.inf
     assert_same_special(*run_both(x, w, 1.0, 1.0, 2.0, kernel=kernel))
     assert_same_special(*run_both(x, w, 1.0, 1.0, 2.0, kernel=kernel, tuning=dict(nh_first_radius=1.0)))
+    # the tile walk (other metrics, wrapped grids, more than three dimensions)
+    assert_same_special(*run_both(x, w, 1.0, 1.0, 2.0, kernel=kernel, metric="manhattan"))
+    assert_same_special(*run_both(x, w, 1.0, 1.0, 2.0, kernel=kernel, wrapped=[1, 0]))
+    x4 = np.concatenate([x, x[::-1]], axis=0)
+    assert_same_special(*run_both(x4, w, 1.5, 1.0, 2.5, kernel=kernel))
 
 
 @pytest.mark.parametrize("offset,scale", [(1.0e6, 1.0), (-3.0e5, 1.0), (0.0, 1.0e-3), (0.0, 250.0)])
