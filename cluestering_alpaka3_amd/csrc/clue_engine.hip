@@ -573,15 +573,23 @@ constexpr int kOrderMax = 16384;  // query blocks the in-LDS ordering handles
 
 // Query blocks in decreasing cost (ties by index), one workgroup bitonic sort in
 // LDS: heavy blocks start first, so the light ones fill in behind them and the
-// pass does not end on a few late heavy blocks.
-__global__ __launch_bounds__(1024) void k_query_block_order(const int32_t* cost, int32_t nb, int32_t* order) {
+// pass does not end on a few late heavy blocks.  With xcd set, the blocks are
+// first split into 8 contiguous ranges of the Morton order, one per XCD (the
+// dispatch gives workgroup b to XCD b % 8), and ordered by cost inside each
+// range: neighbouring queries -- which read the same tiles -- share one L2.
+__global__ __launch_bounds__(1024) void k_query_block_order(const int32_t* cost, int32_t nb, int xcd,
+                                                            int32_t* order) {
   __shared__ uint64_t key[kOrderMax];
   int m = 1;
   while (m < nb) m <<= 1;
+  const int q = nb >> 3, r = nb & 7;
   for (int i = threadIdx.x; i < m; i += 1024) {
-    // larger cost first: sort ascending on (~cost, index)
-    const uint32_t c = i < nb ? static_cast<uint32_t>(cost[i]) : 0u;
-    key[i] = i < nb ? ((static_cast<uint64_t>(~c) << 32) | static_cast<uint32_t>(i)) : ~uint64_t{0};
+    // larger cost first: sort ascending on (range, ~cost, index)
+    const uint32_t c = i < nb ? min(static_cast<uint32_t>(cost[i]), 0xffffffu) : 0u;
+    uint64_t range = 0;
+    if (xcd) range = i < r * (q + 1) ? i / (q + 1) : r + (q > 0 ? (i - r * (q + 1)) / q : 0);
+    key[i] = i < nb ? ((range << 56) | (static_cast<uint64_t>(0xffffffu - c) << 32) | static_cast<uint32_t>(i))
+                    : ~uint64_t{0};
   }
   __syncthreads();
   for (int k = 2; k <= m; k <<= 1) {
@@ -600,7 +608,14 @@ __global__ __launch_bounds__(1024) void k_query_block_order(const int32_t* cost,
       __syncthreads();
     }
   }
-  for (int i = threadIdx.x; i < nb; i += 1024) order[i] = static_cast<int32_t>(key[i] & 0xffffffffu);
+  for (int b = threadIdx.x; b < nb; b += 1024) {
+    int src = b;
+    if (xcd) {  // workgroup b runs on XCD b % 8: the (b / 8)-th block of that XCD's range
+      const int x = b & 7, k = b >> 3;
+      src = (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+    }
+    order[b] = static_cast<int32_t>(key[src] & 0xffffffffu);
+  }
 }
 
 // ===========================================================================
@@ -2011,7 +2026,10 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
   if (blocks && order && grid <= kOrderMax) {
     k_query_block_cost<<<grid, kThreads, 0, st>>>(a.kn, a.mshift, a.offsets, a.n, p->qcost.as<int32_t>());
     CLUE_LAUNCH_CHECK("query_block_cost");
-    k_query_block_order<<<1, 1024, 0, st>>>(p->qcost.as<int32_t>(), grid, p->qorder.as<int32_t>());
+    // xcd_remap bit 2: XCD ranges in the heaviest-first order (a third of the
+    // HBM traffic, a few per cent slower from the ranges' unequal costs)
+    k_query_block_order<<<1, 1024, 0, st>>>(p->qcost.as<int32_t>(), grid, (p->tuning.xcd_remap >> 2) & 1,
+                                            p->qorder.as<int32_t>());
     CLUE_LAUNCH_CHECK("query_block_order");
     a.qorder = p->qorder.as<int32_t>();
   }
@@ -2933,7 +2951,7 @@ clue_status clue_plan_copy_delta(clue_plan_t plan, float* delta) {
 clue_status clue_plan_set_tuning(clue_plan_t plan, const clue_tuning* tuning) {
   if (!plan || !tuning) return fail(CLUE_ERR_INVALID_ARGUMENT, "NULL argument");
   if (!(tuning->nh_first_radius > 0.f) || tuning->morton_bits > 10 ||
-      tuning->xcd_remap < 0 || tuning->xcd_remap > 3 || tuning->pending_blocks < 0 ||
+      tuning->xcd_remap < 0 || tuning->xcd_remap > 7 || tuning->pending_blocks < 0 ||
       (tuning->pending_group != 8 && tuning->pending_group != 16 && tuning->pending_group != 32 &&
        tuning->pending_group != 64) ||
       tuning->density_order < -1 || tuning->density_order > 1)

@@ -154,7 +154,8 @@ TUNINGS = [dict(nh_first_radius=0.05, morton_bits=5),
            dict(xcd_remap=0, pending_blocks=1, pending_group=8),
            dict(xcd_remap=3, pending_blocks=7, pending_group=64),
            dict(xcd_remap=2, pending_blocks=0, nh_first_radius=0.01, pending_group=32, density_order=0),
-           dict(density_order=1, xcd_remap=0, pending_group=16)]
+           dict(density_order=1, xcd_remap=0, pending_group=16),
+           dict(density_order=1, xcd_remap=7)]
 
 
 @pytest.mark.parametrize("tuning", TUNINGS)
