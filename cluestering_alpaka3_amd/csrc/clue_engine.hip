@@ -586,8 +586,13 @@ __global__ __launch_bounds__(1024) void k_query_block_order(const int32_t* cost,
   for (int i = threadIdx.x; i < m; i += 1024) {
     // larger cost first: sort ascending on (range, ~cost, index)
     const uint32_t c = i < nb ? min(static_cast<uint32_t>(cost[i]), 0xffffffu) : 0u;
+    // XCD of a block: its chunk of the Morton order (32 chunks) modulo 8, so each
+    // XCD holds four separate eighths-of-a-quarter of the domain (their costs even
+    // out); a group's size may miss its XCD's share by a chunk, which only moves
+    // those blocks to a neighbouring XCD
     uint64_t range = 0;
-    if (xcd) range = i < r * (q + 1) ? i / (q + 1) : r + (q > 0 ? (i - r * (q + 1)) / q : 0);
+    const int chunk = (nb + 31) / 32;
+    if (xcd) range = (i / chunk) & 7;
     key[i] = i < nb ? ((range << 56) | (static_cast<uint64_t>(0xffffffu - c) << 32) | static_cast<uint32_t>(i))
                     : ~uint64_t{0};
   }
@@ -1830,7 +1835,7 @@ struct clue_plan_s {
   DevBuf pending, pending_flag, qcost, qorder;
   // cluster -> points map (post-processing): sort keys and values
   DevBuf mkey, mkey2, mval, mval2;
-  clue_tuning tuning{0.15f, -1, 3, 0, 16, -1};
+  clue_tuning tuning{0.15f, -1, 7, 0, 16, -1};
   int64_t ldb = 0;  // leading dimension of the block boxes
   int64_t ld = 0;   // leading dimension of xs / xn
   // pinned staging ring of the event and chunk tables (a slot is reused once its
@@ -2026,8 +2031,8 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
   if (blocks && order && grid <= kOrderMax) {
     k_query_block_cost<<<grid, kThreads, 0, st>>>(a.kn, a.mshift, a.offsets, a.n, p->qcost.as<int32_t>());
     CLUE_LAUNCH_CHECK("query_block_cost");
-    // xcd_remap bit 2: XCD ranges in the heaviest-first order (a third of the
-    // HBM traffic, a few per cent slower from the ranges' unequal costs)
+    // xcd_remap bit 2: XCD groups of Morton chunks inside the heaviest-first
+    // order (less than half the HBM traffic at the same speed)
     k_query_block_order<<<1, 1024, 0, st>>>(p->qcost.as<int32_t>(), grid, (p->tuning.xcd_remap >> 2) & 1,
                                             p->qorder.as<int32_t>());
     CLUE_LAUNCH_CHECK("query_block_order");
@@ -3240,7 +3245,7 @@ clue_status clue_autotune_batched(clue_plan_t plan, void* stream, clue_params* p
   auto set_xcd = [](clue_tuning& t, float v) { t.xcd_remap = static_cast<int32_t>(v); };
   auto set_group = [](clue_tuning& t, float v) { t.pending_group = static_cast<int32_t>(v); };
   try_field(set_radius, {0.08f, 0.12f, 0.18f, 0.25f, 0.35f, 0.5f, 0.75f, 1.0f});
-  try_field(set_xcd, {0.f, 1.f, 2.f, 3.f});
+  try_field(set_xcd, {0.f, 1.f, 2.f, 3.f, 4.f, 5.f, 6.f, 7.f});
   try_field(set_group, {8.f, 16.f, 32.f, 64.f});
   try_field([](clue_tuning& t, float v) { t.density_order = static_cast<int32_t>(v); }, {0.f, 1.f});
   try_field(set_radius, {cur.nh_first_radius * 0.8f, cur.nh_first_radius * 1.25f});

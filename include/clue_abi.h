@@ -102,8 +102,8 @@ typedef struct {
  *  xcd_remap           bit 0: density, bit 1: nearest-higher -- give each
  *                      XCD a contiguous range of the pass's query blocks, so
  *                      neighbouring queries share one L2; bit 2: the same for
- *                      the density's heaviest-first order (XCD ranges, each
- *                      heaviest first)
+ *                      the density's heaviest-first order (each XCD four
+ *                      separate chunks of the Morton order, heaviest first)
  *  pending_blocks      workgroups of the second nearest-higher phase, 0 auto
  *  pending_group       lanes per unresolved point in that phase: 8, 16, 32, 64
  *  density_order       1: launch the density pass's query blocks heaviest first
