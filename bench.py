@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--batch-events", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--no-served", action="store_true", help="skip the served-requests measurement")
-    ap.add_argument("--served-events", type=int, default=600)
+    ap.add_argument("--served-events", type=int, default=1500)
     ap.add_argument("--served-slots", type=int, default=4)
     ap.add_argument("--served-load", type=float, default=0.7,
                     help="Poisson arrival rate as a fraction of the measured saturated throughput")

@@ -32,6 +32,7 @@ namespace clue_hip {
 constexpr int kThreads = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kThreads * kScanItems;  // 4096
+constexpr int kScanRawBlocks = 4 * kThreads;
 constexpr int kSortItems = 16;
 constexpr int kSortTile = kThreads * kSortItems;  // 4096 keys per block, 1024 per wave
 constexpr int kBBoxBlocks = 256;
@@ -128,8 +129,12 @@ __global__ __launch_bounds__(1024) void k_scan_partials(int32_t* part, int nb, i
   if (threadIdx.x == 0 && total_out) *total_out = carry;
 }
 
+// raw_partials: part holds the blocks' sums (not yet scanned); each block then
+// adds up those before it itself, and the last block writes out[n] -- the
+// middle phase's launch saved where the blocks are few.
 __global__ __launch_bounds__(kThreads) void k_scan_down(const int32_t* in, int64_t n,
-                                                         const int32_t* part, int32_t* out) {
+                                                         const int32_t* part, int32_t* out,
+                                                         int raw_partials) {
   __shared__ int32_t tile[kScanTile];
   __shared__ int smem[kThreads / kWave];
   const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
@@ -138,13 +143,22 @@ __global__ __launch_bounds__(kThreads) void k_scan_down(const int32_t* in, int64
     const int64_t idx = base + k * kThreads + threadIdx.x;
     tile[k * kThreads + threadIdx.x] = idx < n ? in[idx] : 0;
   }
+  int carry;
+  if (raw_partials) {
+    int v = 0;
+    for (int b = threadIdx.x; b < static_cast<int>(blockIdx.x); b += kThreads) v += part[b];
+    block_incl_scan<kThreads>(v, smem, carry);
+  } else {
+    carry = part[blockIdx.x];
+  }
   __syncthreads();
   int local = 0;
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) local += tile[threadIdx.x * kScanItems + k];
   int total;
   const int incl = block_incl_scan<kThreads>(local, smem, total);
-  int prefix = part[blockIdx.x] + incl - local;
+  if (raw_partials && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = carry + total;
+  int prefix = carry + incl - local;
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
     const int v = tile[threadIdx.x * kScanItems + k];
@@ -292,10 +306,27 @@ __global__ __launch_bounds__(kThreads) void k_radix_hist(const int32_t* keys, in
   for (int t = threadIdx.x; t < 256; t += kThreads) h[t] = 0;
   __syncthreads();
   const int64_t base = static_cast<int64_t>(blockIdx.x) * kSortTile;
-#pragma unroll 4
+  const int lane = __lane_id();
+  int32_t kk[kSortItems];
+#pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
     const int64_t idx = base + r * kThreads + threadIdx.x;
-    if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & (radix - 1)], 1);
+    kk[r] = idx < n ? keys[idx] : -1;
+  }
+  // lanes holding the same digit are matched by one ballot per digit bit and
+  // counted by their lowest lane: one LDS atomic per distinct digit of a wave
+  // (keys of later passes arrive partly sorted, so many lanes share a digit)
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const bool valid = base + r * kThreads + threadIdx.x < n;
+    const int digit = (kk[r] >> shift) & (radix - 1);
+    uint64_t peers = __ballot(valid);
+    for (int b = 0; b < bits; ++b) {
+      const bool bit = (digit >> b) & 1;
+      const uint64_t m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    if (valid && (peers & ((1ull << lane) - 1)) == 0) atomicAdd(&h[digit], __popcll(peers));
   }
   __syncthreads();
   for (int t = threadIdx.x; t < radix; t += kThreads) hist[t * nblocks + blockIdx.x] = h[t];
@@ -1973,9 +2004,13 @@ clue_status scan_exclusive(const int32_t* in, int32_t* out, int64_t n, int32_t* 
   }
   k_scan_reduce<<<nb, kThreads, 0, st>>>(in, n, part);
   CLUE_LAUNCH_CHECK("scan_reduce");
-  k_scan_partials<<<1, 1024, 0, st>>>(part, nb, out + n);
-  CLUE_LAUNCH_CHECK("scan_partials");
-  k_scan_down<<<nb, kThreads, 0, st>>>(in, n, part, out);
+  // up to kScanRawBlocks blocks, each sums the partials before it (at most 4 per thread)
+  const int raw = nb <= kScanRawBlocks;
+  if (!raw) {
+    k_scan_partials<<<1, 1024, 0, st>>>(part, nb, out + n);
+    CLUE_LAUNCH_CHECK("scan_partials");
+  }
+  k_scan_down<<<nb, kThreads, 0, st>>>(in, n, part, out, raw);
   CLUE_LAUNCH_CHECK("scan_down");
   return CLUE_OK;
 }
