@@ -478,33 +478,52 @@ __global__ void k_block_count(const int32_t* offsets, int32_t T, int32_t* cnt) {
   if (t < T) cnt[t] = (offsets[t + 1] - offsets[t] + kBlock - 1) / kBlock;
 }
 
-// Bounding box of each block of 32 consecutive Morton-ordered points of a tile.
+// Tile of every block of the Morton layout (the blocks are numbered tile by tile).
+__global__ void k_block_tiles(const int32_t* blk_off, int32_t T, int32_t* btile) {
+  const int t = blockIdx.x * kThreads + threadIdx.x;
+  if (t >= T) return;
+  const int b1 = blk_off[t + 1];
+  for (int b = blk_off[t]; b < b1; ++b) btile[b] = t;
+}
+
+// Bounding box of each block of 32 consecutive Morton-ordered points of a tile:
+// kBoxLanes lanes per block, each reading every kBoxLanes-th point, then a
+// reduction across the lane group.
+constexpr int kBoxLanes = 8;
 __global__ __launch_bounds__(kThreads) void k_block_boxes(const float* xn, int64_t ld, int D,
                                                            const int32_t* offsets,
-                                                           const int32_t* blk_off, int32_t T,
-                                                           float* bmin, float* bmax,
+                                                           const int32_t* blk_off, const int32_t* btile,
+                                                           int32_t T, float* bmin, float* bmax,
                                                            int64_t ldb) {
-  const int b = logical_block(1) * kThreads + threadIdx.x;
+  const int64_t gid = static_cast<int64_t>(logical_block(1)) * kThreads + threadIdx.x;
+  const int b = static_cast<int>(gid / kBoxLanes);
+  const int sub = threadIdx.x % kBoxLanes;
   const int nblk = blk_off[T];
-  if (b >= nblk) return;
-  int lo = 0, hi = T;  // last t with blk_off[t] <= b
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) / 2;
-    if (blk_off[mid] <= b) lo = mid;
-    else hi = mid;
-  }
-  const int t = lo;
+  if (b >= nblk) return;  // uniform over the lane group
+  const int t = btile[b];
   const int start = offsets[t] + (b - blk_off[t]) * kBlock;
   const int end = min(start + kBlock, offsets[t + 1]);
   for (int d = 0; d < D; ++d) {
     float mn = __builtin_inff(), mx = -__builtin_inff();
-    for (int j = start; j < end; ++j) {
-      const float v = xn[d * ld + j];
-      mn = v < mn ? v : mn;
-      mx = mx < v ? v : mx;
+#pragma unroll
+    for (int k = 0; k < kBlock / kBoxLanes; ++k) {
+      const int j = start + sub + k * kBoxLanes;
+      if (j < end) {
+        const float v = xn[d * ld + j];
+        mn = v < mn ? v : mn;
+        mx = mx < v ? v : mx;
+      }
     }
-    bmin[d * ldb + b] = mn;
-    bmax[d * ldb + b] = mx;
+#pragma unroll
+    for (int off = kBoxLanes / 2; off > 0; off >>= 1) {
+      const float omn = __shfl_xor(mn, off), omx = __shfl_xor(mx, off);
+      mn = omn < mn ? omn : mn;
+      mx = mx < omx ? omx : mx;
+    }
+    if (sub == 0) {
+      bmin[d * ldb + b] = mn;
+      bmax[d * ldb + b] = mx;
+    }
   }
 }
 
@@ -1859,6 +1878,7 @@ struct clue_plan_s {
   // events of the call: descriptors, chunks, point -> event, per-event counters
   DevBuf events, chunks, pev, sc, cap_used, rank_base;
   // Morton-ordered layout and the block boxes of the nearest-higher search
+  DevBuf btile;
   DevBuf kn, kn2, pn, pn2, mid, xn, wn, hk, bcnt, blk_off, bbox_lo, bbox_hi, tbox_lo, tbox_hi;
   DevBuf bmaxk, tmaxk;  // largest rank key of each block / tile
   DevBuf qk, qk2, qv, qv2;  // more than three dimensions: sort of the queries by box size
@@ -1897,7 +1917,7 @@ struct clue_plan_s {
                    &cap_used, &rank_base, &kn,   &kn2,     &pn,      &pn2,      &mid,     &xn,
                    &wn,      &hk,       &bcnt,    &blk_off, &bbox_lo,  &bbox_hi, &tbox_lo,
                    &tbox_hi, &bmaxk, &tmaxk, &tnf, &qk, &qk2, &qv, &qv2, &pending, &pending_flag, &qcost, &qorder, &mkey,  &mkey2,
-                   &mval,    &mval2};
+                   &mval,    &mval2,  &btile};
     return k < static_cast<int>(sizeof(b) / sizeof(b[0])) ? b[k] : nullptr;
   }
 };
@@ -1982,6 +2002,7 @@ clue_status reserve(clue_plan_s* p, int32_t D, int32_t n, int32_t T, int32_t n_e
   const int64_t ldb = div_up(n, kBlock) + T + 1;
   CLUE_HIP(p->bbox_lo.ensure(static_cast<size_t>(ldb) * D * 4));
   CLUE_HIP(p->bbox_hi.ensure(static_cast<size_t>(ldb) * D * 4));
+  CLUE_HIP(p->btile.ensure(static_cast<size_t>(ldb) * 4));
   p->ldb = ldb;
   CLUE_HIP(p->tbox_lo.ensure(static_cast<size_t>(T) * D * 4));
   CLUE_HIP(p->tbox_hi.ensure(static_cast<size_t>(T) * D * 4));
@@ -2517,9 +2538,12 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
     CLUE_LAUNCH_CHECK("block_count");
     rc = scan_exclusive(p->bcnt.as<int32_t>(), p->blk_off.as<int32_t>(), Ttot, p->part.as<int32_t>(), st);
     if (rc != CLUE_OK) return rc;
-    k_block_boxes<<<div_up(div_up(n, kBlock) + Ttot, kThreads), kThreads, 0, st>>>(
-        p->xn.as<float>(), p->ld, D, p->offsets.as<int32_t>(), p->blk_off.as<int32_t>(), Ttot,
-        p->bbox_lo.as<float>(), p->bbox_hi.as<float>(), p->ldb);
+    k_block_tiles<<<div_up(Ttot, kThreads), kThreads, 0, st>>>(p->blk_off.as<int32_t>(), Ttot,
+                                                                p->btile.as<int32_t>());
+    CLUE_LAUNCH_CHECK("block_tiles");
+    k_block_boxes<<<div_up((div_up(n, kBlock) + Ttot) * kBoxLanes, kThreads), kThreads, 0, st>>>(
+        p->xn.as<float>(), p->ld, D, p->offsets.as<int32_t>(), p->blk_off.as<int32_t>(), p->btile.as<int32_t>(),
+        Ttot, p->bbox_lo.as<float>(), p->bbox_hi.as<float>(), p->ldb);
     CLUE_LAUNCH_CHECK("block_boxes");
     k_tile_boxes<<<div_up(static_cast<int64_t>(Ttot) * kWave, kThreads), kThreads, 0, st>>>(
         p->bbox_lo.as<float>(), p->bbox_hi.as<float>(), p->ldb, D, p->blk_off.as<int32_t>(), Ttot,
