@@ -89,7 +89,7 @@ def traffic(fetch_csv, write_csv, copy_mib=64):
         fb, wb = fk * 1024 * ff, wk * 1024 * fw
         if prev is None or fb + wb > prev["traffic_bytes"]:
             out["kernels"][key] = {"name": name, "dispatches": len(fetch[name]), "fetch_bytes": round(fb),
-                                   "write_bytes": round(wb), "traffic_bytes": round(fb + wb)}
+                                   "write_bytes": round(wb), "traffic_bytes": round(fb) + round(wb)}
     return out
 
 
