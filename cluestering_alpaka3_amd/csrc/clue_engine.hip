@@ -489,7 +489,8 @@ __global__ void k_block_tiles(const int32_t* blk_off, int32_t T, int32_t* btile)
 // Bounding box of each block of 32 consecutive Morton-ordered points of a tile:
 // kBoxLanes lanes per block, each reading every kBoxLanes-th point, then a
 // reduction across the lane group.
-constexpr int kBoxLanes = 8;
+constexpr int kBoxLanes = 8;    // lanes per block
+constexpr int kTileLanes = 16;  // lanes per tile
 __global__ __launch_bounds__(kThreads) void k_block_boxes(const float* xn, int64_t ld, int D,
                                                            const int32_t* offsets,
                                                            const int32_t* blk_off, const int32_t* btile,
@@ -531,24 +532,25 @@ __global__ __launch_bounds__(kThreads) void k_tile_boxes(const float* bmin, cons
                                                           int64_t ldb, int D,
                                                           const int32_t* blk_off, int32_t T,
                                                           float* tmin, float* tmax) {
-  // one wave per tile: the lanes split its blocks, a wave reduction combines
-  const int t = static_cast<int>((static_cast<int64_t>(logical_block(1)) * kThreads + threadIdx.x) / kWave);
-  if (t >= T) return;  // wave-uniform
-  const int lane = __lane_id();
+  // kTileLanes lanes per tile split its blocks; a reduction across the group combines
+  const int t = static_cast<int>((static_cast<int64_t>(logical_block(1)) * kThreads + threadIdx.x) / kTileLanes);
+  if (t >= T) return;  // uniform over the lane group
+  const int sub = threadIdx.x % kTileLanes;
   const int b0 = blk_off[t], b1 = blk_off[t + 1];
   for (int d = 0; d < D; ++d) {
     float mn = __builtin_inff(), mx = -__builtin_inff();
-    for (int b = b0 + lane; b < b1; b += kWave) {
+#pragma unroll 4
+    for (int b = b0 + sub; b < b1; b += kTileLanes) {
       mn = bmin[d * ldb + b] < mn ? bmin[d * ldb + b] : mn;
       mx = mx < bmax[d * ldb + b] ? bmax[d * ldb + b] : mx;
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
+    for (int off = kTileLanes / 2; off > 0; off >>= 1) {
       const float omn = __shfl_xor(mn, off), omx = __shfl_xor(mx, off);
       mn = omn < mn ? omn : mn;
       mx = mx < omx ? omx : mx;
     }
-    if (lane == 0) {
+    if (sub == 0) {
       tmin[d * T + t] = mn;
       tmax[d * T + t] = mx;
     }
@@ -557,47 +559,57 @@ __global__ __launch_bounds__(kThreads) void k_tile_boxes(const float* bmin, cons
 
 // Largest rank key of each block and of each tile (the density filter of the
 // nearest-higher searches: a block whose largest key does not exceed a query's
-// threshold holds no point higher than it).  One lane per Morton position; runs
-// of equal block (tile) inside the wave are reduced by a segmented suffix
-// maximum and their first lane combines the run into the block (tile) with an
-// atomic maximum (a block spans at most two waves).  bmaxk and tmaxk start at 0.
-__global__ __launch_bounds__(kThreads) void k_block_maxkey(const uint64_t* hk, const int32_t* kn, int mshift,
-                                                            const int32_t* offsets, const int32_t* blk_off,
-                                                            int32_t n, unsigned long long* bmaxk,
-                                                            unsigned long long* tmaxk) {
-  const int p = logical_block(1) * kThreads + threadIdx.x;
-  const int lane = __lane_id();
-  const bool valid = p < n;
-  int t = -1, b = -1;
-  uint64_t v = 0;
-  if (valid) {
-    t = kn[p] >> mshift;
-    b = blk_off[t] + (p - offsets[t]) / kBlock;
-    v = hk[p];
-  }
-  uint64_t vt = v;
+// threshold holds no point higher than it).  kBoxLanes lanes per block read
+// every kBoxLanes-th key of its Morton positions; then kTileLanes lanes per tile
+// combine its blocks.  An empty tile gets 0, below every key.
+__global__ __launch_bounds__(kThreads) void k_block_maxkey(const uint64_t* hk, const int32_t* offsets,
+                                                            const int32_t* blk_off, const int32_t* btile,
+                                                            int32_t T, uint64_t* bmaxk) {
+  const int64_t gid = static_cast<int64_t>(logical_block(1)) * kThreads + threadIdx.x;
+  const int b = static_cast<int>(gid / kBoxLanes);
+  const int sub = threadIdx.x % kBoxLanes;
+  if (b >= blk_off[T]) return;  // uniform over the lane group
+  const int t = btile[b];
+  const int start = offsets[t] + (b - blk_off[t]) * kBlock;
+  const int end = min(start + kBlock, offsets[t + 1]);
+  uint64_t m = 0;
 #pragma unroll
-  for (int off = 1; off < kWave; off <<= 1) {
-    const int ob = __shfl_down(b, off);
-    const int ot = __shfl_down(t, off);
-    const uint32_t vlo = __shfl_down(static_cast<uint32_t>(v), off);
-    const uint32_t vhi = __shfl_down(static_cast<uint32_t>(v >> 32), off);
-    const uint32_t tlo = __shfl_down(static_cast<uint32_t>(vt), off);
-    const uint32_t thi = __shfl_down(static_cast<uint32_t>(vt >> 32), off);
-    const uint64_t ov = (static_cast<uint64_t>(vhi) << 32) | vlo;
-    const uint64_t otv = (static_cast<uint64_t>(thi) << 32) | tlo;
-    const bool in = lane + off < kWave;
-    if (in && ob == b && ov > v) v = ov;
-    if (in && ot == t && otv > vt) vt = otv;
+  for (int k = 0; k < kBlock / kBoxLanes; ++k) {
+    const int j = start + sub + k * kBoxLanes;
+    if (j < end) {
+      const uint64_t v = hk[j];
+      m = v > m ? v : m;
+    }
   }
-  const int pb = __shfl_up(b, 1);
-  const int pt = __shfl_up(t, 1);
-  if (valid && (lane == 0 || pb != b)) atomicMax(bmaxk + b, static_cast<unsigned long long>(v));
-  if (valid && (lane == 0 || pt != t)) atomicMax(tmaxk + t, static_cast<unsigned long long>(vt));
+#pragma unroll
+  for (int off = kBoxLanes / 2; off > 0; off >>= 1) {
+    const uint32_t lo = __shfl_xor(static_cast<uint32_t>(m), off);
+    const uint32_t hi = __shfl_xor(static_cast<uint32_t>(m >> 32), off);
+    const uint64_t o = (static_cast<uint64_t>(hi) << 32) | lo;
+    m = o > m ? o : m;
+  }
+  if (sub == 0) bmaxk[b] = m;
 }
 
-// Estimated cost of each block of kThreads Morton-ordered queries: the sum of
-// the sizes of the queries' tiles (the candidates grow with the local density).
+__global__ __launch_bounds__(kThreads) void k_tile_maxkey(const uint64_t* bmaxk, const int32_t* blk_off,
+                                                           int32_t T, uint64_t* tmaxk) {
+  const int t = static_cast<int>((static_cast<int64_t>(logical_block(1)) * kThreads + threadIdx.x) / kTileLanes);
+  if (t >= T) return;  // uniform over the lane group
+  const int sub = threadIdx.x % kTileLanes;
+  const int b1 = blk_off[t + 1];
+  uint64_t m = 0;
+#pragma unroll 4
+  for (int b = blk_off[t] + sub; b < b1; b += kTileLanes) m = bmaxk[b] > m ? bmaxk[b] : m;
+#pragma unroll
+  for (int off = kTileLanes / 2; off > 0; off >>= 1) {
+    const uint32_t lo = __shfl_xor(static_cast<uint32_t>(m), off);
+    const uint32_t hi = __shfl_xor(static_cast<uint32_t>(m >> 32), off);
+    const uint64_t o = (static_cast<uint64_t>(hi) << 32) | lo;
+    m = o > m ? o : m;
+  }
+  if (sub == 0) tmaxk[t] = m;
+}
+
 __global__ __launch_bounds__(kThreads) void k_query_block_cost(const int32_t* kn, int mshift,
                                                                 const int32_t* offsets, int32_t n,
                                                                 int32_t* cost) {
@@ -701,6 +713,8 @@ struct PassArgs {
   const int32_t* kn;         // sorted Morton keys (tile id << mshift | code)
   int32_t mshift;
   const int32_t* blk_off;    // first block of each tile
+  const int32_t* btile;      // tile of each block
+  int32_t ntiles;
   const float* bmin[kMaxDim];
   const float* bmax[kMaxDim];
   const float* tmin[kMaxDim];
@@ -2134,10 +2148,12 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
         }
         a.lvl[a.nlvl++] = a.thr;
       }
-      k_block_maxkey<<<grid, kThreads, 0, st>>>(a.hk, a.kn, a.mshift, a.offsets, a.blk_off, a.n,
-                                                reinterpret_cast<unsigned long long*>(p->bmaxk.as<uint64_t>()),
-                                                reinterpret_cast<unsigned long long*>(p->tmaxk.as<uint64_t>()));
+      k_block_maxkey<<<div_up((div_up(a.n, kBlock) + a.ntiles) * kBoxLanes, kThreads), kThreads, 0, st>>>(
+          a.hk, a.offsets, a.blk_off, a.btile, a.ntiles, p->bmaxk.as<uint64_t>());
       CLUE_LAUNCH_CHECK("block_maxkey");
+      k_tile_maxkey<<<div_up(static_cast<int64_t>(a.ntiles) * kTileLanes, kThreads), kThreads, 0, st>>>(
+          p->bmaxk.as<uint64_t>(), a.blk_off, a.ntiles, p->tmaxk.as<uint64_t>());
+      CLUE_LAUNCH_CHECK("tile_maxkey");
       k_nearest_higher_blocks<D><<<grid, kThreads, 0, st>>>(a);
       CLUE_LAUNCH_CHECK("nearest_higher_blocks");
       rc = scan_exclusive(a.pending_flag, a.pending_flag, a.n, p->part.as<int32_t>(), st);
@@ -2545,15 +2561,10 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
         p->xn.as<float>(), p->ld, D, p->offsets.as<int32_t>(), p->blk_off.as<int32_t>(), p->btile.as<int32_t>(),
         Ttot, p->bbox_lo.as<float>(), p->bbox_hi.as<float>(), p->ldb);
     CLUE_LAUNCH_CHECK("block_boxes");
-    k_tile_boxes<<<div_up(static_cast<int64_t>(Ttot) * kWave, kThreads), kThreads, 0, st>>>(
+    k_tile_boxes<<<div_up(static_cast<int64_t>(Ttot) * kTileLanes, kThreads), kThreads, 0, st>>>(
         p->bbox_lo.as<float>(), p->bbox_hi.as<float>(), p->ldb, D, p->blk_off.as<int32_t>(), Ttot,
         p->tbox_lo.as<float>(), p->tbox_hi.as<float>());
     CLUE_LAUNCH_CHECK("tile_boxes");
-    // the largest rank keys start at 0 (below every key) for the atomic maxima
-    k_fill_i32<<<div_up(2 * p->ldb, kThreads), kThreads, 0, st>>>(p->bmaxk.as<int32_t>(), 0, 2 * p->ldb);
-    k_fill_i32<<<div_up(2 * static_cast<int64_t>(Ttot), kThreads), kThreads, 0, st>>>(p->tmaxk.as<int32_t>(), 0,
-                                                                                      2 * static_cast<int64_t>(Ttot));
-    CLUE_LAUNCH_CHECK("fill");
   }
 
   // ---- density + nearest-higher
@@ -2593,6 +2604,8 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
     a.kn = mkeys;
     a.mshift = sub_bits * D;
     a.blk_off = p->blk_off.as<int32_t>();
+    a.btile = p->btile.as<int32_t>();
+    a.ntiles = Ttot;
     a.bmaxk = p->bmaxk.as<uint64_t>();
     a.tmaxk = p->tmaxk.as<uint64_t>();
     a.pending = p->pending.as<int32_t>();
