@@ -262,22 +262,29 @@ __device__ __forceinline__ EvGrid<D> load_event(const EventDesc* events, int e, 
 // ===========================================================================
 // Tile id per point + histogram (K1 + K2 fused).
 // ===========================================================================
+// Floats per point of the packed copy written by k_bin: D coordinates, the
+// weight and the (tile, Morton code) key, rounded up to whole 16-byte vectors.
 template <int D>
-__global__ __launch_bounds__(kThreads) void k_bin(Coords c, const Chunk* chunks,
+constexpr int packed_stride() { return (D + 2 + 3) / 4 * 4; }
+
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_bin(Coords c, const float* w, const Chunk* chunks,
                                                    const EventDesc* events, Wrap wrap, int32_t* key,
-                                                   int32_t* key_nh, int sub_bits,
-                                                   int32_t* pev) {
+                                                   int morton, int sub_bits, int32_t* pev, float* packed) {
+  constexpr int S = packed_stride<D>();
   const Chunk ch = chunks[blockIdx.x];
   const EvGrid<D> g = load_event<D>(events, ch.event, wrap);
   for (int i = ch.begin + threadIdx.x; i < ch.end; i += kThreads) {
     int32_t gb = g.tile_base;
     uint32_t code = 0;
+    float v[S];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       const float x = c.p[d][i];
+      v[d] = x;
       const int32_t b = tile_bin(x, g.mn[d], g.mx[d], g.ts[d], g.npd, g.wrapped[d]);
       gb += g.stride[d] * b;
-      if (key_nh) {
+      if (morton) {
         // position inside the tile on a 2^sub_bits grid, interleaved into a Morton code
         const float f = (x - g.mn[d]) / g.ts[d] - static_cast<float>(b);
         int32_t sub = f2i(f * static_cast<float>(1 << sub_bits));
@@ -287,7 +294,13 @@ __global__ __launch_bounds__(kThreads) void k_bin(Coords c, const Chunk* chunks,
     }
     key[i] = gb;
     pev[i] = ch.event;
-    if (key_nh) key_nh[i] = static_cast<int32_t>((static_cast<uint32_t>(gb) << (sub_bits * D)) | code);
+    v[D] = w[i];
+    v[D + 1] = __int_as_float(morton ? static_cast<int32_t>((static_cast<uint32_t>(gb) << (sub_bits * D)) | code) : 0);
+#pragma unroll
+    for (int k = D + 2; k < S; ++k) v[k] = 0.f;
+    float4* out = reinterpret_cast<float4*>(packed + static_cast<int64_t>(i) * S);
+#pragma unroll
+    for (int k = 0; k < S / 4; ++k) out[k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
   }
 }
 
@@ -415,19 +428,31 @@ __global__ __launch_bounds__(kThreads) void k_radix_scatter(const int32_t* keys_
   }
 }
 
-// Tile-ordered SoA copy: xs[d][slot] = x[d][perm[slot]] (the coalesced tile array).
-__global__ __launch_bounds__(kThreads) void k_gather(Coords c, const float* w, const int32_t* perm,
-                                                      int32_t n, int D, int64_t ld, float* xs,
-                                                      float* ws, float* hw, float h,
-                                                      const int32_t* key_nh, int32_t* key_nh_s) {
+// Tile-ordered SoA copy: xs[d][slot] = x[d][perm[slot]] (the coalesced tile
+// array), read from the packed copy: one random 16-byte read per 4 values.
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_gather(const float* packed, const int32_t* perm, int32_t n,
+                                                      int64_t ld, float* xs, float* ws, float* hw, float h,
+                                                      int32_t* key_nh_s) {
+  constexpr int S = packed_stride<D>();
   const int s = logical_block(1) * kThreads + threadIdx.x;
   if (s >= n) return;
   const int32_t id = perm[s];
-  for (int d = 0; d < D; ++d) xs[d * ld + s] = c.p[d][id];
-  const float wv = w[id];
-  ws[s] = wv;
-  hw[s] = h * wv;  // FlatKernel term k*w with k = h, precomputed once per point
-  if (key_nh) key_nh_s[s] = key_nh[id];  // (tile, Morton code) key, in tile order
+  const float4* in = reinterpret_cast<const float4*>(packed + static_cast<int64_t>(id) * S);
+  float v[S];
+#pragma unroll
+  for (int k = 0; k < S / 4; ++k) {
+    const float4 q = in[k];
+    v[4 * k] = q.x;
+    v[4 * k + 1] = q.y;
+    v[4 * k + 2] = q.z;
+    v[4 * k + 3] = q.w;
+  }
+#pragma unroll
+  for (int d = 0; d < D; ++d) xs[d * ld + s] = v[d];
+  ws[s] = v[D];
+  hw[s] = h * v[D];  // FlatKernel term k*w with k = h, precomputed once per point
+  if (key_nh_s) key_nh_s[s] = __float_as_int(v[D + 1]);  // (tile, Morton code) key, in tile order
 }
 
 // Morton-ordered SoA copy (tile-major, Morton code inside the tile): the query
@@ -1893,6 +1918,7 @@ struct clue_plan_s {
   DevBuf events, chunks, pev, sc, cap_used, rank_base;
   // Morton-ordered layout and the block boxes of the nearest-higher search
   DevBuf btile;
+  DevBuf packed;  // per point: coordinates, weight, Morton key (k_bin -> k_gather)
   DevBuf kn, kn2, pn, pn2, mid, xn, wn, hk, bcnt, blk_off, bbox_lo, bbox_hi, tbox_lo, tbox_hi;
   DevBuf bmaxk, tmaxk;  // largest rank key of each block / tile
   DevBuf qk, qk2, qv, qv2;  // more than three dimensions: sort of the queries by box size
@@ -1931,7 +1957,7 @@ struct clue_plan_s {
                    &cap_used, &rank_base, &kn,   &kn2,     &pn,      &pn2,      &mid,     &xn,
                    &wn,      &hk,       &bcnt,    &blk_off, &bbox_lo,  &bbox_hi, &tbox_lo,
                    &tbox_hi, &bmaxk, &tmaxk, &tnf, &qk, &qk2, &qv, &qv2, &pending, &pending_flag, &qcost, &qorder, &mkey,  &mkey2,
-                   &mval,    &mval2,  &btile};
+                   &mval,    &mval2,  &btile, &packed};
     return k < static_cast<int>(sizeof(b) / sizeof(b[0])) ? b[k] : nullptr;
   }
 };
@@ -1968,6 +1994,7 @@ clue_status reserve(clue_plan_s* p, int32_t D, int32_t n, int32_t T, int32_t n_e
   const size_t TT = static_cast<size_t>(T) + 1;
   const size_t E = static_cast<size_t>(n_events);
   CLUE_HIP(p->key.ensure(N * 4));
+  CLUE_HIP(p->packed.ensure(N * static_cast<size_t>((D + 2 + 3) / 4 * 4) * 4));
   CLUE_HIP(p->key2.ensure(N * 4));
   CLUE_HIP(p->perm.ensure(N * 4));
   CLUE_HIP(p->perm2.ensure(N * 4));
@@ -2051,13 +2078,21 @@ clue_status scan_exclusive(const int32_t* in, int32_t* out, int64_t n, int32_t* 
 }
 
 template <int D>
-clue_status launch_bin(const Coords& c, int32_t n_chunks, clue_plan_s* p, const Wrap& wrap,
+clue_status launch_bin(const Coords& c, const float* weights, int32_t n_chunks, clue_plan_s* p, const Wrap& wrap,
                        int sub_bits, hipStream_t st) {
-  k_bin<D><<<n_chunks, kThreads, 0, st>>>(
-      c, p->chunks.as<Chunk>(), p->events.as<EventDesc>(), wrap, p->key.as<int32_t>(),
-      sub_bits >= 0 ? p->kn.as<int32_t>() : nullptr, sub_bits < 0 ? 0 : sub_bits,
-      p->pev.as<int32_t>());
+  k_bin<D><<<n_chunks, kThreads, 0, st>>>(c, weights, p->chunks.as<Chunk>(), p->events.as<EventDesc>(), wrap,
+                                          p->key.as<int32_t>(), sub_bits >= 0 ? 1 : 0, sub_bits < 0 ? 0 : sub_bits,
+                                          p->pev.as<int32_t>(), p->packed.as<float>());
   CLUE_LAUNCH_CHECK("bin");
+  return CLUE_OK;
+}
+
+template <int D>
+clue_status launch_gather(clue_plan_s* p, const int32_t* perm, int32_t n, float h, bool blocks, hipStream_t st) {
+  k_gather<D><<<div_up(n, kThreads), kThreads, 0, st>>>(p->packed.as<float>(), perm, n, p->ld, p->xs.as<float>(),
+                                                        p->ws.as<float>(), p->hw.as<float>(), h,
+                                                        blocks ? p->kn2.as<int32_t>() : nullptr);
+  CLUE_LAUNCH_CHECK("gather");
   return CLUE_OK;
 }
 
@@ -2509,7 +2544,7 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
   // ---- tile ids + histogram (+ Morton keys, point -> event)
   CLUE_MARK(1);
   rc = dispatch_dim(D, [&](auto dc_) {
-    return launch_bin<decltype(dc_)::value>(c, n_chunks, p, wrap, sub_bits, st);
+    return launch_bin<decltype(dc_)::value>(c, weights, n_chunks, p, wrap, sub_bits, st);
   });
   if (rc != CLUE_OK) return rc;
 
@@ -2529,11 +2564,10 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
 
   // ---- tile-ordered SoA (+ Morton-ordered SoA and block boxes)
   const float h = kd.type == CLUE_KERNEL_FLAT ? kd.params[0] : 0.f;
-  k_gather<<<div_up(n, kThreads), kThreads, 0, st>>>(c, weights, perm, n, D, p->ld,
-                                                     p->xs.as<float>(), p->ws.as<float>(),
-                                                     p->hw.as<float>(), h, blocks ? p->kn.as<int32_t>() : nullptr,
-                                                     p->kn2.as<int32_t>());
-  CLUE_LAUNCH_CHECK("gather");
+  rc = dispatch_dim(D, [&](auto dc_) {
+    return launch_gather<decltype(dc_)::value>(p, perm, n, h, blocks, st);
+  });
+  if (rc != CLUE_OK) return rc;
   k_fill_i32<<<div_up(Ttot, kThreads), kThreads, 0, st>>>(p->tnf.as<int32_t>(), 0, Ttot);
   k_tile_nonfinite<<<div_up(n, kThreads), kThreads, 0, st>>>(p->ws.as<float>(), sorted_keys, n,
                                                               p->tnf.as<int32_t>());
