@@ -260,7 +260,8 @@ __device__ __forceinline__ EvGrid<D> load_event(const EventDesc* events, int e, 
 }
 
 // ===========================================================================
-// Tile id per point + histogram (K1 + K2 fused).
+// Tile id per point (getGlobalBin, TilesView.hpp:53-61), its event, its Morton
+// key inside the tile, and a packed copy of the point for the tile-order gather.
 // ===========================================================================
 // Floats per point of the packed copy written by k_bin: D coordinates, the
 // weight and the (tile, Morton code) key, rounded up to whole 16-byte vectors.
