@@ -274,6 +274,24 @@ def test_edge_cases():
     assert_same(*run_both(x, np.ones(3000, np.float32), 1.0, 2.0, 2.0, ppb=100000))
 
 
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+def test_dense_tile_and_flat_dimension(ndim):
+    # three 8k-point blobs inside one tile (two far points spread the grid:
+    # hundreds of 32-point blocks in one tile, most tiles empty), then the same
+    # points with one coordinate constant (zero extent, zero tile size there)
+    rng = np.random.default_rng(11 + ndim)
+    x = rng.normal(size=(ndim, 8000)) * 0.3
+    x[0] += 1.2 * (np.arange(8000) % 3)
+    x = x.astype(np.float32)
+    x[:, 0] = -500.0
+    x[:, 1] = 500.0
+    w = rng.uniform(0.5, 1.5, 8000).astype(np.float32)
+    assert_same(*run_both(x, w, 0.2, 3.0, 0.4))
+    if ndim > 1:
+        x[ndim - 1, :] = 7.0
+        assert_same(*run_both(x, w, 0.2, 3.0, 0.4, kernel=("gaussian", 0.0, 0.5, 1.0)))
+
 def assert_same_special(got, ref, tiles, grid, stats, delta):
     # densities may be NaN here: equal as values (NaN matches NaN), not as bits,
     # since the payload of a NaN made by inf - inf differs between host and GPU
