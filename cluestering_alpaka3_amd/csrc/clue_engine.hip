@@ -902,7 +902,7 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
 // dimensions, where one lane walks its own box and a wave lasts as long as its
 // largest box -- queries sorted by class keep a wave's boxes alike.
 template <int D>
-__global__ __launch_bounds__(kThreads) void k_box_class(const PassArgs a, float r, int32_t* key) {
+__global__ __launch_bounds__(kThreads) void k_box_class(const PassArgs a, float r, int tile_bits, int32_t* key) {
   const int s = blockIdx.x * kThreads + threadIdx.x;
   const bool valid = s < a.n;
   const int sq = valid ? s : a.n - 1;
@@ -919,7 +919,8 @@ __global__ __launch_bounds__(kThreads) void k_box_class(const PassArgs a, float 
     for (int d = 0; d < D; ++d) count *= (hi[d] >= lo[d] ? hi[d] - lo[d] + 1 : 0);
     int c = 0;
     while (c < 31 && (int64_t{1} << c) < count) ++c;
-    if (act) key[s] = c;
+    // then by the box's first tile: queries with equal boxes walk them in step
+    if (act) key[s] = tile_bits > 0 ? (c << tile_bits) | tile_index<D>(lo, g) : c;
   });
 }
 
@@ -2102,6 +2103,7 @@ clue_status mark(clue_plan_s* p, int k, hipStream_t st);
 clue_status radix_sort(clue_plan_s* p, hipStream_t st, int32_t n, int total_bits, int32_t* k0,
                        int32_t* k1, int32_t* v0, int32_t* v1, bool keep_keys,
                        int32_t** keys_sorted, int32_t** vals_sorted);
+int bits_for(int64_t count);
 
 template <int D, int METRIC>
 clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, float dm,
@@ -2116,19 +2118,21 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
   a.qorder = nullptr;
   a.qperm = nullptr;
   // more than three dimensions: each lane walks its own box; queries sorted by
-  // box size keep a wave's boxes alike (sort of 5-bit classes, stable)
-  auto order_by_box = [&](float r) -> clue_status {
-    k_box_class<D><<<grid, kThreads, 0, st>>>(a, r, p->qk.as<int32_t>());
+  // box size class (5 bits) -- for the density pass then by the box's first
+  // tile -- keep a wave's boxes alike (a stable sort)
+  auto order_by_box = [&](float r, bool by_tile) -> clue_status {
+    const int tile_bits = by_tile ? std::min(bits_for(a.ntiles), 26) : 0;
+    k_box_class<D><<<grid, kThreads, 0, st>>>(a, r, tile_bits, p->qk.as<int32_t>());
     CLUE_LAUNCH_CHECK("box_class");
     int32_t* keys = nullptr;
     int32_t* vals = nullptr;
-    const clue_status r2 = radix_sort(p, st, a.n, 5, p->qk.as<int32_t>(), p->qk2.as<int32_t>(), p->qv.as<int32_t>(),
-                                      p->qv2.as<int32_t>(), false, &keys, &vals);
+    const clue_status r2 = radix_sort(p, st, a.n, 5 + tile_bits, p->qk.as<int32_t>(), p->qk2.as<int32_t>(),
+                                      p->qv.as<int32_t>(), p->qv2.as<int32_t>(), false, &keys, &vals);
     a.qperm = vals;
     return r2;
   };
   if constexpr (!U) {
-    rc = order_by_box(dc);
+    rc = order_by_box(dc, true);
     if (rc != CLUE_OK) return rc;
   }
   // heaviest-first pays in 2D (a shorter tail); in 3D the lost spatial locality
@@ -2162,7 +2166,7 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
   CLUE_LAUNCH_CHECK("density");
   a.radius = dm;
   if constexpr (!U) {
-    rc = order_by_box(dm);
+    rc = order_by_box(dm, false);  // the first tile does not pay here (measured)
     if (rc != CLUE_OK) return rc;
   }
   a.thr = sqrt_le_bound(dm);
@@ -2640,12 +2644,12 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
     a.mshift = sub_bits * D;
     a.blk_off = p->blk_off.as<int32_t>();
     a.btile = p->btile.as<int32_t>();
-    a.ntiles = Ttot;
     a.bmaxk = p->bmaxk.as<uint64_t>();
     a.tmaxk = p->tmaxk.as<uint64_t>();
     a.pending = p->pending.as<int32_t>();
     a.pending_flag = p->pending_flag.as<int32_t>();
   }
+  a.ntiles = Ttot;
   a.xcd = p->tuning.xcd_remap;
   a.tnf = p->tnf.as<int32_t>();
   // rank keys of the densities: Morton order for the block search, tile order for the tile walk
