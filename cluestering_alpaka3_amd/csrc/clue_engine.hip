@@ -1002,11 +1002,14 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
       }
     }
   };
-  auto run = [&](auto self_c, int beg, int end, bool need) {
+  // uni: every lane on this tile (always so in the union walk), so candidates
+  // sit at wave-uniform addresses and are read through the scalar cache
+  auto run = [&](auto self_c, int beg, int end, bool need, bool uni) {
     int j = beg;
-    if constexpr (UNION) {
+    if (UNION || uni) {
       constexpr int U = D <= 3 ? 16 : 8;  // candidates loaded ahead of their use
-      for (; j + U <= end; j += U) {
+      const int ue = __builtin_amdgcn_readfirstlane(end);
+      for (j = __builtin_amdgcn_readfirstlane(beg); j + U <= ue; j += U) {
         float c[U][D], wv4[U], hv[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -1038,8 +1041,10 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
     visit_box<D, UNION>(act, lo, hi, g, as_const(a.offsets), [&](int t, int beg, int end, bool need) {
       // Only the tile holding a lane's own point needs the j == i test.
       const bool self = !UNION || __ballot(s >= beg && s < end) != 0;
-      if (self) run(std::true_type{}, beg, end, need);
-      else run(std::false_type{}, beg, end, need);
+      bool uni = UNION;
+      if constexpr (!UNION) uni = __ballot(t != __builtin_amdgcn_readfirstlane(t)) == 0;
+      if (self) run(std::true_type{}, beg, end, need, uni);
+      else run(std::false_type{}, beg, end, need, uni);
       // out-of-radius candidates with an infinite or NaN weight add 0 * w = NaN
       if (a.tnf[t] != 0) {
         for (int j = beg; j < end; ++j) {
@@ -1120,11 +1125,16 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher(const PassArgs a) {
   const EvGrid<D> g = load_event<D>(a.ev, e, a.wrap);
   int lo[D], hi[D];
   search_box<D>(q, a.radius, g, lo, hi);
-  visit_box<D, UNION>(act, lo, hi, g, as_const(a.offsets), [&](int, int beg, int end, bool need) {
+  visit_box<D, UNION>(act, lo, hi, g, as_const(a.offsets), [&](int t, int beg, int end, bool need) {
     if (!need) return;
     int j = beg;
-    if constexpr (UNION) {
-      for (; j + 4 <= end; j += 4) {
+    // every lane on one tile (always so in the union walk): candidates at
+    // wave-uniform addresses, read through the scalar cache
+    bool uni = UNION;
+    if constexpr (!UNION) uni = __ballot(t != __builtin_amdgcn_readfirstlane(t)) == 0;
+    if (uni) {
+      const int ue = __builtin_amdgcn_readfirstlane(end);
+      for (j = __builtin_amdgcn_readfirstlane(beg); j + 4 <= ue; j += 4) {
         float c[4][D];
         uint64_t kv[4];
 #pragma unroll
