@@ -34,11 +34,33 @@ def run(cmd, cwd=None):
         raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}")
 
 
-def build_engine():
-    srcs = [os.path.join(HERE, "csrc", "clue_engine.hip"), os.path.join(HERE, "csrc", "clue_server.cpp")]
+def build_engine(jobs=None):
+    """Objects compiled in parallel: the host pipeline and C ABI, the server, and the
+    templated kernels once per dimension 1..10; then one shared library."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    csrc = os.path.join(HERE, "csrc")
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    units = [(os.path.join(csrc, "clue_engine.hip"), "clue_engine.o", []),
+             (os.path.join(csrc, "clue_server.cpp"), "clue_server.o", [])]
+    # heaviest instantiations first (the block search exists up to three dimensions)
+    for d in (3, 2, 1, 10, 9, 8, 7, 6, 5, 4):
+        units.append((os.path.join(csrc, "clue_passes.hip"), f"clue_passes_d{d}.o", [f"-DCLUE_DIM={d}"]))
+    jobs = jobs or int(os.environ.get("CLUE_BUILD_JOBS", "0")) or min(8, os.cpu_count() or 1)
+
+    def compile_one(u):
+        src, obj, extra = u
+        out = os.path.join(objdir, obj)
+        run([HIPCC, *HIP_FLAGS, "-pthread", *extra, "-c", "-o", out + ".tmp", src])
+        os.replace(out + ".tmp", out)
+        return out
+
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(compile_one, units))
     out = os.path.join(HERE, "libclue_hip.so")
     tmp = out + ".tmp"
-    run([HIPCC, *HIP_FLAGS, "-pthread", "-shared", "-o", tmp, *srcs])
+    run([HIPCC, f"--offload-arch={ARCH}", "-pthread", "-shared", "-o", tmp, *objs])
     os.replace(tmp, out)
     return out
 

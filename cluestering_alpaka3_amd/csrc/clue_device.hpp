@@ -108,7 +108,7 @@ __device__ __forceinline__ int32_t tile_bin(float coord, float mn, float mx, flo
 // the same sequence.  Written without branches: out-of-range and NaN inputs
 // are resolved by selects after a clamped evaluation.
 // ---------------------------------------------------------------------------
-__device__ __constant__ double kExp2Table[64] = {
+static __device__ __constant__ double kExp2Table[64] = {
     0x1.0000000000000p+0, 0x1.02c9a3e778061p+0, 0x1.059b0d3158574p+0, 0x1.0874518759bc8p+0,
     0x1.0b5586cf9890fp+0, 0x1.0e3ec32d3d1a2p+0, 0x1.11301d0125b51p+0, 0x1.1429aaea92de0p+0,
     0x1.172b83c7d517bp+0, 0x1.1a35beb6fcb75p+0, 0x1.1d4873168b9aap+0, 0x1.2063b88628cd6p+0,

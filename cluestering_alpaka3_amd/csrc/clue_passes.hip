@@ -1,0 +1,1034 @@
+// clue_passes.hip -- the templated kernels of one dimension D = CLUE_DIM: binning,
+// gathers, the density pass (K4, ClusteringKernels.hpp:18-107), the nearest-higher
+// searches (K5, ClusteringKernels.hpp:109-212) and the seed classification (K6,
+// ClusteringKernels.hpp:214-244).  The build compiles this file once per dimension
+// 1..10 (-DCLUE_DIM=D) so that the instantiations compile in parallel.
+#include "clue_internal.hpp"
+
+#ifndef CLUE_DIM
+#error "compile with -DCLUE_DIM=<1..10>"
+#endif
+
+namespace clue_hip {
+
+// ===========================================================================
+// Tile id per point (getGlobalBin, TilesView.hpp:53-61), its event, its Morton
+// key inside the tile, and a packed copy of the point for the tile-order gather.
+// ===========================================================================
+
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_bin(Coords c, const float* w, const Chunk* chunks,
+                                                   const EventDesc* events, Wrap wrap, int32_t* key,
+                                                   int morton, int sub_bits, int32_t* pev, float* packed) {
+  constexpr int S = packed_stride<D>();
+  const Chunk ch = chunks[blockIdx.x];
+  const EvGrid<D> g = load_event<D>(events, ch.event, wrap);
+  for (int i = ch.begin + threadIdx.x; i < ch.end; i += kThreads) {
+    int32_t gb = g.tile_base;
+    uint32_t code = 0;
+    float v[S];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const float x = c.p[d][i];
+      v[d] = x;
+      const int32_t b = tile_bin(x, g.mn[d], g.mx[d], g.ts[d], g.npd, g.wrapped[d]);
+      gb += g.stride[d] * b;
+      if (morton) {
+        // position inside the tile on a 2^sub_bits grid, interleaved into a Morton code
+        const float f = (x - g.mn[d]) / g.ts[d] - static_cast<float>(b);
+        int32_t sub = f2i(f * static_cast<float>(1 << sub_bits));
+        sub = sub < 0 ? 0 : (sub > (1 << sub_bits) - 1 ? (1 << sub_bits) - 1 : sub);
+        for (int bit = 0; bit < sub_bits; ++bit) code |= ((sub >> bit) & 1u) << (bit * D + d);
+      }
+    }
+    key[i] = gb;
+    pev[i] = ch.event;
+    v[D] = w[i];
+    v[D + 1] = __int_as_float(morton ? static_cast<int32_t>((static_cast<uint32_t>(gb) << (sub_bits * D)) | code) : 0);
+#pragma unroll
+    for (int k = D + 2; k < S; ++k) v[k] = 0.f;
+    float4* out = reinterpret_cast<float4*>(packed + static_cast<int64_t>(i) * S);
+#pragma unroll
+    for (int k = 0; k < S / 4; ++k) out[k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+  }
+}
+
+// Tile-ordered SoA copy: xs[d][slot] = x[d][perm[slot]] (the coalesced tile
+// array), read from the packed copy: one random 16-byte read per 4 values.
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_gather(const float* packed, const int32_t* perm, int32_t n,
+                                                      int64_t ld, float* xs, float* ws, float* hw, float h,
+                                                      int32_t* key_nh_s) {
+  constexpr int S = packed_stride<D>();
+  const int s = logical_block(1) * kThreads + threadIdx.x;
+  if (s >= n) return;
+  const int32_t id = perm[s];
+  const float4* in = reinterpret_cast<const float4*>(packed + static_cast<int64_t>(id) * S);
+  float v[S];
+#pragma unroll
+  for (int k = 0; k < S / 4; ++k) {
+    const float4 q = in[k];
+    v[4 * k] = q.x;
+    v[4 * k + 1] = q.y;
+    v[4 * k + 2] = q.z;
+    v[4 * k + 3] = q.w;
+  }
+#pragma unroll
+  for (int d = 0; d < D; ++d) xs[d * ld + s] = v[d];
+  ws[s] = v[D];
+  hw[s] = h * v[D];  // FlatKernel term k*w with k = h, precomputed once per point
+  if (key_nh_s) key_nh_s[s] = __float_as_int(v[D + 1]);  // (tile, Morton code) key, in tile order
+}
+
+// Size class of each tile-ordered point's search box of radius r (ceil(log2) of
+// its tile count): the launch order of the tile-walk passes in more than three
+// dimensions, where one lane walks its own box and a wave lasts as long as its
+// largest box -- queries sorted by class keep a wave's boxes alike.
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_box_class(const PassArgs a, float r, int tile_bits, int32_t* key) {
+  const int s = blockIdx.x * kThreads + threadIdx.x;
+  const bool valid = s < a.n;
+  const int sq = valid ? s : a.n - 1;
+  float q[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) q[d] = a.xs[d][sq];
+  const int my_e = a.pev[a.perm[sq]];
+  for_each_event(valid, my_e, [&](int e, bool act) {
+    const EvGrid<D> g = load_event<D>(a.ev, e, a.wrap);
+    int lo[D], hi[D];
+    search_box<D>(q, r, g, lo, hi);
+    int64_t count = 1;
+#pragma unroll
+    for (int d = 0; d < D; ++d) count *= (hi[d] >= lo[d] ? hi[d] - lo[d] + 1 : 0);
+    int c = 0;
+    while (c < 31 && (int64_t{1} << c) < count) ++c;
+    // then by the box's first tile: queries with equal boxes walk them in step
+    if (act) key[s] = tile_bits > 0 ? (c << tile_bits) | tile_index<D>(lo, g) : c;
+  });
+}
+
+// QNH: the lanes take their queries in Morton order (spatially coherent waves,
+// so the lanes of a wave share most of their search box); otherwise in tile order.
+template <int D, int METRIC, int KERNEL, bool UNION, bool QNH>
+__global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
+  // the exponential's 2^(j/64) table, read through LDS by the kernel evaluations
+  __shared__ double s_exp2[KERNEL == KFLAT ? 1 : 64];
+  if constexpr (KERNEL != KFLAT) {
+    if (threadIdx.x < 64) s_exp2[threadIdx.x] = kExp2Table[threadIdx.x];
+    __syncthreads();
+  }
+  const int qb = a.qorder ? a.qorder[blockIdx.x] : logical_block(a.xcd & 1);
+  const int pos = qb * kThreads + threadIdx.x;
+  const bool valid = pos < a.n;
+  const int pq = valid ? pos : a.n - 1;
+  const int lane = __lane_id();
+  const int wv = threadIdx.x / kWave;
+  float q[D];
+  float w_self;  // the point's own term is 1 * w_i (kernel value 1 for j == i)
+  int s;         // the query's slot in the tile-ordered candidate layout
+  int pid;       // the query's point id
+  if constexpr (QNH) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) q[d] = a.xn[d][pq];
+    w_self = a.wn[pq];
+    s = valid ? a.sn[pq] : -1;
+    pid = a.pn[pq];
+  } else {
+    const int sq = a.qperm ? a.qperm[pq] : pq;
+#pragma unroll
+    for (int d = 0; d < D; ++d) q[d] = a.xs[d][sq];
+    w_self = a.ws[sq];
+    s = valid ? sq : -1;
+    pid = a.perm[sq];
+  }
+  const int my_e = a.pev[pid];
+  cptr<float> X[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) X[d] = as_const(a.xs[d]);
+  cptr<float> W = as_const(a.ws);
+  cptr<float> HW = as_const(a.hw);
+  float rho = 0.f;
+  bool poisoned = false;
+  // The Euclidean flat-kernel loop reads the precomputed h*w_j; every other
+  // combination reads w_j and evaluates the kernel.
+  constexpr bool kUsesHW = (METRIC == MEUCLID) && (KERNEL == KFLAT);
+
+  // One candidate: rho_i += ((float)(d <= dc) * k) * w_j in the visit order
+  // (ClusteringKernels.hpp:38-44); out-of-radius terms are +0 and skipped.
+  auto eval = [&](auto self_c, int j, const float* c, float wj, float hwj) {
+    constexpr bool SELF = decltype(self_c)::value;  // may this tile hold the lane's own point?
+    if constexpr (METRIC == MEUCLID) {
+      const float s2 = squared_distance<D>(q, c);
+      if constexpr (KERNEL == KFLAT) {
+        float t = hwj;
+        if constexpr (SELF) t = (j == s) ? w_self : hwj;
+        if (s2 <= a.thr) rho += t;
+      } else {
+        if (s2 <= a.thr) {
+          float k = 1.f;
+          if (j != s) {
+            if (a.exp_nonpos && s2 >= 0x1p-96f) {
+              k = kernel_value<KERNEL, true>(sqrt_rn_normal(s2), a.kp[0], a.kp[1], a.kp[2], a.kden_rcp,
+                                             s_exp2);
+            } else {
+              k = kernel_value<KERNEL>(__builtin_sqrtf(s2), a.kp[0], a.kp[1], a.kp[2], a.kden_rcp);
+            }
+          }
+          rho += k * wj;
+        }
+      }
+    } else {
+      const float dist = metric_distance<D, METRIC>(q, c, a.mp);
+      if (dist <= a.radius) {
+        const float k = (j == s) ? 1.f : kernel_value<KERNEL>(dist, a.kp[0], a.kp[1], a.kp[2], a.kden_rcp);
+        rho += k * wj;
+      }
+    }
+  };
+  // uni: every lane on this tile (always so in the union walk), so candidates
+  // sit at wave-uniform addresses and are read through the scalar cache
+  auto run = [&](auto self_c, int beg, int end, bool need, bool uni) {
+    int j = beg;
+    if (UNION || uni) {
+      constexpr int U = D <= 3 ? 16 : 8;  // candidates loaded ahead of their use
+      const int ue = __builtin_amdgcn_readfirstlane(end);
+      for (j = __builtin_amdgcn_readfirstlane(beg); j + U <= ue; j += U) {
+        float c[U][D], wv4[U], hv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+          for (int d = 0; d < D; ++d) c[u][d] = X[d][j + u];
+          if constexpr (kUsesHW) hv[u] = HW[j + u];
+          else wv4[u] = W[j + u];
+        }
+        if (need) {
+#pragma unroll
+          for (int u = 0; u < U; ++u) eval(self_c, j + u, c[u], wv4[u], hv[u]);
+        }
+      }
+    }
+    for (; j < end; ++j) {
+      float c[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) c[d] = X[d][j];
+      float wj = 0.f, hwj = 0.f;
+      if constexpr (kUsesHW) hwj = HW[j];
+      else wj = W[j];
+      if (need) eval(self_c, j, c, wj, hwj);
+    }
+  };
+  for_each_event(valid, my_e, [&](int e, bool act) {
+    const EvGrid<D> g = load_event<D>(a.ev, e, a.wrap);
+    int lo[D], hi[D];
+    search_box<D>(q, a.radius, g, lo, hi);
+    visit_box<D, UNION>(act, lo, hi, g, as_const(a.offsets), [&](int t, int beg, int end, bool need) {
+      // Only the tile holding a lane's own point needs the j == i test.
+      const bool self = !UNION || __ballot(s >= beg && s < end) != 0;
+      bool uni = UNION;
+      if constexpr (!UNION) uni = __ballot(t != __builtin_amdgcn_readfirstlane(t)) == 0;
+      if (self) run(std::true_type{}, beg, end, need, uni);
+      else run(std::false_type{}, beg, end, need, uni);
+      // out-of-radius candidates with an infinite or NaN weight add 0 * w = NaN
+      if (a.tnf[t] != 0) {
+        for (int j = beg; j < end; ++j) {
+          const float wj = W[j];
+          if (need && !__builtin_isfinite(wj)) {
+            float c[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) c[d] = X[d][j];
+            bool in;
+            if constexpr (METRIC == MEUCLID) in = squared_distance<D>(q, c) <= a.thr;
+            else in = metric_distance<D, METRIC>(q, c, a.mp) <= a.radius;
+            poisoned = poisoned || !in;
+          }
+        }
+      }
+    });
+  });
+  if (poisoned) rho = __builtin_nanf("");
+  if (valid) {
+    a.rho_s_out[s] = rho;
+    if constexpr (QNH) {
+      const int32_t id = a.pn[pos];
+      a.hk_out[pos] = rank_key(rho, id);
+      a.rho_out[id] = rho;
+    } else {
+      const int32_t id = a.perm[s];
+      a.rho_out[id] = rho;
+      a.hk_out[s] = rank_key(rho, id);  // tile order: the tile walk's candidates
+    }
+  }
+}
+
+template <int D, int METRIC, bool UNION>
+__global__ __launch_bounds__(kThreads) void k_nearest_higher(const PassArgs a) {
+  const int s = logical_block((a.xcd >> 1) & 1) * kThreads + threadIdx.x;
+  const bool valid = s < a.n;
+  const int sq = a.qperm ? a.qperm[valid ? s : a.n - 1] : (valid ? s : a.n - 1);
+  float q[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) q[d] = a.xs[d][sq];
+  const uint64_t key_i = a.hk[sq];
+  const uint64_t thr_i = higher_threshold(key_i);
+  const int32_t id_i = static_cast<int32_t>(key_i & 0xffffffffu);
+  const int my_e = a.pev[id_i];
+  cptr<float> X[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) X[d] = as_const(a.xs[d]);
+  cptr<uint64_t> HK = as_const(a.hk);
+  float delta = FLT_MAX;
+  int32_t nh = -1;
+  // updates need s2 <= bound = min(thr(dm), max{s : sqrtf(s) < delta}): the float
+  // below the smallest squared distance whose sqrtf is delta
+  float bound = a.thr;
+  // One candidate (ClusteringKernels.hpp:129-144): j is "higher" if rho_j > rho_i,
+  // or equal, positive and with a larger id (one comparison of rank keys); the
+  // first strict minimum of the distance in the visit order wins.
+  auto eval = [&](const float* c, uint64_t key_j) {
+    const bool higher = key_j > thr_i;
+    if constexpr (METRIC == MEUCLID) {
+      const float s2 = squared_distance<D>(q, c);
+      if (higher && s2 <= bound) {
+        delta = __builtin_sqrtf(s2);
+        nh = static_cast<int32_t>(key_j & 0xffffffffu);
+        float lo2, hi2;
+        sqrt_class(delta, lo2, hi2);
+        const float lim = lo2 > 0.f ? __uint_as_float(__float_as_uint(lo2) - 1u) : -1.f;
+        bound = lim < a.thr ? lim : a.thr;
+      }
+    } else {
+      const float dist = metric_distance<D, METRIC>(q, c, a.mp);
+      if (higher && dist <= a.radius && dist < delta) {
+        delta = dist;
+        nh = static_cast<int32_t>(key_j & 0xffffffffu);
+      }
+    }
+  };
+  for_each_event(valid, my_e, [&](int e, bool act) {
+  const EvGrid<D> g = load_event<D>(a.ev, e, a.wrap);
+  int lo[D], hi[D];
+  search_box<D>(q, a.radius, g, lo, hi);
+  visit_box<D, UNION>(act, lo, hi, g, as_const(a.offsets), [&](int t, int beg, int end, bool need) {
+    if (!need) return;
+    int j = beg;
+    // every lane on one tile (always so in the union walk): candidates at
+    // wave-uniform addresses, read through the scalar cache
+    bool uni = UNION;
+    if constexpr (!UNION) uni = __ballot(t != __builtin_amdgcn_readfirstlane(t)) == 0;
+    if (uni) {
+      const int ue = __builtin_amdgcn_readfirstlane(end);
+      for (j = __builtin_amdgcn_readfirstlane(beg); j + 4 <= ue; j += 4) {
+        float c[4][D];
+        uint64_t kv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+#pragma unroll
+          for (int d = 0; d < D; ++d) c[u][d] = X[d][j + u];
+          kv[u] = HK[j + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) eval(c[u], kv[u]);
+      }
+    }
+    for (; j < end; ++j) {
+      float c[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) c[d] = X[d][j];
+      eval(c, HK[j]);
+    }
+  });
+  });
+  if (valid) {
+    a.nh_out[id_i] = nh;
+    a.delta_out[id_i] = delta;
+    if (nh == -1) atomicAdd(&a.sc[my_e], 1);
+  }
+}
+
+// Squared distance from q to the nearest point of a box, with the fold order of
+// squared_distance.  Rounding is monotone, so for every point c of the box
+// lower_bound2 <= squared_distance(q, c) exactly: a box whose bound exceeds a
+// lane's threshold holds no candidate that lane can accept.
+template <int D>
+__device__ __forceinline__ float lower_bound2(const float* q, const float* lo, const float* hi) {
+  float s = 0.f;
+#pragma unroll
+  for (int d = D - 1; d >= 0; --d) {
+    float g = lo[d] - q[d];
+    const float g2 = q[d] - hi[d];
+    g = g2 > g ? g2 : g;
+    g = g > 0.f ? g : 0.f;
+    const float t = g * g;
+    s = (d == D - 1) ? t : t + s;
+  }
+  return s;
+}
+
+// Squared distance from q to the farthest point of a box, with the fold order of
+// squared_distance: an upper bound of squared_distance(q, c) for every c in the box.
+template <int D>
+__device__ __forceinline__ float upper_bound2(const float* q, const float* lo, const float* hi) {
+  float s = 0.f;
+#pragma unroll
+  for (int d = D - 1; d >= 0; --d) {
+    const float g1 = q[d] - lo[d];
+    const float g2 = hi[d] - q[d];
+    const float g = g1 > g2 ? g1 : g2;
+    const float t = g * g;
+    s = (d == D - 1) ? t : t + s;
+  }
+  return s;
+}
+
+// Nearest-higher (K5) as a block search over the Morton-ordered layout
+// (Euclidean metric, no wrapped dimension).  The result is the serial one: the
+// candidate minimising (distance, visit position) where the visit position is
+// (tile id, point id) -- ascending tile ids and ascending ids inside a tile are
+// the order of for_recursion_nearest_higher (ClusteringKernels.hpp:109-166) on a
+// grid without wrapping -- so candidates may be examined in any order, and a
+// block whose bounding box cannot beat a lane's current distance is skipped.
+//
+// This first phase (one lane per query, a wave per 64 Morton-consecutive
+// queries) limits the search to the radius thr1: the wave's own 64 points
+// first, then the blocks of its lanes' search boxes.  A lane that finds a
+// higher point within it is final (every candidate at or below its distance was
+// examined); the others are queued for the second phase (k_nh_pending), the
+// full search within dm.
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassArgs a) {
+  const int gid = logical_block((a.xcd >> 1) & 1) * kThreads + threadIdx.x;
+  const bool valid = gid < a.n;
+  const int pos = valid ? gid : a.n - 1;
+  float q[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) q[d] = a.xn[d][pos];
+  const uint64_t key_i = a.hk[pos];
+  const uint64_t thr_i = higher_threshold(key_i);
+  const int32_t id_i = static_cast<int32_t>(key_i & 0xffffffffu);
+  const int my_e = a.pev[id_i];
+  cptr<float> X[D], TMIN[D], TMAX[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    X[d] = as_const(a.xn[d]);
+    TMIN[d] = as_const(a.tmin[d]);
+    TMAX[d] = as_const(a.tmax[d]);
+  }
+  cptr<uint64_t> HK = as_const(a.hk);
+  cptr<uint64_t> BMK = as_const(a.bmaxk);
+  cptr<uint64_t> TMK = as_const(a.tmaxk);
+  cptr<int32_t> OFF = as_const(a.offsets);
+  cptr<int32_t> BOFF = as_const(a.blk_off);
+  // a block or tile whose largest rank key is at most a lane's threshold holds
+  // no point higher than that lane's; at most the wave's least threshold, none
+  // higher than any of its lanes'
+  const uint64_t wthr = uniform(wave_min_u64(valid ? thr_i : ~uint64_t{0}));
+  float delta = FLT_MAX;
+  int32_t nh = -1;
+  int64_t best = INT64_MAX;
+  // The best candidate so far is at distance delta; the squared distances with
+  // sqrtf equal to delta are [lo2, hi2].  A candidate replaces it when it is
+  // nearer (s2 < lo2) or as near and earlier in the visit order (key < best), so
+  // candidates need s2 <= bound = min(search radius, hi2).
+  const float cap = a.thr1 < a.thr ? a.thr1 : a.thr;
+  float bound = cap;
+  float lo2 = __builtin_inff();
+  // the wave's query box, for the block prefilter
+  float qlo[D], qhi[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    qlo[d] = wave_min_f32(valid ? q[d] : __builtin_inff());
+    qhi[d] = wave_max_f32(valid ? q[d] : -__builtin_inff());
+  }
+
+  auto cand = [&](int t, const float* c, uint64_t key_j) {
+    const float s2 = squared_distance<D>(q, c);
+    const int64_t key = (static_cast<int64_t>(t) << 32) | static_cast<uint32_t>(key_j);
+    if (key_j > thr_i && s2 <= bound && (s2 < lo2 || key < best)) {
+      delta = __builtin_sqrtf(s2);
+      nh = static_cast<int32_t>(key_j & 0xffffffffu);
+      best = key;
+      float hi2;
+      sqrt_class(delta, lo2, hi2);
+      bound = hi2 < cap ? hi2 : cap;
+    }
+  };
+  // candidates [beg, end) of tile t, wave-uniform: eight at a time, all loads
+  // issued before the first use
+  auto scan = [&](int t, int beg, int end) {
+    constexpr int U = 8;
+    int j = beg;
+    for (; j + U <= end; j += U) {
+      float c[U][D];
+      uint64_t k[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) c[u][d] = X[d][j + u];
+        k[u] = HK[j + u];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) cand(t, c[u], k[u]);
+    }
+    for (; j < end; ++j) {
+      float c[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) c[d] = X[d][j];
+      cand(t, c, HK[j]);
+    }
+  };
+
+  for_each_event(valid, my_e, [&](int e, bool act) {
+    const EvGrid<D> g = load_event<D>(a.ev, e, a.wrap);
+    // only candidates within thr1: a box of a slightly larger radius (1/1024 of
+    // it, plus 2^-20 of the coordinate for the rounding of q -+ r) holds every
+    // such candidate and lies inside the reference's box of dm
+    int lo[D], hi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      float r = a.box1 + __builtin_fabsf(q[d]) * 0x1p-20f;
+      r = r < a.radius ? r : a.radius;
+      lo[d] = tile_bin(q[d] - r, g.mn[d], g.mx[d], g.ts[d], g.npd, false);
+      hi[d] = tile_bin(q[d] + r, g.mn[d], g.mx[d], g.ts[d], g.npd, false);
+    }
+    auto in_box = [&](int t) {
+      const int tl = t - g.tile_base;
+      bool in = act && tl >= 0 && tl < g.ntiles;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const int b = (tl / g.stride[d]) % g.npd;
+        in = in && b >= lo[d] && b <= hi[d];
+      }
+      return in;
+    };
+
+    // the wave's own points: same tile, neighbouring Morton codes
+    int t0 = -1;
+    {
+      const int base = uniform(gid & ~(kWave - 1));
+      if (base < a.n) {
+        t0 = uniform(as_const(a.kn)[base] >> a.mshift);
+        const int beg = uniform(max(base, OFF[t0]));
+        const int end = uniform(min(min(base + kWave, OFF[t0 + 1]), a.n));
+        const bool need0 = in_box(t0);
+        if (__ballot(need0) != 0 && need0) scan(t0, beg, end);
+      }
+    }
+
+    int ulo[D], uhi[D];
+    bool any = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      ulo[d] = uniform(wave_min_i32(act ? lo[d] : INT_MAX));
+      uhi[d] = uniform(wave_max_i32(act ? hi[d] : INT_MIN));
+      any = any && ulo[d] <= uhi[d];
+    }
+    if (!any) return;
+    const int lane = __lane_id();
+    // one tile of the union: the blocks whose box and largest key can still
+    // improve a lane that needs the tile (need: the tile is in the lane's box)
+    auto tile_pass = [&](int t, bool need) {
+      float tl[D], th[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        tl[d] = TMIN[d][t];
+        th[d] = TMAX[d][t];
+      }
+      need = need && TMK[t] > thr_i && lower_bound2<D>(q, tl, th) <= bound;
+      const float maxb = wave_max_f32(need ? bound : -1.f);
+      if (maxb >= 0.f) {
+        const int k0 = BOFF[t], k1 = BOFF[t + 1];
+        const int off0 = OFF[t], off1 = OFF[t + 1];
+        for (int kb = k0; kb < k1; kb += kWave) {
+          // one block per lane against the wave's query box
+          const int k = kb + lane;
+          const bool inb = k < k1;
+          float bl[D], bh[D];
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            bl[d] = inb ? a.bmin[d][k] : __builtin_inff();
+            bh[d] = inb ? a.bmax[d][k] : -__builtin_inff();
+          }
+          const uint64_t bk = inb ? BMK[k] : 0;
+          float lbw = 0.f;
+#pragma unroll
+          for (int d = D - 1; d >= 0; --d) {
+            float gg = bl[d] - qhi[d];
+            const float g2 = qlo[d] - bh[d];
+            gg = g2 > gg ? g2 : gg;
+            gg = gg > 0.f ? gg : 0.f;
+            const float tt = gg * gg;
+            lbw = (d == D - 1) ? tt : tt + lbw;
+          }
+          uint64_t m = __ballot(inb && lbw <= maxb && bk > wthr);
+          while (m) {
+            const int kl = __builtin_ctzll(m);
+            m &= m - 1;
+            float ul[D], uh[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+              ul[d] = readlane_f32(bl[d], kl);
+              uh[d] = readlane_f32(bh[d], kl);
+            }
+            const bool nk = need && readlane_u64(bk, kl) > thr_i && lower_bound2<D>(q, ul, uh) <= bound;
+            if (__ballot(nk) != 0) {
+              const int start = off0 + (kb + kl - k0) * kBlock;
+              const int end = min(start + kBlock, off1);
+              if (nk) scan(t, start, end);
+            }
+          }
+        }
+      }
+    };
+    // the wave's own tile first (its blocks hold the nearest candidates, so the
+    // lanes' bounds shrink before the neighbouring tiles are walked), then the
+    // rest of the union's tiles in lexicographic order
+    if (t0 >= 0) {
+      const bool need0 = in_box(t0);
+      if (__ballot(need0) != 0) tile_pass(t0, need0);
+    }
+    int b[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) b[d] = ulo[d];
+    while (true) {
+      bool need = act;
+#pragma unroll
+      for (int d = 0; d < D; ++d) need = need && b[d] >= lo[d] && b[d] <= hi[d];
+      if (__ballot(need) != 0) {
+        const int t = uniform(tile_index<D>(b, g));
+        if (t != t0) tile_pass(t, need);
+      }
+      int d = D - 1;
+      for (; d >= 0; --d) {
+        if (++b[d] <= uhi[d]) break;
+        b[d] = ulo[d];
+      }
+      if (d < 0) break;
+    }
+  });
+
+  // unresolved positions are compacted in Morton order (flag -> scan -> scatter)
+  // so that the full search runs on spatially coherent waves
+  if (valid) a.pending_flag[pos] = nh == -1 ? 1 : 0;
+  if (valid && nh != -1) {
+    a.nh_out[id_i] = nh;
+    a.delta_out[id_i] = delta;
+  }
+}
+
+// Second nearest-higher phase: a group of G lanes per unresolved point
+// (pending[0..npend), positions in the Morton layout), kWave / G points per
+// wave.  The group's lanes split the candidates of the point's search box
+// (tiles, then their 32-point Morton blocks, pruned by their boxes against the
+// group's best bound), each keeps its best (distance, tile id, point id), and a
+// group reduction takes the least:
+// the serial backend's first strict minimum on a grid without wrapping.  The
+// grid is persistent: every wave strides over the pending list and leaves when
+// it is exhausted.
+template <int G>
+__device__ __forceinline__ float group_min_f32(float v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) {
+    const float o = __shfl_xor(v, off);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+template <int D, int G>
+__global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
+  static_assert(kWave % G == 0, "groups must tile the wave");
+  const int npend = uniform(*a.pending_count);
+  const int lane = __lane_id();
+  const int gl = lane % G;
+  const int wave = static_cast<int>((blockIdx.x * kThreads + threadIdx.x) / kWave);
+  const int per_wave = kWave / G;
+  const int stride = gridDim.x * (kThreads / kWave) * per_wave;
+  const int grp_shift = (lane / G) * G;  // the group's bits in a wave ballot
+  for (int base = uniform(wave * per_wave); base < npend; base += stride) {
+    const int w = base + lane / G;
+    const bool active = w < npend;
+    const int pos = a.pending[active ? w : npend - 1];
+    float q[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) q[d] = a.xn[d][pos];
+    const uint64_t key_i = a.hk[pos];
+    const uint64_t thr_i = higher_threshold(key_i);
+    const int32_t id_i = static_cast<int32_t>(key_i & 0xffffffffu);
+    const int e = a.pev[id_i];
+    const EventDesc* ev = a.ev + e;
+    EvGrid<D> g;
+    g.npd = ev->npd;
+    g.tile_base = ev->tile_base;
+    g.ntiles = ev->ntiles;
+    g.begin = ev->begin;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      g.stride[d] = ev->stride[d];
+      g.mn[d] = ev->mn[d];
+      g.mx[d] = ev->mx[d];
+      g.ts[d] = ev->ts[d];
+      g.wrapped[d] = a.wrap.w[d] != 0;
+    }
+    int lo[D], hi[D];
+    search_box<D>(q, a.radius, g, lo, hi);
+    float delta = FLT_MAX;
+    int64_t best = INT64_MAX;
+    // candidates need s2 <= bound = min(thr(dm), hi2), and replace the lane's best
+    // when s2 < lo2 or, as near, key < best ([lo2, hi2]: the sqrtf class of delta)
+    float bound = a.thr;
+    float lo2 = __builtin_inff();
+    float group_bound = a.thr;
+    // Levels of growing radius: a level examines the candidates within its
+    // radius that lie outside the previous one (blocks entirely inside were
+    // examined before and hold no higher point); the first level that finds a
+    // higher point is the last, since everything nearer has then been examined.
+    float prev = a.lvl_prev0;
+    bool found = false;
+    // The box's tiles, one per lane of the group when they fit (their boxes and
+    // ranges are then loaded once, together, and shuffled to the group at each
+    // level); otherwise each level walks them and loads them one by one.
+    int dims[D];
+    int nt = 1;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      dims[d] = hi[d] - lo[d] + 1;
+      nt *= dims[d];
+    }
+    const bool cached = nt <= G;
+    auto tile_of = [&](int ti) {
+      int bb[D];
+      int rem = ti;
+#pragma unroll
+      for (int d = D - 1; d >= 0; --d) {
+        bb[d] = lo[d] + rem % dims[d];
+        rem /= dims[d];
+      }
+      return tile_index<D>(bb, g);
+    };
+    int my_t = 0, my_k0 = 0, my_k1 = 0, my_o0 = 0, my_o1 = 0;
+    uint64_t my_tk = 0;
+    float my_tl[D], my_th[D];
+    if (cached) {
+      const bool tv = active && gl < nt;
+      my_t = tv ? tile_of(gl) : 0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        my_tl[d] = tv ? a.tmin[d][my_t] : __builtin_inff();
+        my_th[d] = tv ? a.tmax[d][my_t] : -__builtin_inff();
+      }
+      my_k0 = tv ? a.blk_off[my_t] : 0;
+      my_k1 = tv ? a.blk_off[my_t + 1] : 0;
+      my_o0 = tv ? a.offsets[my_t] : 0;
+      my_o1 = tv ? a.offsets[my_t + 1] : 0;
+      my_tk = tv ? a.tmaxk[my_t] : 0;
+    }
+    for (int lv = 0; lv < a.nlvl; ++lv) {
+      const float cap = a.lvl[lv];
+      int ti = 0;
+      bool more = active && !found;
+      while (__ballot(more) != 0) {
+        if (more) {  // uniform within a group
+          int t, k0, k1, off0, off1;
+          uint64_t tk;
+          float tl[D], th[D];
+          if (cached) {
+            const int src = grp_shift + ti;
+            t = __shfl(my_t, src);
+            tk = (static_cast<uint64_t>(__shfl(static_cast<uint32_t>(my_tk >> 32), src)) << 32) |
+                 __shfl(static_cast<uint32_t>(my_tk), src);
+            k0 = __shfl(my_k0, src);
+            k1 = __shfl(my_k1, src);
+            off0 = __shfl(my_o0, src);
+            off1 = __shfl(my_o1, src);
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+              tl[d] = __shfl(my_tl[d], src);
+              th[d] = __shfl(my_th[d], src);
+            }
+          } else {
+            t = tile_of(ti);
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+              tl[d] = a.tmin[d][t];
+              th[d] = a.tmax[d][t];
+            }
+            k0 = a.blk_off[t];
+            k1 = a.blk_off[t + 1];
+            off0 = a.offsets[t];
+            off1 = a.offsets[t + 1];
+            tk = a.tmaxk[t];
+          }
+          const float lim_t = cap < group_bound ? cap : group_bound;
+          // tiles and blocks without a point higher than this one are skipped
+          if (tk > thr_i && lower_bound2<D>(q, tl, th) <= lim_t && upper_bound2<D>(q, tl, th) > prev) {
+            // the tile's Morton blocks: G at a time against the group's bound,
+            // then the points of each block that can still hold a better candidate
+            for (int kb = k0; kb < k1; kb += G) {
+              const int k = kb + gl;
+              const bool inb = k < k1;
+              float bl[D], bh[D];
+#pragma unroll
+              for (int d = 0; d < D; ++d) {
+                bl[d] = inb ? a.bmin[d][k] : __builtin_inff();
+                bh[d] = inb ? a.bmax[d][k] : -__builtin_inff();
+              }
+              const uint64_t bk = inb ? a.bmaxk[k] : 0;
+              const float lim_b = cap < group_bound ? cap : group_bound;
+              const bool pass = inb && bk > thr_i && lower_bound2<D>(q, bl, bh) <= lim_b &&
+                                upper_bound2<D>(q, bl, bh) > prev;
+              uint64_t gm = __ballot(pass) >> grp_shift;
+              if constexpr (G < kWave) gm &= (uint64_t{1} << G) - 1;
+              while (gm) {
+                const int kl = __builtin_ctzll(gm);
+                gm &= gm - 1;
+                const int start = off0 + (kb + kl - k0) * kBlock;
+                const int end = min(start + kBlock, off1);
+                const float lane_cap = bound < cap ? bound : cap;
+                for (int j = start + gl; j < end; j += G) {
+                  float c[D];
+#pragma unroll
+                  for (int d = 0; d < D; ++d) c[d] = a.xn[d][j];
+                  const uint64_t key_j = a.hk[j];
+                  const float s2 = squared_distance<D>(q, c);
+                  const int64_t key = (static_cast<int64_t>(t) << 32) | static_cast<uint32_t>(key_j);
+                  if (key_j > thr_i && s2 <= lane_cap && s2 <= bound && (s2 < lo2 || key < best)) {
+                    delta = __builtin_sqrtf(s2);
+                    best = key;
+                    float hi2;
+                    sqrt_class(delta, lo2, hi2);
+                    bound = hi2 < a.thr ? hi2 : a.thr;
+                  }
+                }
+              }
+              group_bound = group_min_f32<G>(bound);  // the point's best so far over its lanes
+            }
+          }
+          more = ++ti < nt;
+        }
+      }
+      uint64_t fm = __ballot(best != INT64_MAX) >> grp_shift;
+      if constexpr (G < kWave) fm &= (uint64_t{1} << G) - 1;
+      found = fm != 0;
+      prev = cap;
+    }
+    // lexicographic (distance, key) minimum over the group
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+      const float od = __shfl_xor(delta, off);
+      const int32_t klo = __shfl_xor(static_cast<int32_t>(best & 0xffffffff), off);
+      const int32_t khi = __shfl_xor(static_cast<int32_t>(best >> 32), off);
+      const int64_t ok = (static_cast<int64_t>(khi) << 32) | static_cast<uint32_t>(klo);
+      if (od < delta || (od == delta && ok < best)) {
+        delta = od;
+        best = ok;
+      }
+    }
+    if (active && gl == 0) {
+      const int32_t nh = best == INT64_MAX ? -1 : static_cast<int32_t>(best & 0xffffffff);
+      a.nh_out[id_i] = nh;
+      a.delta_out[id_i] = nh == -1 ? FLT_MAX : delta;
+      if (nh == -1) atomicAdd(&a.sc[e], 1);
+    }
+  }
+}
+
+template <int D, int METRIC>
+__global__ __launch_bounds__(kThreads) void k_classify(const ClassifyArgs a) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= a.n) return;
+  const int32_t nhv = a.nh[i];
+  float dist;
+  if (nhv == -1) {
+    // metric(point_i, PointsView[-1]) with the all-FLT_MAX sentinel (PointsCommon.hpp:75-81)
+    float x[D], m[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      x[d] = a.c.p[d][i];
+      m[d] = FLT_MAX;
+    }
+    dist = metric_distance<D, METRIC>(x, m, a.mp);
+  } else {
+    dist = a.delta[i];  // the nearest-higher kernel's metric(point_i, point_nh)
+  }
+  const bool seed = (dist > a.seed_dc) && (a.rho[i] >= a.rhoc);
+  a.is_seed[i] = seed ? 1 : 0;
+  a.flag[i] = seed ? 1 : 0;
+  const int32_t parent = seed ? -1 : nhv;
+  if (seed) a.nh[i] = -1;
+  a.root[i] = parent == -1 ? i : parent;
+}
+
+// ===========================================================================
+// Launches
+// ===========================================================================
+template <int D>
+clue_status launch_bin(const Coords& c, const float* weights, int32_t n_chunks, clue_plan_s* p, const Wrap& wrap,
+                       int sub_bits, hipStream_t st) {
+  k_bin<D><<<n_chunks, kThreads, 0, st>>>(c, weights, p->chunks.as<Chunk>(), p->events.as<EventDesc>(), wrap,
+                                          p->key.as<int32_t>(), sub_bits >= 0 ? 1 : 0, sub_bits < 0 ? 0 : sub_bits,
+                                          p->pev.as<int32_t>(), p->packed.as<float>());
+  CLUE_LAUNCH_CHECK("bin");
+  return CLUE_OK;
+}
+
+template <int D>
+clue_status launch_gather(clue_plan_s* p, const int32_t* perm, int32_t n, float h, bool blocks, hipStream_t st) {
+  k_gather<D><<<div_up(n, kThreads), kThreads, 0, st>>>(p->packed.as<float>(), perm, n, p->ld, p->xs.as<float>(),
+                                                        p->ws.as<float>(), p->hw.as<float>(), h,
+                                                        blocks ? p->kn2.as<int32_t>() : nullptr);
+  CLUE_LAUNCH_CHECK("gather");
+  return CLUE_OK;
+}
+
+template <int D, int METRIC>
+clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, float dm,
+                          hipStream_t st, clue_plan_s* p, bool blocks) {
+  constexpr bool U = (D <= 3);
+  const int grid = static_cast<int>(div_up(base_args.n, kThreads));
+  PassArgs a = base_args;
+  a.radius = dc;
+  a.thr = sqrt_le_bound(dc);
+  clue_status rc = mark(p, 4, st);
+  if (rc != CLUE_OK) return rc;
+  a.qorder = nullptr;
+  a.qperm = nullptr;
+  // more than three dimensions: each lane walks its own box; queries sorted by
+  // box size class (5 bits) -- for the density pass then by the box's first
+  // tile -- keep a wave's boxes alike (a stable sort)
+  auto order_by_box = [&](float r, bool by_tile) -> clue_status {
+    const int tile_bits = by_tile ? std::min(bits_for(a.ntiles), 26) : 0;
+    k_box_class<D><<<grid, kThreads, 0, st>>>(a, r, tile_bits, p->qk.as<int32_t>());
+    CLUE_LAUNCH_CHECK("box_class");
+    int32_t* keys = nullptr;
+    int32_t* vals = nullptr;
+    const clue_status r2 = radix_sort(p, st, a.n, 5 + tile_bits, p->qk.as<int32_t>(), p->qk2.as<int32_t>(),
+                                      p->qv.as<int32_t>(), p->qv2.as<int32_t>(), false, &keys, &vals);
+    a.qperm = vals;
+    return r2;
+  };
+  if constexpr (!U) {
+    rc = order_by_box(dc, true);
+    if (rc != CLUE_OK) return rc;
+  }
+  // heaviest-first pays in 2D (a shorter tail); in 3D the lost spatial locality
+  // of concurrent blocks costs more than it saves (measured), hence the default
+  const bool order = p->tuning.density_order > 0 || (p->tuning.density_order < 0 && D <= 2);
+  if (blocks && order && grid <= kOrderMax) {
+    rc = launch_density_order(p, a, grid, st);
+    if (rc != CLUE_OK) return rc;
+    a.qorder = p->qorder.as<int32_t>();
+  }
+  if (blocks) {
+    switch (kernel_type) {
+      case KFLAT: k_density<D, METRIC, KFLAT, U, true><<<grid, kThreads, 0, st>>>(a); break;
+      case KGAUSS: k_density<D, METRIC, KGAUSS, U, true><<<grid, kThreads, 0, st>>>(a); break;
+      case KEXP: k_density<D, METRIC, KEXP, U, true><<<grid, kThreads, 0, st>>>(a); break;
+      default: return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown convolutional kernel type");
+    }
+  } else {
+    switch (kernel_type) {
+      case KFLAT: k_density<D, METRIC, KFLAT, U, false><<<grid, kThreads, 0, st>>>(a); break;
+      case KGAUSS: k_density<D, METRIC, KGAUSS, U, false><<<grid, kThreads, 0, st>>>(a); break;
+      case KEXP: k_density<D, METRIC, KEXP, U, false><<<grid, kThreads, 0, st>>>(a); break;
+      default: return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown convolutional kernel type");
+    }
+  }
+  CLUE_LAUNCH_CHECK("density");
+  a.radius = dm;
+  if constexpr (!U) {
+    rc = order_by_box(dm, false);  // the first tile does not pay here (measured)
+    if (rc != CLUE_OK) return rc;
+  }
+  a.thr = sqrt_le_bound(dm);
+  rc = mark(p, 5, st);
+  if (rc != CLUE_OK) return rc;
+  if constexpr (METRIC == MEUCLID && D <= 3) {
+    if (blocks) {
+      a.thr1 = sqrt_le_bound(dm * p->tuning.nh_first_radius);
+      a.box1 = dm * p->tuning.nh_first_radius * (1.f + 1.f / 1024.f);
+      // second phase levels: twice the radius each time, dm last
+      a.nlvl = 0;
+      a.lvl_prev0 = std::min(a.thr1, a.thr);
+      float r = dm * p->tuning.nh_first_radius;
+      if (r < dm) {
+        while (a.nlvl < 7) {
+          r *= 2.f;
+          if (!(r < dm)) break;
+          a.lvl[a.nlvl++] = sqrt_le_bound(r);
+        }
+        a.lvl[a.nlvl++] = a.thr;
+      }
+      rc = launch_nh_maxkeys(p, a, st);
+      if (rc != CLUE_OK) return rc;
+      k_nearest_higher_blocks<D><<<grid, kThreads, 0, st>>>(a);
+      CLUE_LAUNCH_CHECK("nearest_higher_blocks");
+      rc = scan_exclusive(a.pending_flag, a.pending_flag, a.n, p->part.as<int32_t>(), st);
+      if (rc != CLUE_OK) return rc;
+      rc = launch_compact(a.pending_flag, a.n, a.pending, st);
+      if (rc != CLUE_OK) return rc;
+      a.pending_count = a.pending_flag + a.n;
+      // enough waves to fill the GPU; surplus waves find the pending list exhausted
+      // default grid of the persistent second phase: 8192 workgroups for one
+      // event of 2^20 points or more (2D 1M points: 0.51 -> 0.49 ms, 3D 4M: 7.43
+      // -> 7.31 ms), 2048 otherwise (the 4096-event batch: 7.08 ms against 7.23 ms
+      // with 8192; small served requests lose more than they gain)
+      const int pblocks = p->tuning.pending_blocks > 0 ? p->tuning.pending_blocks
+                                                       : (a.n_events == 1 && a.n >= (1 << 20) ? 8192 : 2048);
+      const int nblk = static_cast<int>(std::min<int64_t>(pblocks, div_up(a.n, kThreads / kWave)));
+      switch (p->tuning.pending_group) {
+        case 8: k_nh_pending<D, 8><<<nblk, kThreads, 0, st>>>(a); break;
+        case 16: k_nh_pending<D, 16><<<nblk, kThreads, 0, st>>>(a); break;
+        case 32: k_nh_pending<D, 32><<<nblk, kThreads, 0, st>>>(a); break;
+        default: k_nh_pending<D, 64><<<nblk, kThreads, 0, st>>>(a); break;
+      }
+      CLUE_LAUNCH_CHECK("nearest_higher_pending");
+      return CLUE_OK;
+    }
+  }
+  k_nearest_higher<D, METRIC, U><<<grid, kThreads, 0, st>>>(a);
+  CLUE_LAUNCH_CHECK("nearest_higher");
+  return CLUE_OK;
+}
+
+template <int D, int METRIC>
+clue_status launch_classify(const ClassifyArgs& a, hipStream_t st) {
+  k_classify<D, METRIC><<<div_up(a.n, kThreads), kThreads, 0, st>>>(a);
+  CLUE_LAUNCH_CHECK("classify");
+  return CLUE_OK;
+}
+
+template <int D>
+clue_status dispatch_metric_passes(int metric, const PassArgs& a, int kt, float dc, float dm,
+                                   hipStream_t st, clue_plan_s* p, bool blocks) {
+  switch (metric) {
+    case MEUCLID: return launch_passes<D, MEUCLID>(a, kt, dc, dm, st, p, blocks);
+    case MWEUCLID: return launch_passes<D, MWEUCLID>(a, kt, dc, dm, st, p, false);
+    case MPERIODIC: return launch_passes<D, MPERIODIC>(a, kt, dc, dm, st, p, false);
+    case MMANHATTAN: return launch_passes<D, MMANHATTAN>(a, kt, dc, dm, st, p, false);
+    case MCHEBYSHEV: return launch_passes<D, MCHEBYSHEV>(a, kt, dc, dm, st, p, false);
+    case MWCHEBYSHEV: return launch_passes<D, MWCHEBYSHEV>(a, kt, dc, dm, st, p, false);
+  }
+  return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown distance metric type");
+}
+
+template <int D>
+clue_status dispatch_metric_classify(int metric, const ClassifyArgs& a, hipStream_t st) {
+  switch (metric) {
+    case MEUCLID: return launch_classify<D, MEUCLID>(a, st);
+    case MWEUCLID: return launch_classify<D, MWEUCLID>(a, st);
+    case MPERIODIC: return launch_classify<D, MPERIODIC>(a, st);
+    case MMANHATTAN: return launch_classify<D, MMANHATTAN>(a, st);
+    case MCHEBYSHEV: return launch_classify<D, MCHEBYSHEV>(a, st);
+    case MWCHEBYSHEV: return launch_classify<D, MWCHEBYSHEV>(a, st);
+  }
+  return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown distance metric type");
+}
+
+template clue_status launch_bin<CLUE_DIM>(const Coords&, const float*, int32_t, clue_plan_s*, const Wrap&, int,
+                                          hipStream_t);
+template clue_status launch_gather<CLUE_DIM>(clue_plan_s*, const int32_t*, int32_t, float, bool, hipStream_t);
+template clue_status dispatch_metric_passes<CLUE_DIM>(int, const PassArgs&, int, float, float, hipStream_t,
+                                                      clue_plan_s*, bool);
+template clue_status dispatch_metric_classify<CLUE_DIM>(int, const ClassifyArgs&, hipStream_t);
+
+}  // namespace clue_hip
