@@ -35,7 +35,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="2d_1m_gauss")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="OpenMP threads of the CPU baseline; 0: every core of this process's affinity, at most 16 "
+                         "(a GPU's share of the host)")
+    ap.add_argument("--no-serial-full", action="store_true",
+                    help="skip the serial oracle on the whole workload (about a minute of CPU)")
     ap.add_argument("--calibration-copy-mib", type=int, default=64,
                     help="size of one device-to-device copy made before timing; its rocprofv3 FETCH_SIZE / "
                          "WRITE_SIZE calibrate the HBM byte counters (0 disables)")
@@ -274,15 +278,36 @@ def batched_bench(clue, datasets, args):
     return res
 
 
-def cpu_baseline(ps, prm, args, gpu_labels):
+def host_description(threads):
+    """CPU model, core counts and the OpenMP binding of the CPU baseline."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "os_cpu_count": os.cpu_count(), "affinity_cores": len(os.sched_getaffinity(0)),
+            "omp_threads": threads, "omp_proc_bind": os.environ.get("OMP_PROC_BIND", ""),
+            "omp_places": os.environ.get("OMP_PLACES", "")}
+
+
+def cpu_baseline(ps, prm, args, gpu):
     """The oracle (CPU restatement of the reference's serial backend) on the box's
-    host cores: serial on a spatial window (same local density), OpenMP on all.
-    Timed in its build with the reference's Release flags (fast math, vectorised);
-    the IEEE-strict build (the parity reference) is timed beside it and checks the
-    GPU's labels."""
+    host cores: serial on a spatial window (same local density) and on the whole
+    workload, OpenMP on every core of this process's CPU share (threads bound
+    close, as alpaka's CpuOmpBlocks executor runs).  Timed in its build with the
+    reference's Release flags (fast math, vectorised); the IEEE-strict build (the
+    parity reference) is timed beside it, and its results check the GPU's and
+    give the margins of the parity claim."""
+    # OpenMP reads these when the oracle library initialises its runtime
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
+    threads = args.cpu_threads or max(1, min(16, len(os.sched_getaffinity(0))))
     p = oracle.make_params(prm["dc"], prm["rhoc"], prm["dm"], kernel=prm["kernel"])
     lo = ps.coords.min(axis=1)
     hi = ps.coords.max(axis=1)
@@ -308,27 +333,50 @@ def cpu_baseline(ps, prm, args, gpu_labels):
         "strict_ieee_build": {"value": round(sub_c.shape[1] / times[False] / 1e6, 4), "unit": "Mpoints/s",
                               "seconds": round(times[False], 2)},
     }
-    threads = args.cpu_threads
+    if not args.no_serial_full:
+        t0 = time.perf_counter()
+        oracle.make_clusters(ps.coords, ps.weights, p, with_tiles=False, release=True)
+        t_full = time.perf_counter() - t0
+        out["serial_full"] = {"value": round(ps.n / t_full / 1e6, 4), "unit": "Mpoints/s", "cores": 1,
+                              "sample": f"serial oracle (Release flags), the whole workload ({ps.n} points), "
+                                        f"{t_full:.2f} s"}
     t0 = time.perf_counter()
     oracle.make_clusters_omp(ps.coords, ps.weights, p, threads, release=True)
     t_omp = time.perf_counter() - t0
     t0 = time.perf_counter()
-    r = oracle.make_clusters_omp(ps.coords, ps.weights, p, threads)
+    r = oracle.make_clusters_par(ps.coords, ps.weights, p, threads)
     t_omp_strict = time.perf_counter() - t0
     out["omp"] = {"value": round(ps.n / t_omp / 1e6, 4), "unit": "Mpoints/s", "cores": threads,
                   "sample": f"OpenMP oracle (Release flags), full workload, {t_omp:.2f} s",
                   "strict_ieee_build": {"value": round(ps.n / t_omp_strict / 1e6, 4), "unit": "Mpoints/s",
                                         "seconds": round(t_omp_strict, 2)}}
-    parity = {"labels_equal_oracle": bool(np.array_equal(gpu_labels, r.cluster)),
+    out["host"] = host_description(threads)
+
+    # parity of the GPU outputs against the strict oracle, and the margins: points
+    # within 1e-6 of the rhoc test or of a nearest-higher tie, the FP64 density
+    # error, and the oracle's libm-exp mode (the reference's host expf)
+    band = 1e-6
+    margins = oracle.parity_margins(ps.coords, ps.weights, p, r.rho, threads, band)
+    parity = {"labels_equal_oracle": bool(np.array_equal(gpu["cluster"], r.cluster)),
+              "rho_bits_equal_oracle": bool(np.array_equal(gpu["rho"].view(np.int32), r.rho.view(np.int32))),
+              "nh_equal_oracle": bool(np.array_equal(gpu["nh"], r.nh)),
+              "is_seed_equal_oracle": bool(np.array_equal(gpu["is_seed"], r.is_seed)),
               "n_clusters_oracle": int(r.stats["n_stored_seeds"]),
+              "max_dfs_stack": int(r.stats["max_dfs_stack"]),
+              "band": band, "margins": margins,
               "inradius_pairs_density": int(r.stats["inradius_pairs_density"]),
               "inradius_pairs_nh": int(r.stats["inradius_pairs_nh"]),
               "candidate_pairs_density": int(r.stats["candidate_pairs_density"]),
               "candidate_pairs_nh": int(r.stats["candidate_pairs_nh"])}
-    try:
-        out["host_cpu"] = open("/proc/cpuinfo").read().split("model name")[1].split("\n")[0].strip(" :\t")
-    except Exception:
-        pass
+    if prm["kernel"][0] != "flat":
+        pl = oracle.make_params(prm["dc"], prm["rhoc"], prm["dm"], kernel=prm["kernel"], exp_mode=oracle.EXP_LIBM)
+        rl = oracle.make_clusters_par(ps.coords, ps.weights, pl, threads, with_tiles=False)
+        rel = np.abs(gpu["rho"].astype(np.float64) - rl.rho) / np.abs(rl.rho.astype(np.float64))
+        parity["libm_exp"] = {"densities_differing": int(np.count_nonzero(gpu["rho"] != rl.rho)),
+                              "max_rel_diff": float(rel.max()),
+                              "labels_equal": bool(np.array_equal(gpu["cluster"], rl.cluster)),
+                              "nh_equal": bool(np.array_equal(gpu["nh"], rl.nh)),
+                              "is_seed_equal": bool(np.array_equal(gpu["is_seed"], rl.is_seed))}
     return out, parity
 
 
@@ -395,7 +443,7 @@ def main():
     plan.set_timing(False)
     stages = {k: round(v / args.steps, 4) for k, v in acc.items()}
     stats = plan.stats()
-    labels = out["cluster"].download(stream)
+    gpu_out = {k: a.download(stream) for k, a in out.items()}
 
     # dominant kernel roofline: algorithmic bytes per point (density: read D
     # coords + weight, write rho = 4D+8; nearest-higher: read D coords + rho,
@@ -445,7 +493,7 @@ def main():
     if not args.no_served:
         result["served"] = served_bench(clue, datasets, args)
     if not args.no_cpu_baseline:
-        cb, parity = cpu_baseline(ps, prm, args, labels)
+        cb, parity = cpu_baseline(ps, prm, args, gpu_out)
         result["cpu_baseline"] = cb
         result["parity"] = parity
         pairs = parity["inradius_pairs_density"] + parity["inradius_pairs_nh"]

@@ -148,6 +148,7 @@ _SIGNATURES = {
     "clue_autotune_batched": ([_vp, _vp, _P(ClueParams), ctypes.c_int32, ctypes.c_int32, _P(ctypes.c_int64),
                                _P(_vp), _vp, _P(ClueKernelDesc), _P(ClueMetricDesc), _P(ctypes.c_uint8),
                                ctypes.c_int32, _P(ClueTuning), _P(ctypes.c_float)], ctypes.c_int),
+    "clue_autotune_candidates": ([_P(ctypes.c_int32), ctypes.c_int32, _P(ctypes.c_int32)], ctypes.c_int),
     "clue_server_create": ([ctypes.c_int32, _P(ClueServerConfig), _P(_vp)], ctypes.c_int),
     "clue_server_submit": ([_vp, _P(ClueRequest), _P(ctypes.c_uint64)], ctypes.c_int),
     "clue_server_wait": ([_vp, ctypes.c_uint64, _P(ClueRequestTiming)], ctypes.c_int),
@@ -188,6 +189,15 @@ def lib():
             fn.restype = res
         _lib = L
     return _lib
+
+
+def autotune_candidates():
+    """The points-per-tile values the autotuner tries."""
+    cnt = ctypes.c_int32(0)
+    check(lib().clue_autotune_candidates(None, 0, ctypes.byref(cnt)))
+    out = (ctypes.c_int32 * cnt.value)()
+    check(lib().clue_autotune_candidates(out, cnt.value, ctypes.byref(cnt)))
+    return tuple(int(v) for v in out)
 
 
 def exported_symbols():

@@ -18,6 +18,9 @@
 
 namespace clue_hip {
 
+// Points per tile the autotuner tries (with CLUE_TUNE_POINTS_PER_TILE).
+constexpr int32_t kTunePointsPerTile[] = {16, 32, 64, 128, 256, 512};
+
 // Stages timed by clue_plan_stage_times, in pipeline order.
 const char* const kStageNames[kStages] = {"bbox",    "bin",       "sort",     "offsets_gather",
                                           "density", "nearest_higher", "classify_rank", "labels"};
@@ -553,6 +556,17 @@ __global__ void k_compact(const int32_t* idx, int32_t n, int32_t* out) {
   if (p < n && idx[p + 1] != idx[p]) out[idx[p]] = p;
 }
 
+// The event and chunk tables of a call over one event, written on the device
+// from the descriptor passed by value (no host staging, no copy).
+__global__ void k_single_event_tables(EventDesc ed, EventDesc* ev, Chunk* chunks) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c == 0) *ev = ed;
+  if (c < ed.chunk_end) {
+    const int32_t b = ed.begin + c * kChunk;
+    chunks[c] = Chunk{0, b, min(b + kChunk, ed.begin + ed.n), 0};
+  }
+}
+
 // Per event: seeds, capacity and the base of its seed ranks.  counters (totals
 // over the events): [0] seed candidates, [1] grow-only capacity carried across
 // single-event calls, [2] capacity used, [3] seeds, [4] stored seeds
@@ -756,6 +770,19 @@ float sqrt_le_bound(float r) {
 clue_status reserve(clue_plan_s* p, int32_t D, int32_t n, int32_t T, int32_t n_events,
                     int32_t n_chunks) {
   CLUE_HIP(hipSetDevice(p->device));
+  // a plan over an arena lays every buffer out afresh, 256-byte aligned, and
+  // allocates nothing; arena_used then holds the bytes this call needs (it may
+  // exceed arena_bytes: the caller checks and refuses the call)
+  if (p->arena_bytes) p->arena_used = 0;
+  auto need = [p](DevBuf& b, size_t bytes) -> hipError_t {
+    if (!p->arena_bytes) return b.ensure(bytes);
+    const size_t a = std::max<size_t>(256, (bytes + 255) / 256 * 256);
+    const bool fits = p->arena_used + a <= p->arena_bytes;
+    b.p = fits ? p->arena + p->arena_used : nullptr;
+    b.bytes = fits ? a : 0;
+    p->arena_used += a;
+    return hipSuccess;
+  };
   const size_t N = static_cast<size_t>(n);
   const size_t nb = static_cast<size_t>(div_up(n, kSortTile));
   const size_t hist_entries = 256 * nb + 1;
@@ -763,66 +790,68 @@ clue_status reserve(clue_plan_s* p, int32_t D, int32_t n, int32_t T, int32_t n_e
   const int64_t ld = div_up(n, 64) * 64;
   const size_t TT = static_cast<size_t>(T) + 1;
   const size_t E = static_cast<size_t>(n_events);
-  CLUE_HIP(p->key.ensure(N * 4));
-  CLUE_HIP(p->packed.ensure(N * static_cast<size_t>((D + 2 + 3) / 4 * 4) * 4));
-  CLUE_HIP(p->key2.ensure(N * 4));
-  CLUE_HIP(p->perm.ensure(N * 4));
-  CLUE_HIP(p->perm2.ensure(N * 4));
-  CLUE_HIP(p->offsets.ensure(TT * 4));
-  CLUE_HIP(p->tnf.ensure(TT * 4));
+  CLUE_HIP(need(p->key, N * 4));
+  CLUE_HIP(need(p->packed, N * static_cast<size_t>((D + 2 + 3) / 4 * 4) * 4));
+  CLUE_HIP(need(p->key2, N * 4));
+  CLUE_HIP(need(p->perm, N * 4));
+  CLUE_HIP(need(p->perm2, N * 4));
+  CLUE_HIP(need(p->offsets, TT * 4));
+  CLUE_HIP(need(p->tnf, TT * 4));
   if (D > 3) {
-    CLUE_HIP(p->qk.ensure(N * 4));
-    CLUE_HIP(p->qk2.ensure(N * 4));
-    CLUE_HIP(p->qv.ensure(N * 4));
-    CLUE_HIP(p->qv2.ensure(N * 4));
+    CLUE_HIP(need(p->qk, N * 4));
+    CLUE_HIP(need(p->qk2, N * 4));
+    CLUE_HIP(need(p->qv, N * 4));
+    CLUE_HIP(need(p->qv2, N * 4));
   }
-  CLUE_HIP(p->hist.ensure(hist_entries * 4));
-  CLUE_HIP(p->part.ensure((div_up(scan_max, kScanTile) + 1) * 4));
-  CLUE_HIP(p->xs.ensure(static_cast<size_t>(ld) * D * 4));
+  CLUE_HIP(need(p->hist, hist_entries * 4));
+  CLUE_HIP(need(p->part, (div_up(scan_max, kScanTile) + 1) * 4));
+  CLUE_HIP(need(p->xs, static_cast<size_t>(ld) * D * 4));
   p->ld = ld;
-  CLUE_HIP(p->ws.ensure(N * 4));
-  CLUE_HIP(p->hw.ensure(N * 4));
-  CLUE_HIP(p->rho_s.ensure(N * 4));
-  CLUE_HIP(p->delta.ensure(N * 4));
-  CLUE_HIP(p->flag.ensure((N + 1) * 4));
-  CLUE_HIP(p->root.ensure(N * 4));
-  CLUE_HIP(p->nh_tmp.ensure(N * 4));
-  CLUE_HIP(p->rho_tmp.ensure(N * 4));
-  CLUE_HIP(p->seed_tmp.ensure(N * 4));
-  CLUE_HIP(p->bbox_part.ensure(static_cast<size_t>(n_chunks) * D * 2 * 4));
-  CLUE_HIP(p->events.ensure(E * sizeof(EventDesc)));
-  CLUE_HIP(p->chunks.ensure(static_cast<size_t>(n_chunks) * sizeof(Chunk)));
-  CLUE_HIP(p->pev.ensure(N * 4));
-  CLUE_HIP(p->sc.ensure(E * 4));
-  CLUE_HIP(p->cap_used.ensure(E * 4));
-  CLUE_HIP(p->rank_base.ensure(E * 4));
-  if (!p->counters.p) {
-    CLUE_HIP(p->counters.ensure(16 * 4));
+  CLUE_HIP(need(p->ws, N * 4));
+  CLUE_HIP(need(p->hw, N * 4));
+  CLUE_HIP(need(p->rho_s, N * 4));
+  CLUE_HIP(need(p->delta, N * 4));
+  CLUE_HIP(need(p->flag, (N + 1) * 4));
+  CLUE_HIP(need(p->root, N * 4));
+  CLUE_HIP(need(p->nh_tmp, N * 4));
+  CLUE_HIP(need(p->rho_tmp, N * 4));
+  CLUE_HIP(need(p->seed_tmp, N * 4));
+  CLUE_HIP(need(p->bbox_part, static_cast<size_t>(n_chunks) * D * 2 * 4));
+  CLUE_HIP(need(p->events, E * sizeof(EventDesc)));
+  CLUE_HIP(need(p->chunks, static_cast<size_t>(n_chunks) * sizeof(Chunk)));
+  CLUE_HIP(need(p->pev, N * 4));
+  CLUE_HIP(need(p->sc, E * 4));
+  CLUE_HIP(need(p->cap_used, E * 4));
+  CLUE_HIP(need(p->rank_base, E * 4));
+  if (p->arena_bytes) {
+    CLUE_HIP(need(p->counters, 16 * 4));  // cleared by a fill kernel on every arena call
+  } else if (!p->counters.p) {
+    CLUE_HIP(need(p->counters, 16 * 4));
     CLUE_HIP(hipMemset(p->counters.p, 0, 16 * 4));
   }
-  CLUE_HIP(p->kn.ensure(N * 4));
-  CLUE_HIP(p->kn2.ensure(N * 4));
-  CLUE_HIP(p->pn.ensure(N * 4));
-  CLUE_HIP(p->pn2.ensure(N * 4));
-  CLUE_HIP(p->mid.ensure(N * 4));
-  CLUE_HIP(p->xn.ensure(static_cast<size_t>(ld) * D * 4));
-  CLUE_HIP(p->wn.ensure(N * 4));
-  CLUE_HIP(p->hk.ensure(N * 8));
-  CLUE_HIP(p->bcnt.ensure(TT * 4));
-  CLUE_HIP(p->blk_off.ensure(TT * 4));
+  CLUE_HIP(need(p->kn, N * 4));
+  CLUE_HIP(need(p->kn2, N * 4));
+  CLUE_HIP(need(p->pn, N * 4));
+  CLUE_HIP(need(p->pn2, N * 4));
+  CLUE_HIP(need(p->mid, N * 4));
+  CLUE_HIP(need(p->xn, static_cast<size_t>(ld) * D * 4));
+  CLUE_HIP(need(p->wn, N * 4));
+  CLUE_HIP(need(p->hk, N * 8));
+  CLUE_HIP(need(p->bcnt, TT * 4));
+  CLUE_HIP(need(p->blk_off, TT * 4));
   const int64_t ldb = div_up(n, kBlock) + T + 1;
-  CLUE_HIP(p->bbox_lo.ensure(static_cast<size_t>(ldb) * D * 4));
-  CLUE_HIP(p->bbox_hi.ensure(static_cast<size_t>(ldb) * D * 4));
-  CLUE_HIP(p->btile.ensure(static_cast<size_t>(ldb) * 4));
+  CLUE_HIP(need(p->bbox_lo, static_cast<size_t>(ldb) * D * 4));
+  CLUE_HIP(need(p->bbox_hi, static_cast<size_t>(ldb) * D * 4));
+  CLUE_HIP(need(p->btile, static_cast<size_t>(ldb) * 4));
   p->ldb = ldb;
-  CLUE_HIP(p->tbox_lo.ensure(static_cast<size_t>(T) * D * 4));
-  CLUE_HIP(p->tbox_hi.ensure(static_cast<size_t>(T) * D * 4));
-  CLUE_HIP(p->bmaxk.ensure(static_cast<size_t>(ldb) * 8));
-  CLUE_HIP(p->tmaxk.ensure(static_cast<size_t>(T) * 8));
-  CLUE_HIP(p->pending.ensure((N + 64) * 4));
-  CLUE_HIP(p->pending_flag.ensure((N + 1) * 4));
-  CLUE_HIP(p->qcost.ensure(static_cast<size_t>(div_up(n, kThreads)) * 4));
-  CLUE_HIP(p->qorder.ensure(static_cast<size_t>(div_up(n, kThreads)) * 4));
+  CLUE_HIP(need(p->tbox_lo, static_cast<size_t>(T) * D * 4));
+  CLUE_HIP(need(p->tbox_hi, static_cast<size_t>(T) * D * 4));
+  CLUE_HIP(need(p->bmaxk, static_cast<size_t>(ldb) * 8));
+  CLUE_HIP(need(p->tmaxk, static_cast<size_t>(T) * 8));
+  CLUE_HIP(need(p->pending, (N + 64) * 4));
+  CLUE_HIP(need(p->pending_flag, (N + 1) * 4));
+  CLUE_HIP(need(p->qcost, static_cast<size_t>(div_up(n, kThreads)) * 4));
+  CLUE_HIP(need(p->qorder, static_cast<size_t>(div_up(n, kThreads)) * 4));
   return CLUE_OK;
 }
 
@@ -1051,6 +1080,26 @@ size_t table_bytes(const BatchCall& bc) {
   return static_cast<size_t>(bc.n_events) * sizeof(EventDesc) + static_cast<size_t>(bc.n_chunks) * sizeof(Chunk);
 }
 
+// Descriptor of one event: its grid shape with the reference's formulas, tile
+// base, integer strides npd^(D-1-d) and chunk range (extremes and tile sizes are
+// filled on the device by the bounding-box kernels).
+EventDesc event_desc(int32_t D, int32_t begin, int32_t n, int32_t ppb, int32_t tile_base, int32_t chunk_begin) {
+  EventDesc ed;
+  std::memset(&ed, 0, sizeof(ed));
+  ed.begin = begin;
+  ed.n = n;
+  grid_shape(D, n, ppb, ed.npd, ed.ntiles);
+  ed.tile_base = tile_base;
+  int32_t s1 = 1;
+  for (int d = D - 1; d >= 0; --d) {
+    ed.stride[d] = s1;
+    s1 *= ed.npd;
+  }
+  ed.chunk_begin = chunk_begin;
+  ed.chunk_end = chunk_begin + static_cast<int32_t>(div_up(n, kChunk));
+  return ed;
+}
+
 // Event descriptors (grid shape, tile base, integer strides, chunk range) and
 // the chunk table, written to host memory laid out as [events | chunks].
 void fill_tables(const BatchCall& bc, const clue_params* prm, const int64_t* event_offsets, void* host) {
@@ -1059,21 +1108,12 @@ void fill_tables(const BatchCall& bc, const clue_params* prm, const int64_t* eve
   int32_t tile_base = 0, chunk = 0;
   for (int e = 0; e < bc.n_events; ++e) {
     EventDesc& ed = hev[e];
-    std::memset(&ed, 0, sizeof(ed));
-    ed.begin = static_cast<int32_t>(event_offsets[e]);
-    ed.n = static_cast<int32_t>(event_offsets[e + 1] - event_offsets[e]);
-    grid_shape(bc.D, ed.n, prm->points_per_tile, ed.npd, ed.ntiles);
-    ed.tile_base = tile_base;
+    ed = event_desc(bc.D, static_cast<int32_t>(event_offsets[e]),
+                    static_cast<int32_t>(event_offsets[e + 1] - event_offsets[e]), prm->points_per_tile,
+                    tile_base, chunk);
     tile_base += ed.ntiles;
-    int32_t s1 = 1;
-    for (int d = bc.D - 1; d >= 0; --d) {
-      ed.stride[d] = s1;
-      s1 *= ed.npd;
-    }
-    ed.chunk_begin = chunk;
     for (int32_t b = ed.begin; b < ed.begin + ed.n; b += kChunk)
       hch[chunk++] = Chunk{e, b, std::min(b + kChunk, ed.begin + ed.n), 0};
-    ed.chunk_end = chunk;
   }
 }
 
@@ -1882,6 +1922,15 @@ clue_status clue_batch_graph_destroy(clue_batch_graph_t graph) {
   return CLUE_OK;
 }
 
+clue_status clue_autotune_candidates(int32_t* points_per_tile, int32_t capacity, int32_t* count) {
+  if (!count) return fail(CLUE_ERR_INVALID_ARGUMENT, "count is NULL");
+  constexpr int32_t k = static_cast<int32_t>(sizeof(kTunePointsPerTile) / sizeof(kTunePointsPerTile[0]));
+  *count = k;
+  if (points_per_tile)
+    for (int32_t i = 0; i < k && i < capacity; ++i) points_per_tile[i] = kTunePointsPerTile[i];
+  return CLUE_OK;
+}
+
 // Coordinate descent over the launch tuning (and, when asked, the points per
 // tile) on the given batch: every candidate is run once untimed and three times
 // timed with HIP events on `stream`; the fastest median wins.
@@ -1932,7 +1981,7 @@ clue_status clue_autotune_batched(clue_plan_t plan, void* stream, clue_params* p
   if (rc == CLUE_OK && (flags & CLUE_TUNE_POINTS_PER_TILE)) {
     const int32_t keep = prm.points_per_tile;
     int32_t best_ppb = keep;
-    for (int32_t ppb : {16, 32, 64, 128, 256, 512}) {
+    for (int32_t ppb : kTunePointsPerTile) {
       if (ppb == keep || rc != CLUE_OK) continue;
       prm.points_per_tile = ppb;
       float ms;

@@ -406,6 +406,10 @@ struct clue_plan_s {
   hipEvent_t done = nullptr;  // recorded after each call's last kernel (introspection waits on it)
   int32_t last_n = 0, last_d = 0, last_T = 0, last_events = 0;
   int32_t* last_perm = nullptr;
+  // a plan over a region of device memory it does not own (the no-allocation
+  // entry point): reserve() carves every buffer from it and allocates nothing
+  unsigned char* arena = nullptr;
+  size_t arena_bytes = 0, arena_used = 0;
   // optional per-stage HIP event timing of the last call
   bool timing = false;
   hipEvent_t ev[kStages + 1] = {};

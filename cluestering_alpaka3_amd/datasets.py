@@ -142,6 +142,30 @@ def workload(name, seed=None, n=None):
     raise KeyError(name)
 
 
+def workload_scaled(name, n, seed=None):
+    """A named workload at n points with its local density kept: the box side
+    scales by (n / N)^(1/D) and the number of blobs by n / N (N the workload's
+    full size), as in the served mix -- so a reduced test sees the full
+    workload's tile occupancy and neighbour counts, not a sparser regime."""
+    full = {"2d_16k_flat": (16384, 2, 10, 100.0), "2d_1m_gauss": (1048576, 2, 200, 1000.0),
+            "3d_4m_exp": (4194304, 3, 1000, 500.0), "10d_256k_flat": (262144, 10, 50, 100.0)}
+    N, D, nb_full, side_full = full[name]
+    if n is None or n == N:
+        return workload(name, seed=seed)
+    f = n / N
+    side = side_full * f ** (1.0 / D)
+    nb = max(1, int(round(nb_full * f)))
+    if name == "2d_16k_flat":
+        return blobs(n, 2, nb, 0.0, side, 1.5, 0.10, ("const", 1.0), 1001 if seed is None else seed, name)
+    if name == "2d_1m_gauss":
+        return blobs(n, 2, nb, 0.0, side, ("uniform", 1.0, 4.0), 0.20, ("uniform", 0.5, 1.5),
+                     2002 if seed is None else seed, name)
+    if name == "3d_4m_exp":
+        return blobs(n, 3, nb, 0.0, side, ("uniform_axis", 0.5, 3.0), 0.15, ("exponential", 1.0),
+                     3003 if seed is None else seed, name)
+    return blobs(n, 10, nb, 0.0, side, 1.0, 0.05, ("const", 1.0), 3004 if seed is None else seed, name)
+
+
 def calorimeter_event(n, seed):
     """A toy calorimeter layer: 5-40 showers in [-150,150]^2 with radial Gaussian
     spread 2, plus uniform noise hits; energies exponential (mean 0.5) + 0.1."""

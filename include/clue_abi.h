@@ -230,6 +230,11 @@ clue_status clue_autotune_batched(clue_plan_t plan, void* stream, clue_params* p
                                   const clue_kernel_desc* kernel, const clue_metric_desc* metric,
                                   const uint8_t* wrapped, int32_t flags, clue_tuning* best, float* best_ms);
 
+/* The points-per-tile values clue_autotune_batched tries with
+ * CLUE_TUNE_POINTS_PER_TILE: *count receives their number, the first
+ * min(count, capacity) go to points_per_tile (which may be NULL). */
+clue_status clue_autotune_candidates(int32_t* points_per_tile, int32_t capacity, int32_t* count);
+
 /* ---- served clustering (no reference counterpart: the reference clusters
  *      one event per synchronous make_clusters call).  A server owns `slots`
  *      batch slots (stream, plan, device and pinned buffers), sized once so

@@ -316,10 +316,41 @@ struct Counters {
   int64_t cand_rho = 0, cand_nh = 0, in_rho = 0, in_nh = 0;
 };
 
-// K4: KernelCalculateLocalDensity (ClusteringKernels.hpp:18-107), one point.
+// The points in tile order (the CSR's slot order): coordinates and weights of
+// slot s are those of point indexes[s].  The loops below read candidates from
+// this copy -- the values and their visit order are the reference's; only the
+// memory they are read from is contiguous per tile.
+struct TileOrdered {
+  std::vector<float> x;  // D x n, x[d * n + s]
+  std::vector<float> w;
+};
+
 template <int D>
-inline float density_point(const Tiles& t, int32_t n, const float* coords, const float* w,
-                           const oracle_params& p, int32_t i, Counters& cnt) {
+void tile_order(const Tiles& t, int32_t n, const float* coords, const float* w, TileOrdered& to) {
+  to.x.resize(static_cast<size_t>(D) * n);
+  to.w.resize(n);
+  for (int32_t s = 0; s < n; ++s) {
+    const int32_t j = t.indexes[s];
+    for (int d = 0; d < D; ++d) to.x[static_cast<size_t>(d) * n + s] = coords[static_cast<size_t>(d) * n + j];
+    to.w[s] = w[j];
+  }
+}
+
+template <int D>
+inline void slot_point(const TileOrdered& to, int32_t n, int32_t s, float* out) {
+  for (int d = 0; d < D; ++d) out[d] = to.x[static_cast<size_t>(d) * n + s];
+}
+
+// K4: KernelCalculateLocalDensity (ClusteringKernels.hpp:18-107), one point.
+// Every candidate of the search box contributes ((float)(d <= dc) * k) * w_j
+// (:38-44).  For a candidate outside dc whose distance is a number, whose weight
+// is finite and whose kernel value is finite for every such distance
+// (`bounded`), that term is a signed zero and adding it leaves rho_i unchanged
+// (rho_i starts at +0 and a float sum never becomes -0 from such terms), so the
+// kernel is not evaluated for it.
+template <int D>
+inline float density_point(const Tiles& t, const TileOrdered& to, int32_t n, const float* coords,
+                           const oracle_params& p, bool bounded, int32_t i, Counters& cnt) {
   float ci[D], cj[D];
   load_point<D>(coords, n, i, ci);
   float rho_i = 0.f;
@@ -328,22 +359,25 @@ inline float density_point(const Tiles& t, int32_t n, const float* coords, const
   for_each_box_tile<D>(t, lo, hi, [&](int32_t tile) {
     for (int32_t s = t.offsets[tile]; s < t.offsets[tile + 1]; ++s) {
       const int32_t j = t.indexes[s];
-      load_point<D>(coords, n, j, cj);
+      slot_point<D>(to, n, s, cj);
+      const float wj = to.w[s];
       const float dist = metric_distance<D>(ci, cj, p);
-      const float k = kernel_value(p, dist, i, j);
       const bool inside = (g_mutation == MUT_DENSITY_STRICT) ? (dist < p.dc) : (dist <= p.dc);
-      rho_i += static_cast<float>(static_cast<int>(inside)) * k * w[j];
       cnt.cand_rho += 1;
       cnt.in_rho += inside ? 1 : 0;
+      if (!inside && bounded && dist == dist && std::isfinite(wj)) continue;
+      const float k = kernel_value(p, dist, i, j);
+      rho_i += static_cast<float>(static_cast<int>(inside)) * k * wj;
     }
   });
   return rho_i;
 }
 
 // K5: KernelCalculateNearestHigher (ClusteringKernels.hpp:109-212), one point.
+// rho_s: the densities in tile order.
 template <int D>
-inline int32_t nh_point(const Tiles& t, int32_t n, const float* coords, const float* rho,
-                        const oracle_params& p, int32_t i, float* delta_out, Counters& cnt) {
+inline int32_t nh_point(const Tiles& t, const TileOrdered& to, int32_t n, const float* coords, const float* rho,
+                        const float* rho_s, const oracle_params& p, int32_t i, float* delta_out, Counters& cnt) {
   float ci[D], cj[D];
   load_point<D>(coords, n, i, ci);
   float delta_i = std::numeric_limits<float>::max();
@@ -354,11 +388,11 @@ inline int32_t nh_point(const Tiles& t, int32_t n, const float* coords, const fl
   for_each_box_tile<D>(t, lo, hi, [&](int32_t tile) {
     for (int32_t s = t.offsets[tile]; s < t.offsets[tile + 1]; ++s) {
       const int32_t j = t.indexes[s];
-      const float rho_j = rho[j];
+      const float rho_j = rho_s[s];
       bool higher = (rho_j > rho_i);
       if (g_mutation != MUT_NO_EQUAL_RHO_TIE)
         higher = higher || ((rho_j == rho_i) && (rho_j > 0.f) && (j > i));
-      load_point<D>(coords, n, j, cj);
+      slot_point<D>(to, n, s, cj);
       const float dist = metric_distance<D>(ci, cj, p);
       cnt.cand_nh += 1;
       cnt.in_nh += (dist <= p.dm) ? 1 : 0;
@@ -373,6 +407,22 @@ inline int32_t nh_point(const Tiles& t, int32_t n, const float* coords, const fl
   });
   if (delta_out) *delta_out = delta_i;
   return nh_i;
+}
+
+// Kernel values finite for every distance in [0, +inf] (see density_point).
+bool kernel_bounded(const oracle_params& p) {
+  const float* k = p.kernel_params;
+  switch (p.kernel_type) {
+    case ORACLE_KERNEL_FLAT:
+      return std::isfinite(k[0]);
+    case ORACLE_KERNEL_GAUSSIAN: {
+      const float den = (2.f * k[1]) * k[1];
+      return std::isfinite(k[0]) && std::isfinite(k[2]) && std::isfinite(den) && den > 0.f;
+    }
+    case ORACLE_KERNEL_EXPONENTIAL:
+      return std::isfinite(k[0]) && k[0] > 0.f && std::isfinite(k[1]);
+  }
+  return false;
 }
 
 // K6 + K7 (ClusteringKernels.hpp:214-281) with SeedArray capacity semantics.
@@ -462,20 +512,31 @@ int run_pipeline(int32_t n, const float* coords, const float* w, const oracle_pa
   if (tile_offsets) std::copy(t.offsets.begin(), t.offsets.end(), tile_offsets);
   if (tile_indexes) std::copy(t.indexes.begin(), t.indexes.end(), tile_indexes);
 
-  std::vector<float> rho(n), delta(n);
+  std::vector<float> rho(n), delta(n), rho_s(n);
   std::vector<int32_t> nh(n);
+  TileOrdered to;
+  tile_order<D>(t, n, coords, w, to);
+  const bool bounded = kernel_bounded(p);
   int64_t cand_rho = 0, cand_nh = 0, in_rho = 0, in_nh = 0, seed_candidates = 0;
 #ifdef _OPENMP
   if (parallel && nthreads > 0) omp_set_num_threads(nthreads);
 #endif
+  // every point's density and nearest-higher are independent of the others';
+  // the queries run in tile order (neighbouring queries read the same tiles)
 #pragma omp parallel if (parallel)
   {
     Counters c;
 #pragma omp for schedule(dynamic, 256)
-    for (int32_t i = 0; i < n; ++i) rho[i] = density_point<D>(t, n, coords, w, p, i, c);
+    for (int32_t q = 0; q < n; ++q) {
+      const int32_t i = t.indexes[q];
+      rho[i] = density_point<D>(t, to, n, coords, p, bounded, i, c);
+    }
+#pragma omp for schedule(static)
+    for (int32_t q = 0; q < n; ++q) rho_s[q] = rho[t.indexes[q]];
 #pragma omp for schedule(dynamic, 256) reduction(+ : seed_candidates)
-    for (int32_t i = 0; i < n; ++i) {
-      nh[i] = nh_point<D>(t, n, coords, rho.data(), p, i, &delta[i], c);
+    for (int32_t q = 0; q < n; ++q) {
+      const int32_t i = t.indexes[q];
+      nh[i] = nh_point<D>(t, to, n, coords, rho.data(), rho_s.data(), p, i, &delta[i], c);
       seed_candidates += (nh[i] == -1);
     }
 #pragma omp atomic
@@ -574,6 +635,18 @@ int oracle_make_clusters_omp(int32_t ndim, int32_t n, const float* coords,
   });
 }
 
+int oracle_make_clusters_par(int32_t ndim, int32_t n, const float* coords, const float* weights,
+                             const oracle_params* p, int32_t nthreads, float* rho, int32_t* nh, float* delta,
+                             int32_t* is_seed, int32_t* cluster, oracle_stats* stats, int32_t* tile_offsets,
+                             int32_t* tile_indexes, oracle_grid* grid_out) {
+  if (!params_valid(ndim, n, p) || !cluster) return -1;
+  return dispatch_dim(ndim, [&](auto dimc) {
+    return run_pipeline<decltype(dimc)::value>(n, coords, weights, *p, true, nthreads, rho, nh, delta,
+                                               is_seed, cluster, stats, tile_offsets, tile_indexes,
+                                               grid_out);
+  });
+}
+
 int oracle_bruteforce_density_fp64(int32_t ndim, int32_t n, const float* coords,
                                    const float* weights, const oracle_params* p, double* rho) {
   if (!params_valid(ndim, n, p)) return -1;
@@ -651,6 +724,130 @@ int oracle_bruteforce_nh(int32_t ndim, int32_t n, const float* coords, const flo
       nh[i] = best_j;
       if (delta) delta[i] = best;
     }
+    return 0;
+  });
+}
+
+// Margins of a result (rho as computed by the pipeline, float32) against the
+// decisions that depend on it; see oracle_margins in clue_oracle.h.  The search
+// boxes, visit order and predicates are those of K4/K5
+// (ClusteringKernels.hpp:18-212); the FP64 density keeps the reference's
+// membership test (float distance <= dc) and evaluates distance, kernel and sum
+// in double from the float inputs.
+int oracle_parity_margins(int32_t ndim, int32_t n, const float* coords, const float* weights,
+                          const oracle_params* p, const float* rho, int32_t nthreads, double band,
+                          oracle_margins* out) {
+  if (!params_valid(ndim, n, p) || !rho || !out) return -1;
+  if (p->metric_type != ORACLE_METRIC_EUCLIDEAN) return -2;
+  init_exp_table();
+  return dispatch_dim(ndim, [&](auto dimc) {
+    constexpr int D = decltype(dimc)::value;
+    oracle_grid g;
+    grid_setup(D, n, coords, p->points_per_tile, &g);
+    Tiles t;
+    tiles_init(t, D, g, *p);
+    tiles_fill(t, n, coords);
+    TileOrdered to;
+    tile_order<D>(t, n, coords, weights, to);
+    std::vector<float> rho_s(n);
+    for (int32_t q = 0; q < n; ++q) rho_s[q] = rho[t.indexes[q]];
+    int64_t rhoc_band = 0, dist_ties = 0, dist_near = 0, dens_near = 0, above = 0, boundary = 0;
+    double max_rel = 0.0;
+    const float rhoc = p->rhoc;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : rhoc_band, dist_ties, dist_near, dens_near, above, boundary) reduction(max : max_rel)
+    for (int32_t q = 0; q < n; ++q) {  // queries in tile order
+      const int32_t i = t.indexes[q];
+      float ci[D], cj[D];
+      load_point<D>(coords, n, i, ci);
+      const float rho_i = rho[i];
+      if (std::fabs(static_cast<double>(rho_i) - rhoc) <= band * rhoc) ++rhoc_band;
+      // FP64 density with the reference's membership
+      int32_t lo[D], hi[D];
+      t.search_box(ci, p->dc, lo, hi);
+      double acc = 0.0;
+      for_each_box_tile<D>(t, lo, hi, [&](int32_t tile) {
+        for (int32_t s = t.offsets[tile]; s < t.offsets[tile + 1]; ++s) {
+          const int32_t j = t.indexes[s];
+          slot_point<D>(to, n, s, cj);
+          const float df = metric_distance<D>(ci, cj, *p);
+          double s2 = 0.0;
+          for (int d = 0; d < D; ++d) {
+            const double dd = static_cast<double>(ci[d]) - static_cast<double>(cj[d]);
+            s2 += dd * dd;
+          }
+          const double dist = std::sqrt(s2);
+          const bool in_f = df <= p->dc;
+          if (in_f != (dist <= static_cast<double>(p->dc))) ++boundary;
+          if (!in_f) continue;
+          double k;
+          if (i == j) {
+            k = 1.0;
+          } else if (p->kernel_type == ORACLE_KERNEL_FLAT) {
+            k = p->kernel_params[0];
+          } else if (p->kernel_type == ORACLE_KERNEL_GAUSSIAN) {
+            const double a = dist - p->kernel_params[0];
+            const double sd = p->kernel_params[1];
+            k = p->kernel_params[2] * std::exp(-(a * a) / (2.0 * sd * sd));
+          } else {
+            k = p->kernel_params[1] * std::exp(-static_cast<double>(p->kernel_params[0]) * dist);
+          }
+          acc += k * static_cast<double>(to.w[s]);
+        }
+      });
+      const double rel = acc != 0.0 ? std::fabs(static_cast<double>(rho_i) - acc) / std::fabs(acc)
+                                    : std::fabs(static_cast<double>(rho_i));
+      if (rel > max_rel) max_rel = rel;
+      if (rel > band) ++above;
+      // nearest-higher: best and runner-up higher candidates, and candidates at least
+      // as near as the best whose density lies in the band around rho_i
+      t.search_box(ci, p->dm, lo, hi);
+      float d1 = std::numeric_limits<float>::max(), d2 = std::numeric_limits<float>::max();
+      for_each_box_tile<D>(t, lo, hi, [&](int32_t tile) {
+        for (int32_t s = t.offsets[tile]; s < t.offsets[tile + 1]; ++s) {
+          const int32_t j = t.indexes[s];
+          const float rho_j = rho_s[s];
+          const bool higher = rho_j > rho_i || (rho_j == rho_i && rho_j > 0.f && j > i);
+          if (!higher) continue;
+          slot_point<D>(to, n, s, cj);
+          const float dist = metric_distance<D>(ci, cj, *p);
+          if (!(dist <= p->dm)) continue;
+          if (dist < d1) {
+            d2 = d1;
+            d1 = dist;
+          } else if (dist < d2) {
+            d2 = dist;
+          }
+        }
+      });
+      if (d2 <= p->dm) {
+        if (d2 == d1) ++dist_ties;
+        if (static_cast<double>(d2) - d1 <= band * d1) ++dist_near;
+      }
+      const float reach = d1 <= p->dm ? d1 : p->dm;
+      bool fragile = false;
+      for_each_box_tile<D>(t, lo, hi, [&](int32_t tile) {
+        for (int32_t s = t.offsets[tile]; s < t.offsets[tile + 1] && !fragile; ++s) {
+          const int32_t j = t.indexes[s];
+          if (j == i) continue;
+          const double a = rho_s[s], b = rho_i;
+          const double m = std::max(std::fabs(a), std::fabs(b));
+          if (!(std::fabs(a - b) <= band * m)) continue;
+          slot_point<D>(to, n, s, cj);
+          if (metric_distance<D>(ci, cj, *p) <= reach) fragile = true;
+        }
+      });
+      if (fragile) ++dens_near;
+    }
+    out->rhoc_band = rhoc_band;
+    out->nh_distance_ties = dist_ties;
+    out->nh_distance_near_ties = dist_near;
+    out->nh_density_near_ties = dens_near;
+    out->density_max_rel_err_fp64 = max_rel;
+    out->density_above_band_fp64 = above;
+    out->dc_boundary_pairs = boundary;
     return 0;
   });
 }

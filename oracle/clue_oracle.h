@@ -104,6 +104,45 @@ int oracle_make_clusters_omp(int32_t ndim, int32_t n, const float* coords,
                              float* rho, int32_t* nh, int32_t* is_seed, int32_t* cluster,
                              oracle_stats* stats);
 
+/* All outputs of oracle_make_clusters (delta, tile CSR and grid included), the
+ * density / nearest-higher / assignment loops on nthreads OpenMP threads (<= 0:
+ * the default).  Results are identical to the serial call. */
+int oracle_make_clusters_par(int32_t ndim, int32_t n, const float* coords, const float* weights,
+                             const oracle_params* p, int32_t nthreads, float* rho, int32_t* nh, float* delta,
+                             int32_t* is_seed, int32_t* cluster, oracle_stats* stats, int32_t* tile_offsets,
+                             int32_t* tile_indexes, oracle_grid* grid_out);
+
+/* Margins of a clustering result: the points whose decisions lie within a
+ * relative band (1e-6 in the tests) of a flip.  rho is the pipeline's float
+ * density (the oracle's or the engine's, which are bit-identical).
+ *  rhoc_band                 |rho_i - rhoc| <= band * rhoc (the seed test rho >= rhoc,
+ *                            ClusteringKernels.hpp:233)
+ *  nh_distance_ties          the two nearest higher candidates within dm at equal
+ *                            float distance: the visit order decides (:139)
+ *  nh_distance_near_ties     their distances within band (relative), ties included
+ *  nh_density_near_ties      a candidate j != i at distance <= the chosen one's (dm
+ *                            when none) with |rho_j - rho_i| <= band * max(|rho_i|,
+ *                            |rho_j|): the "higher" predicate (:132-133) could flip
+ *  density_max_rel_err_fp64  max_i |rho_i - rho64_i| / |rho64_i|, rho64 the same sum
+ *                            (membership by the float distance <= dc) with distance,
+ *                            kernel and accumulation in double
+ *  density_above_band_fp64   points whose relative error exceeds band
+ *  dc_boundary_pairs         candidate pairs whose float and double distances fall
+ *                            on different sides of dc
+ * Euclidean metric only (returns -2 otherwise). */
+typedef struct {
+  int64_t rhoc_band;
+  int64_t nh_distance_ties;
+  int64_t nh_distance_near_ties;
+  int64_t nh_density_near_ties;
+  double density_max_rel_err_fp64;
+  int64_t density_above_band_fp64;
+  int64_t dc_boundary_pairs;
+} oracle_margins;
+int oracle_parity_margins(int32_t ndim, int32_t n, const float* coords, const float* weights,
+                          const oracle_params* p, const float* rho, int32_t nthreads, double band,
+                          oracle_margins* out);
+
 /* Brute-force checkers for tiny inputs (no tiles):
  * FP64 density rho_i = sum_{j: d_ij <= dc} w_j * k_ij (kernel and distance in double). */
 int oracle_bruteforce_density_fp64(int32_t ndim, int32_t n, const float* coords,

@@ -82,6 +82,22 @@ class OracleStats(ctypes.Structure):
         return {name: int(getattr(self, name)) for name, _ in self._fields_}
 
 
+class OracleMargins(ctypes.Structure):
+    _fields_ = [
+        ("rhoc_band", ctypes.c_int64),
+        ("nh_distance_ties", ctypes.c_int64),
+        ("nh_distance_near_ties", ctypes.c_int64),
+        ("nh_density_near_ties", ctypes.c_int64),
+        ("density_max_rel_err_fp64", ctypes.c_double),
+        ("density_above_band_fp64", ctypes.c_int64),
+        ("dc_boundary_pairs", ctypes.c_int64),
+    ]
+
+    def as_dict(self):
+        return {name: (float(getattr(self, name)) if t is ctypes.c_double else int(getattr(self, name)))
+                for name, t in self._fields_}
+
+
 _libs = {}
 
 
@@ -101,6 +117,11 @@ def lib(release=False):
         L.oracle_make_clusters_omp.argtypes = [
             ctypes.c_int32, ctypes.c_int32, f32p, f32p, P(OracleParams), ctypes.c_int32,
             f32p, i32p, i32p, i32p, P(OracleStats)]
+        L.oracle_make_clusters_par.argtypes = [
+            ctypes.c_int32, ctypes.c_int32, f32p, f32p, P(OracleParams), ctypes.c_int32,
+            f32p, i32p, f32p, i32p, i32p, P(OracleStats), i32p, i32p, P(OracleGrid)]
+        L.oracle_parity_margins.argtypes = [ctypes.c_int32, ctypes.c_int32, f32p, f32p, P(OracleParams), f32p,
+                                            ctypes.c_int32, ctypes.c_double, P(OracleMargins)]
         L.oracle_bruteforce_density_fp64.argtypes = [ctypes.c_int32, ctypes.c_int32, f32p, f32p, P(OracleParams), f64p]
         L.oracle_bruteforce_nh.argtypes = [ctypes.c_int32, ctypes.c_int32, f32p, f32p, P(OracleParams), i32p, f32p]
         L.oracle_expf_engine.argtypes = [ctypes.c_float]
@@ -214,6 +235,42 @@ def make_clusters(coords, weights, params, with_tiles=True, release=False) -> Or
     if rc != 0:
         raise ValueError("oracle_make_clusters: invalid parameters")
     return OracleResult(rho, nh, delta, is_seed, cluster, st.as_dict(), off, idx, gi)
+
+
+def make_clusters_par(coords, weights, params, nthreads=0, with_tiles=True) -> OracleResult:
+    """make_clusters with every output, its loops on nthreads OpenMP threads (same results)."""
+    c = _soa(coords)
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    ndim, n = c.shape
+    rho = np.empty(n, np.float32)
+    nh = np.empty(n, np.int32)
+    delta = np.empty(n, np.float32)
+    is_seed = np.empty(n, np.int32)
+    cluster = np.empty(n, np.int32)
+    st = OracleStats()
+    g = OracleGrid()
+    gi = grid(c, params.points_per_tile)
+    off = np.empty(gi["ntiles"] + 1, np.int32) if with_tiles else None
+    idx = np.empty(n, np.int32) if with_tiles else None
+    rc = lib().oracle_make_clusters_par(ndim, n, _fp(c), _fp(w), ctypes.byref(params), nthreads, _fp(rho),
+                                        _ip(nh), _fp(delta), _ip(is_seed), _ip(cluster), ctypes.byref(st),
+                                        _ip(off), _ip(idx), ctypes.byref(g))
+    if rc != 0:
+        raise ValueError("oracle_make_clusters_par: invalid parameters")
+    return OracleResult(rho, nh, delta, is_seed, cluster, st.as_dict(), off, idx, gi)
+
+
+def parity_margins(coords, weights, params, rho, nthreads=0, band=1e-6):
+    """Points within `band` (relative) of a decision flip; see oracle_margins in clue_oracle.h."""
+    c = _soa(coords)
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    r = np.ascontiguousarray(rho, dtype=np.float32)
+    m = OracleMargins()
+    rc = lib().oracle_parity_margins(c.shape[0], c.shape[1], _fp(c), _fp(w), ctypes.byref(params), _fp(r),
+                                     nthreads, band, ctypes.byref(m))
+    if rc != 0:
+        raise ValueError("oracle_parity_margins: invalid arguments (Euclidean metric only)")
+    return m.as_dict()
 
 
 def make_clusters_omp(coords, weights, params, nthreads=0, release=False):

@@ -41,3 +41,9 @@ def test_status_translation():
     assert clue.STATUS[1] == "CLUE_ERR_INVALID_ARGUMENT"
     assert issubclass(clue.InvalidArgument, ValueError)
     assert issubclass(clue.OutOfRange, IndexError)
+
+
+def test_autotune_candidates_without_gpu():
+    # a host-only query: the points-per-tile values the autotuner may return
+    c = clue.autotune_candidates()
+    assert c == tuple(sorted(c)) and 16 in c and 128 in c and all(v > 0 for v in c)

@@ -10,49 +10,9 @@ import pytest
 import cluestering_alpaka3_amd as clue
 import oracle
 from cluestering_alpaka3_amd import datasets
+from parity_util import assert_same, run_both
 
 pytestmark = pytest.mark.gpu
-
-
-def run_both(coords, weights, dc, rhoc, dm=None, seed_dc=None, ppb=128, kernel=("flat", 0.5),
-             metric="euclidean", metric_params=None, wrapped=None, seed_mode=0, plan=None, tuning=None):
-    print(f"case: ndim={np.atleast_2d(coords).shape[0]} n={np.atleast_2d(coords).shape[1]} dc={dc} "
-          f"rhoc={rhoc} dm={dm} kernel={kernel} metric={metric} wrapped={wrapped}", flush=True)
-    own = plan is None
-    plan = plan or clue.Plan(0)
-    if tuning:
-        plan.set_tuning(**tuning)
-    try:
-        got = clue.cluster_points(coords, weights, dc, rhoc, dm, seed_dc, ppb, kernel, metric, metric_params,
-                                  wrapped, seed_mode, plan=plan)
-        tiles = plan.tiles()
-        grid = plan.grid()
-        stats = plan.stats()
-        delta = plan.delta()
-    finally:
-        if own:
-            plan.close()
-    p = oracle.make_params(dc, rhoc, dm, seed_dc, ppb, kernel, metric, metric_params, wrapped,
-                           seed_mode=seed_mode)
-    ref = oracle.make_clusters(coords, weights, p)
-    return got, ref, tiles, grid, stats, delta
-
-
-def assert_same(got, ref, tiles, grid, stats, delta):
-    assert grid["npd"] == ref.grid["npd"] and grid["ntiles"] == ref.grid["ntiles"]
-    np.testing.assert_array_equal(grid["minmax"], ref.grid["minmax"])
-    np.testing.assert_array_equal(grid["tile_size"], ref.grid["tile_size"])
-    np.testing.assert_array_equal(tiles[0], ref.tile_offsets)
-    np.testing.assert_array_equal(tiles[1], ref.tile_indexes)
-    np.testing.assert_array_equal(got.rho.view(np.int32), ref.rho.view(np.int32))
-    np.testing.assert_array_equal(delta.view(np.int32), ref.delta.view(np.int32))
-    np.testing.assert_array_equal(got.nearest_higher, ref.nh)
-    np.testing.assert_array_equal(got.is_seed, ref.is_seed)
-    np.testing.assert_array_equal(got.cluster, ref.cluster)
-    assert stats["seed_candidates"] == ref.stats["seed_candidates"]
-    assert stats["n_seeds"] == ref.stats["n_seeds"]
-    assert stats["n_clusters"] == ref.stats["n_stored_seeds"]
-    assert stats["n_dropped_seeds"] == ref.stats["n_dropped_seeds"]
 
 
 def test_fast_sqrt_exhaustive():
@@ -137,13 +97,15 @@ def test_sqrt_class_bounds():
         assert np.all((lo == 0) | (np.sqrt(np.maximum(below, f32(0))) < dd))
 
 
+# reduced sizes with the local density of the full workloads (box and blob count
+# scaled with n); the full sizes are in test_gpu_fullsize.py
 WORKLOAD_CASES = [("2d_16k_flat", 16384), ("2d_1m_gauss", 65536), ("3d_4m_exp", 65536),
-                  ("10d_256k_flat", 8192)]
+                  ("10d_256k_flat", 16384)]
 
 
 @pytest.mark.parametrize("name,n", WORKLOAD_CASES)
 def test_workload_parity(name, n):
-    ps = datasets.workload(name, n=n)
+    ps = datasets.workload_scaled(name, n)
     prm = datasets.WORKLOADS[name]
     assert_same(*run_both(ps.coords, ps.weights, prm["dc"], prm["rhoc"], prm["dm"], kernel=prm["kernel"]))
 
@@ -161,7 +123,7 @@ TUNINGS = [dict(nh_first_radius=0.05, morton_bits=5),
 @pytest.mark.parametrize("tuning", TUNINGS)
 @pytest.mark.parametrize("name,n", [("2d_1m_gauss", 50000), ("3d_4m_exp", 50000)])
 def test_tunings_do_not_change_results(name, n, tuning):
-    ps = datasets.workload(name, n=n, seed=11)
+    ps = datasets.workload_scaled(name, n, seed=11)
     prm = datasets.WORKLOADS[name]
     assert_same(*run_both(ps.coords, ps.weights, prm["dc"], prm["rhoc"], prm["dm"], kernel=prm["kernel"],
                           tuning=tuning))
@@ -257,6 +219,13 @@ def test_grow_only_capacity_across_calls():
 
 def test_wrapped_coordinates(ref_data):
     c, w = ref_data("opposite_angles")
+    # the reference's known answer (tests/test_clustering.cpp:133-150): wrapped
+    # coordinate 1, PeriodicEuclidean with periods {0, 2*pi_v<float>}, one cluster
+    two_pi = float(np.float32(2.0) * np.float32(np.pi))
+    got, ref, *rest = run_both(c, w, 0.2, 5.0, 0.2, wrapped=[0, 1], metric="periodic_euclidean",
+                               metric_params=[0.0, two_pi])
+    assert_same(got, ref, *rest)
+    assert int(got.cluster.max()) + 1 == 1
     assert_same(*run_both(c, w, 0.2, 5.0, 0.2, wrapped=[0, 1]))
     ps = datasets.blobs(5000, 2, 8, 0.0, 50.0, 1.0, 0.1, ("const", 1.0), 4242)
     assert_same(*run_both(ps.coords, ps.weights, 1.0, 3.0, 2.0, wrapped=[1, 1]))

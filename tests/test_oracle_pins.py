@@ -418,3 +418,115 @@ def test_config1_golden_regression(config1_golden):
     np.testing.assert_array_equal(r.rho, g["rho"])
     np.testing.assert_array_equal(r.cluster, g["cluster"])
     np.testing.assert_array_equal(r.nh, g["nh"])
+
+
+# ---------------------------------------------------------------------------
+# the parallel oracle and the margin accounting
+# ---------------------------------------------------------------------------
+def test_parallel_oracle_equals_serial_every_output():
+    cases = [(datasets.workload_scaled("2d_1m_gauss", 30000, seed=4), 2.0, 8.0, 4.0, ("gaussian", 0.0, 1.0, 1.0)),
+             (datasets.workload_scaled("3d_4m_exp", 30000, seed=4), 1.5, 6.0, 3.0, ("exponential", 1.0, 1.0)),
+             (datasets.workload_scaled("10d_256k_flat", 6000, seed=4), 3.0, 10.0, 6.0, ("flat", 0.5))]
+    for ps, dc, rhoc, dm, kernel in cases:
+        p = oracle.make_params(dc, rhoc, dm, kernel=kernel)
+        a = oracle.make_clusters(ps.coords, ps.weights, p)
+        b = oracle.make_clusters_par(ps.coords, ps.weights, p, 4)
+        for x, y in ((a.rho, b.rho), (a.delta, b.delta)):
+            np.testing.assert_array_equal(x.view(np.int32), y.view(np.int32))
+        for x, y in ((a.nh, b.nh), (a.is_seed, b.is_seed), (a.cluster, b.cluster),
+                     (a.tile_offsets, b.tile_offsets), (a.tile_indexes, b.tile_indexes)):
+            np.testing.assert_array_equal(x, y)
+        assert a.stats == b.stats
+
+
+def np_margins(coords, w, rho, dc, rhoc, dm, kernel, band=1e-6):
+    """O(N^2) restatement of oracle_parity_margins (search boxes of the tile grid,
+    float32 distances with the right fold, FP64 sums in numpy)."""
+    D, n = coords.shape
+    npd, T, mn, mx, ts = np_grid(coords)
+    bins = np.stack([np_bins(coords[d], mn[d], ts[d], npd) for d in range(D)])
+    ids = np.arange(n)
+    c64 = coords.astype(np.float64)
+    r64 = rho.astype(np.float64)
+    out = dict(rhoc_band=int(np.count_nonzero(np.abs(r64 - np.float32(rhoc)) <= band * np.float32(rhoc))),
+               nh_distance_ties=0, nh_distance_near_ties=0, nh_density_near_ties=0,
+               density_max_rel_err_fp64=0.0, density_above_band_fp64=0, dc_boundary_pairs=0)
+
+    def box(i, r):
+        m = np.ones(n, bool)
+        for d in range(D):
+            lo = np_bins(np.float32(coords[d, i] - np.float32(r)), mn[d], ts[d], npd)
+            hi = np_bins(np.float32(coords[d, i] + np.float32(r)), mn[d], ts[d], npd)
+            m &= (bins[d] >= lo) & (bins[d] <= hi)
+        return m
+
+    for i in range(n):
+        d32 = np.sqrt(np_distance2(coords, i, ids)).astype(np.float32)
+        d64 = np.sqrt(((c64 - c64[:, i:i + 1]) ** 2).sum(axis=0))
+        bx = box(i, dc)
+        inside = bx & (d32 <= np.float32(dc))
+        out["dc_boundary_pairs"] += int(np.count_nonzero(bx & ((d32 <= np.float32(dc)) != (d64 <= np.float32(dc)))))
+        if kernel[0] == "flat":
+            k = np.full(n, np.float64(np.float32(kernel[1])))
+        else:
+            a = d64 - np.float32(kernel[1])
+            sd = np.float64(np.float32(kernel[2]))
+            k = np.float64(np.float32(kernel[3])) * np.exp(-(a * a) / (2.0 * sd * sd))
+        k[i] = 1.0
+        acc = float((k * w.astype(np.float64))[inside].sum())
+        rel = abs(r64[i] - acc) / abs(acc) if acc != 0 else abs(r64[i])
+        out["density_max_rel_err_fp64"] = max(out["density_max_rel_err_fp64"], rel)
+        out["density_above_band_fp64"] += int(rel > band)
+        bm = box(i, dm)
+        higher = (rho > rho[i]) | ((rho == rho[i]) & (rho > 0) & (ids > i))
+        dh = np.sort(d32[bm & higher & (d32 <= np.float32(dm))])
+        if dh.size >= 2:
+            out["nh_distance_ties"] += int(dh[1] == dh[0])
+            out["nh_distance_near_ties"] += int(np.float64(dh[1]) - np.float64(dh[0]) <= band * np.float64(dh[0]))
+        reach = dh[0] if dh.size else np.float32(dm)
+        close = np.abs(r64 - r64[i]) <= band * np.maximum(np.abs(r64), abs(r64[i]))
+        out["nh_density_near_ties"] += int(np.any(bm & close & (ids != i) & (d32 <= reach)))
+    return out
+
+
+@pytest.mark.parametrize("kernel", [("flat", 0.5), ("gaussian", 0.0, 1.0, 1.0)])
+def test_margins_against_bruteforce(kernel):
+    # a jittered lattice: exact distance ties, equal densities, densities equal to rhoc
+    rng = np.random.default_rng(21)
+    x = (np.round(rng.random((2, 700)) * 24) * 0.5).astype(np.float32)
+    x[:, ::5] += (rng.random((2, 140)) * 0.05).astype(np.float32)
+    w = np.where(rng.random(700) < 0.7, 1.0, 2.0).astype(np.float32)
+    p0 = oracle.make_params(1.0, 1.0, 2.0, kernel=kernel)
+    rho = oracle.make_clusters(x, w, p0).rho
+    rhoc = float(np.median(rho))  # a density many points have exactly
+    p = oracle.make_params(1.0, rhoc, 2.0, kernel=kernel)
+    got = oracle.parity_margins(x, w, p, rho, 2)
+    want = np_margins(x, w, rho, 1.0, rhoc, 2.0, kernel)
+    assert got["rhoc_band"] > 0 and got["nh_distance_ties"] > 0 and got["nh_density_near_ties"] > 0
+    for key in want:
+        if key == "density_max_rel_err_fp64":
+            assert got[key] == pytest.approx(want[key], rel=1e-6, abs=1e-15)
+        else:
+            assert got[key] == want[key], key
+
+
+def test_margins_flat_unit_weights_exact():
+    # sums of 1 and halves are exact in float32: no density error against FP64
+    ps = datasets.workload("2d_16k_flat", n=4000)
+    p = oracle.make_params(1.5, 5.0, 3.0)
+    r = oracle.make_clusters(ps.coords, ps.weights, p)
+    m = oracle.parity_margins(ps.coords, ps.weights, p, r.rho, 2)
+    assert m["density_max_rel_err_fp64"] == 0.0 and m["density_above_band_fp64"] == 0
+    assert m["rhoc_band"] == int(np.count_nonzero(r.rho == np.float32(5.0)))
+
+
+def test_periodic_known_answer(ref_data):
+    # tests/test_clustering.cpp:133-150: opposite_angles with dc=0.2, rhoc=5, dm=0.2,
+    # setWrappedCoordinates(0, 1), PeriodicEuclidean{0, 2*pi_v<float>} -> one cluster
+    c, w = ref_data("opposite_angles")
+    two_pi = float(np.float32(2.0) * np.float32(np.pi))
+    p = oracle.make_params(0.2, 5.0, 0.2, metric="periodic_euclidean", metric_params=[0.0, two_pi],
+                           wrapped=[0, 1])
+    r = oracle.make_clusters(c, w, p)
+    assert r.stats["n_stored_seeds"] == 1
+    assert int(r.cluster.max()) + 1 == 1
