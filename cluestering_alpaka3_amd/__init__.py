@@ -137,6 +137,9 @@ _SIGNATURES = {
     "clue_make_clusters": ([_vp, _vp, _P(ClueParams), ctypes.c_int32, ctypes.c_int32, _P(_vp), _vp,
                             _P(ClueKernelDesc), _P(ClueMetricDesc), _P(ctypes.c_uint8), _vp, _vp,
                             _vp, _vp], ctypes.c_int),
+    "clue_make_clusters_noalloc": ([ctypes.c_int32, _vp, _P(ClueParams), ctypes.c_int32, ctypes.c_int32, _P(_vp),
+                                    _vp, _P(ClueKernelDesc), _P(ClueMetricDesc), _P(ctypes.c_uint8), _vp, _vp,
+                                    _vp, _vp], ctypes.c_int),
     "clue_make_clusters_batched": ([_vp, _vp, _P(ClueParams), ctypes.c_int32, ctypes.c_int32,
                                     _P(ctypes.c_int64), _P(_vp), _vp, _P(ClueKernelDesc),
                                     _P(ClueMetricDesc), _P(ctypes.c_uint8), _vp, _vp, _vp, _vp], ctypes.c_int),

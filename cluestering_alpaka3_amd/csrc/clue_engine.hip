@@ -1397,6 +1397,75 @@ struct BatchGraph {
 // most 2^D * nt tiles after the reference's rounding to npd^D; the bound is taken
 // with D at most 3 (larger grids of tiny high-dimensional events still grow the
 // buffers on their first call).  The pinned staging ring is sized for the tables.
+// ---------------------------------------------------------------------------
+// The no-allocation entry point's workspace: one array per GPU, in the code
+// object's device memory, held from the library's load on that GPU until the
+// process ends.  80 MiB hold a call of 262,144 points in any dimension 1..10 at
+// 16 points per tile or more (72 MiB at ten dimensions); larger calls are
+// refused.  Host code never uses the array's name as an address (on the host it
+// is only a stand-in): its device address comes from hipGetSymbolAddress, once
+// per device, and every buffer of the call is carved from that address.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = size_t{80} << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+
+struct StaticWorkspace {
+  std::mutex mu;          // one host caller at a time per device
+  unsigned char* base = nullptr;
+  clue_plan_s plan;       // its buffers point into the device's scratch array
+};
+StaticWorkspace g_static_ws[kMaxDevices];
+
+clue_status make_clusters_noalloc(int32_t device, hipStream_t st, const clue_params* prm, int32_t D, int32_t n,
+                                  const float* const* coords, const float* weights, const clue_kernel_desc* kernel,
+                                  const clue_metric_desc* metric, const uint8_t* wrapped, int32_t* cluster_out,
+                                  int32_t* is_seed_out, float* rho_out, int32_t* nh_out) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return fail(CLUE_ERR_RUNTIME, "No device available!");
+  if (device < 0 || device >= count || device >= kMaxDevices)
+    return fail(CLUE_ERR_OUT_OF_RANGE, "device index out of range");
+  if (n <= 0)
+    return fail(CLUE_ERR_INVALID_ARGUMENT,
+                "Number of points passed to PointsDevice constructor must be positive.");
+  StaticWorkspace& ws = g_static_ws[device];
+  std::lock_guard<std::mutex> lock(ws.mu);
+  CLUE_HIP(hipSetDevice(device));
+  if (!ws.base) {
+    void* ptr = nullptr;
+    CLUE_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(CLUE_ERR_RUNTIME, "hipGetSymbolAddress returned no address for the workspace");
+    ws.base = static_cast<unsigned char*>(ptr);
+    ws.plan.device = device;
+    ws.plan.arena = ws.base;
+    ws.plan.arena_bytes = kScratchBytes;
+  }
+  const int64_t offsets[2] = {0, n};
+  BatchCall bc;
+  // a fresh Clusterer per call: the event's own seed capacity (persistent = false)
+  clue_status rc = describe_batch(prm, D, 1, offsets, coords, weights, kernel, metric, wrapped, cluster_out,
+                                  is_seed_out, rho_out, nh_out, false, bc);
+  if (rc != CLUE_OK) return rc;
+  clue_plan_s* p = &ws.plan;
+  rc = reserve(p, D, n, bc.Ttot, 1, bc.n_chunks);
+  if (rc != CLUE_OK) return rc;
+  if (p->arena_used > p->arena_bytes)
+    return fail(CLUE_ERR_OUT_OF_RANGE,
+                "clue_make_clusters_noalloc: " + std::to_string(n) + " points in " + std::to_string(D) +
+                    " dimensions (" + std::to_string(bc.Ttot) + " tiles) need " + std::to_string(p->arena_used) +
+                    " bytes of workspace; the static workspace holds " + std::to_string(p->arena_bytes) +
+                    " (262,144 points in any dimension at 16 points per tile or more)");
+  const EventDesc ed = event_desc(D, 0, n, prm->points_per_tile, 0, 0);
+  k_single_event_tables<<<div_up(bc.n_chunks, kThreads), kThreads, 0, st>>>(ed, p->events.as<EventDesc>(),
+                                                                              p->chunks.as<Chunk>());
+  CLUE_LAUNCH_CHECK("single_event_tables");
+  // the workspace keeps the previous call's contents: clear the counters
+  k_fill_i32<<<1, 64, 0, st>>>(p->counters.as<int32_t>(), 0, 16);
+  CLUE_LAUNCH_CHECK("fill");
+  return enqueue_batch(p, st, prm, bc);
+}
+
 clue_status reserve_batches(clue_plan_t p, int32_t ndim, int32_t npoints, int32_t max_events) {
   if (!p || ndim < 1 || ndim > CLUE_MAX_DIM || npoints <= 0 || max_events <= 0)
     return fail(CLUE_ERR_INVALID_ARGUMENT, "invalid reserve size");
@@ -1580,6 +1649,15 @@ clue_status clue_make_clusters(clue_plan_t plan, void* stream, const clue_params
   return make_clusters(plan, static_cast<hipStream_t>(stream), params, ndim, n, coords_dev,
                        weights_dev, kernel, metric, wrapped, cluster_out, is_seed_out, rho_out,
                        nearest_higher_out);
+}
+
+clue_status clue_make_clusters_noalloc(int32_t device, void* stream, const clue_params* params, int32_t ndim,
+                                       int32_t n, const float* const* coords_dev, const float* weights_dev,
+                                       const clue_kernel_desc* kernel, const clue_metric_desc* metric,
+                                       const uint8_t* wrapped, int32_t* cluster_out, int32_t* is_seed_out,
+                                       float* rho_out, int32_t* nearest_higher_out) {
+  return make_clusters_noalloc(device, static_cast<hipStream_t>(stream), params, ndim, n, coords_dev, weights_dev,
+                               kernel, metric, wrapped, cluster_out, is_seed_out, rho_out, nearest_higher_out);
 }
 
 clue_status clue_make_clusters_batched(clue_plan_t plan, void* stream, const clue_params* params,

@@ -184,6 +184,27 @@ clue_status clue_make_clusters(clue_plan_t plan, void* stream, const clue_params
                                int32_t* cluster_out, int32_t* is_seed_out, float* rho_out,
                                int32_t* nearest_higher_out);
 
+/* clue_make_clusters without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Same inputs,
+ * outputs and results as clue_make_clusters on a fresh plan (a fresh Clusterer:
+ * the seed capacity is this call's own), but every intermediate buffer lives in
+ * one static 80 MiB device workspace per GPU, held from the library's load on
+ * that GPU until the process ends; no hipMalloc, hipFree, host-pinned
+ * allocation or event creation happens on any call, and no host copy (the event
+ * table is written by a kernel).  Calls needing more than the workspace
+ * (beyond 262,144 points in any dimension 1..10 at 16 points per tile or more)
+ * return CLUE_ERR_OUT_OF_RANGE.  The workspace keeps the last call's contents
+ * and is shared by every call on that GPU: calls in one stream are ordered,
+ * calls that overlap on one GPU (other streams, other threads' streams) race
+ * and must not be made.  Enqueued on `stream` on `device`; nothing is
+ * synchronised. */
+clue_status clue_make_clusters_noalloc(int32_t device, void* stream, const clue_params* params, int32_t ndim,
+                                       int32_t n, const float* const* coords_dev, const float* weights_dev,
+                                       const clue_kernel_desc* kernel, const clue_metric_desc* metric,
+                                       const uint8_t* wrapped, int32_t* cluster_out, int32_t* is_seed_out,
+                                       float* rho_out, int32_t* nearest_higher_out);
+
 /* Batched clustering of independent events in one set of launches.  No
  * reference counterpart: the reference clusters one event per make_clusters
  * call, reusing the Clusterer's scratch (SetupTiles.hpp:28-37,
