@@ -38,7 +38,9 @@ def workload_results(name):
         print(f"full-size: {name} n={ps.n} ndim={ps.ndim} sha256={ps.sha256()[:16]}", flush=True)
         eng = run_engine(ps.coords, ps.weights, prm["dc"], prm["rhoc"], prm["dm"], kernel=prm["kernel"])
         p = oracle.make_params(prm["dc"], prm["rhoc"], prm["dm"], kernel=prm["kernel"])
+        print(f"full-size: {name} engine done; oracle on {ORACLE_THREADS} threads", flush=True)
         ref = oracle.make_clusters_par(ps.coords, ps.weights, p, ORACLE_THREADS)
+        print(f"full-size: {name} oracle done; margins", flush=True)
         margins = oracle.parity_margins(ps.coords, ps.weights, p, ref.rho, ORACLE_THREADS, BAND)
         _cache[name] = (ps, prm, eng, ref, margins)
     return _cache[name]
@@ -60,6 +62,7 @@ def test_fullsize_against_libm_exp(name):
     ps, prm, eng, ref, margins = workload_results(name)
     got = eng[0]
     p = oracle.make_params(prm["dc"], prm["rhoc"], prm["dm"], kernel=prm["kernel"], exp_mode=oracle.EXP_LIBM)
+    print(f"full-size: {name} libm-exp oracle", flush=True)
     lib = oracle.make_clusters_par(ps.coords, ps.weights, p, ORACLE_THREADS, with_tiles=False)
     rel = np.abs(got.rho.astype(np.float64) - lib.rho) / np.abs(lib.rho.astype(np.float64))
     differing = int(np.count_nonzero(got.rho != lib.rho))
