@@ -278,6 +278,10 @@ def batched_bench(clue, datasets, args):
     return res
 
 
+# this process's CPU set, read before any OpenMP runtime binds the main thread
+AFFINITY_CORES = len(os.sched_getaffinity(0))
+
+
 def host_description(threads):
     """CPU model, core counts and the OpenMP binding of the CPU baseline."""
     model = ""
@@ -288,7 +292,7 @@ def host_description(threads):
                 break
     except OSError:
         pass
-    return {"cpu_model": model, "os_cpu_count": os.cpu_count(), "affinity_cores": len(os.sched_getaffinity(0)),
+    return {"cpu_model": model, "os_cpu_count": os.cpu_count(), "affinity_cores": AFFINITY_CORES,
             "omp_threads": threads, "omp_proc_bind": os.environ.get("OMP_PROC_BIND", ""),
             "omp_places": os.environ.get("OMP_PLACES", "")}
 
@@ -307,7 +311,7 @@ def cpu_baseline(ps, prm, args, gpu):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
-    threads = args.cpu_threads or max(1, min(16, len(os.sched_getaffinity(0))))
+    threads = args.cpu_threads or max(1, min(16, AFFINITY_CORES))
     p = oracle.make_params(prm["dc"], prm["rhoc"], prm["dm"], kernel=prm["kernel"])
     lo = ps.coords.min(axis=1)
     hi = ps.coords.max(axis=1)
