@@ -167,6 +167,27 @@ __device__ __forceinline__ float engine_expf_nonpos(float x, const double* table
   return static_cast<float>(__builtin_ldexp(__builtin_fma(t, p, t), m));
 }
 
+// The correctly rounded float exp of x in [-14, 0], a shorter evaluation for
+// the kernels' in-radius terms once the host has bounded their argument: the
+// reduction with one constant (ln2/64 rounded to double: |k| <= 1293 keeps its
+// error below 2^-49 of the result) and exp(r) - 1 as
+// r + r^2 (1/2 + r/6 + r^2 (1 + r/5) / 24), whose additive constants 0.5 and 1.0
+// are inline operands (no register copies).  Checked on every float in
+// [-14, 0] against long double rounded once to float: the same bits as
+// engine_expf, the correctly rounded value.
+__device__ __forceinline__ float engine_expf_unit(float x, const double* table = kExp2Table) {
+  const double xd = static_cast<double>(x);
+  const double kd = __builtin_rint(xd * 0x1.71547652b82fep+6);
+  const int k = static_cast<int>(kd);
+  const double r = __builtin_fma(-kd, 0x1.62e42fefa39efp-7, xd);
+  const double h = __builtin_fma(r, 1.0 / 6.0, 0.5);
+  const double q = __builtin_fma(r, 0.2, 1.0) * (1.0 / 24.0);
+  const double r2 = r * r;
+  const double p = __builtin_fma(r2, __builtin_fma(r2, q, h), r);
+  const double t = table[k & 63];
+  return static_cast<float>(__builtin_ldexp(__builtin_fma(t, p, t), k >> 6));
+}
+
 // Correctly rounded sqrt of x in [2^-96, FLT_MAX]: the hardware estimate (within
 // one ulp) corrected by the signs of the residuals of its two neighbours -- the
 // IEEE sqrtf lowering without its small-input scaling and special-value paths.
@@ -194,7 +215,9 @@ __device__ __forceinline__ float quotient_via_rcp(float num, double rcp_den) {
 enum { KFLAT = 0, KGAUSS = 1, KEXP = 2 };
 // rcp_den: 1 / ((2*sigma)*sigma) rounded to double (Gaussian only).
 // NONPOS: the exp argument is known to be <= 0 and not NaN (checked on the host).
-template <int KERNEL, bool NONPOS = false>
+// UNIT: also the amplitude is exactly 1 (amplitude * e == e) and the argument is
+// known to lie in [-14, 0] for every in-radius distance (checked on the host).
+template <int KERNEL, bool NONPOS = false, bool UNIT = false>
 __device__ __forceinline__ float kernel_value(float dist, float p0, float p1, float p2, double rcp_den,
                                               const double* table = kExp2Table) {
   if constexpr (KERNEL == KFLAT) {
@@ -203,9 +226,11 @@ __device__ __forceinline__ float kernel_value(float dist, float p0, float p1, fl
     const float a = dist - p0;
     const float num = (-a) * a;  // -(d-mu)*(d-mu), divided by (2*sigma)*sigma
     const float x = quotient_via_rcp(num, rcp_den);
+    if constexpr (UNIT) return engine_expf_unit(x, table);
     return p2 * (NONPOS ? engine_expf_nonpos(x, table) : engine_expf(x));
   } else {
     const float x = (-p0) * dist;
+    if constexpr (UNIT) return engine_expf_unit(x, table);
     return p1 * (NONPOS ? engine_expf_nonpos(x, table) : engine_expf(x));
   }
 }

@@ -707,6 +707,7 @@ __global__ void k_selftest(int op, const float* a, const float* b, float* out, i
     case 3: out[i] = sqrt_lt_bound(x); break;
     case 4: out[i] = quotient_via_rcp(x, 1.0 / static_cast<double>(b[i])); break;
     case 5: out[i] = engine_expf_nonpos(x); break;
+    case 8: out[i] = engine_expf_unit(x); break;
     case 6:
     case 7: {
       float lo, hi;
@@ -1276,6 +1277,21 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
   a.exp_nonpos = (kd.type == CLUE_KERNEL_GAUSSIAN && std::isfinite(a.kden_rcp) && a.kden_rcp > 0.0 &&
                   std::isfinite(kd.params[0])) ||
                  (kd.type == CLUE_KERNEL_EXPONENTIAL && kd.params[0] >= 0.f && std::isfinite(kd.params[0]));
+  // in-radius distances lie in [0, dc]: bound the exp argument from below (with
+  // room for the float roundings of d - mu, its square, the quotient and -lambda*d)
+  if (a.exp_nonpos) {
+    const double dc = prm->dc;
+    double lowest;
+    if (kd.type == CLUE_KERNEL_GAUSSIAN) {
+      const double mu = std::fabs(static_cast<double>(kd.params[0]));
+      const double m = std::max(mu, dc + mu);
+      lowest = m * m * a.kden_rcp;
+    } else {
+      lowest = static_cast<double>(kd.params[0]) * dc;
+    }
+    const float amp = kd.type == CLUE_KERNEL_GAUSSIAN ? kd.params[2] : kd.params[1];
+    a.exp_unit = amp == 1.0f && lowest * 1.001 <= 14.0;
+  }
   for (int d = 0; d < CLUE_MAX_DIM; ++d) a.mp[d] = md.params[d];
   if (blocks) {
     for (int d = 0; d < D; ++d) {

@@ -109,7 +109,8 @@ __global__ __launch_bounds__(kThreads) void k_box_class(const PassArgs a, float 
 
 // QNH: the lanes take their queries in Morton order (spatially coherent waves,
 // so the lanes of a wave share most of their search box); otherwise in tile order.
-template <int D, int METRIC, int KERNEL, bool UNION, bool QNH>
+// UNIT: Euclidean metric, exp argument in [-110, 0] and amplitude 1 (exp_unit).
+template <int D, int METRIC, int KERNEL, bool UNION, bool QNH, bool UNIT = false>
 __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
   // the exponential's 2^(j/64) table, read through LDS by the kernel evaluations
   __shared__ double s_exp2[KERNEL == KFLAT ? 1 : 64];
@@ -167,9 +168,9 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
         if (s2 <= a.thr) {
           float k = 1.f;
           if (j != s) {
-            if (a.exp_nonpos && s2 >= 0x1p-96f) {
-              k = kernel_value<KERNEL, true>(sqrt_rn_normal(s2), a.kp[0], a.kp[1], a.kp[2], a.kden_rcp,
-                                             s_exp2);
+            if ((UNIT || a.exp_nonpos) && s2 >= 0x1p-96f) {
+              k = kernel_value<KERNEL, true, UNIT>(sqrt_rn_normal(s2), a.kp[0], a.kp[1], a.kp[2], a.kden_rcp,
+                                                   s_exp2);
             } else {
               k = kernel_value<KERNEL>(__builtin_sqrtf(s2), a.kp[0], a.kp[1], a.kp[2], a.kden_rcp);
             }
@@ -919,10 +920,18 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
     a.qorder = p->qorder.as<int32_t>();
   }
   if (blocks) {
+    bool unit = false;
+    if constexpr (METRIC == MEUCLID) unit = a.exp_unit != 0;
     switch (kernel_type) {
       case KFLAT: k_density<D, METRIC, KFLAT, U, true><<<grid, kThreads, 0, st>>>(a); break;
-      case KGAUSS: k_density<D, METRIC, KGAUSS, U, true><<<grid, kThreads, 0, st>>>(a); break;
-      case KEXP: k_density<D, METRIC, KEXP, U, true><<<grid, kThreads, 0, st>>>(a); break;
+      case KGAUSS:
+        if (unit) k_density<D, METRIC, KGAUSS, U, true, METRIC == MEUCLID><<<grid, kThreads, 0, st>>>(a);
+        else k_density<D, METRIC, KGAUSS, U, true><<<grid, kThreads, 0, st>>>(a);
+        break;
+      case KEXP:
+        if (unit) k_density<D, METRIC, KEXP, U, true, METRIC == MEUCLID><<<grid, kThreads, 0, st>>>(a);
+        else k_density<D, METRIC, KEXP, U, true><<<grid, kThreads, 0, st>>>(a);
+        break;
       default: return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown convolutional kernel type");
     }
   } else {
