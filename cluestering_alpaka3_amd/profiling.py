@@ -16,6 +16,15 @@
         against one wave64 VALU instruction per 4 cycles on each of 1024 SIMDs at
         2.4 GHz
 
+    python -m cluestering_alpaka3_amd.profiling occupancy <pmc.csv>
+        per-kernel mean of rocprofv3's derived OccupancyPercent (resident waves as
+        a share of 32 per CU over the kernel's active time) and
+        MeanOccupancyPerActiveCU (waves per CU while the CU is busy)
+
+    python -m cluestering_alpaka3_amd.profiling resources <remarks.txt> [--match k_density ...]
+        per-kernel VGPRs, SGPRs, LDS bytes, scratch and the compiler's waves per
+        SIMD, from hipcc -Rpass-analysis=kernel-resource-usage output
+
     python -m cluestering_alpaka3_amd.profiling traffic <fetch.csv> <write.csv> [--copy-mib 64] [--json OUT]
         HBM bytes per dispatch from two --pmc runs (FETCH_SIZE, then WRITE_SIZE),
         each scaled by what the same counter reads for the device-to-device copy
@@ -104,7 +113,34 @@ def sq_counters(path):
     return {k: {c: sum(v) / len(v) for c, v in cs.items()} for k, cs in agg.items()}
 
 
-def run_batched(events, steps, ppb, nh_first_radius):
+def occupancy(path):
+    """Mean of each derived occupancy counter per dispatch, per kernel (full template name)."""
+    agg = defaultdict(lambda: defaultdict(list))
+    for r in csv.DictReader(open(path)):
+        agg[r["Kernel_Name"].split("(")[0].replace("void ", "")][r["Counter_Name"]].append(
+            float(r["Counter_Value"]))
+    return {k: {c: sum(v) / len(v) for c, v in cs.items()} for k, cs in agg.items()}
+
+
+def resources(path, match):
+    """Kernel resource usage from the compiler's kernel-resource-usage remarks."""
+    rows = {}
+    name = None
+    for line in open(path):
+        m = re.search(r"Function Name: (\S+)", line)
+        if m:
+            name = m.group(1)
+            rows[name] = {}
+            continue
+        m = re.search(r"remark:\s+([A-Za-z ]+?)(?: \[[^\]]*\])?: (\d+)", line)
+        if name and m:
+            rows[name][m.group(1).strip()] = int(m.group(2))
+    keep = {k: v for k, v in rows.items() if not match or any(s in k for s in match)}
+    return keep
+
+
+def run_batched(events, steps, ppb, nh_first_radius, xcd_remap=None, pending_group=None, density_order=None,
+                copy_mib=64):
     import numpy as np
 
     import cluestering_alpaka3_amd as clue
@@ -126,7 +162,17 @@ def run_batched(events, steps, ppb, nh_first_radius):
     params = clue.make_params(1.3, 2.0, 2.6, ppb=ppb)
     tuning = plan.get_tuning()
     tuning["nh_first_radius"] = nh_first_radius
+    for key, v in (("xcd_remap", xcd_remap), ("pending_group", pending_group), ("density_order", density_order)):
+        if v is not None:
+            tuning[key] = v
     plan.set_tuning(**tuning)
+    if copy_mib > 0:  # the device-to-device copy of known size that calibrates FETCH_SIZE / WRITE_SIZE
+        src = clue.DeviceArray(((copy_mib << 20) // 4,), np.float32)
+        dst = clue.DeviceArray(((copy_mib << 20) // 4,), np.float32)
+        clue.check(clue.lib().clue_memcpy_d2d(dst.ptr, src.ptr, copy_mib << 20, stream.handle))
+        stream.synchronize()
+        src.free()
+        dst.free()
     kd, md = clue.kernel_desc(("flat", 0.5)), clue.metric_desc("euclidean")
     for step in range(steps):
         plan.make_clusters_batched_device(stream, params, 2, off, cp, dw.ptr, kd, md, None, *[o.ptr for o in outs])
@@ -142,6 +188,10 @@ def main():
     b.add_argument("--steps", type=int, default=5)
     b.add_argument("--ppb", type=int, default=64)
     b.add_argument("--nh-first-radius", type=float, default=1.0)
+    b.add_argument("--xcd-remap", type=int)
+    b.add_argument("--pending-group", type=int)
+    b.add_argument("--density-order", type=int)
+    b.add_argument("--copy-mib", type=int, default=64)
     s = sub.add_parser("stats")
     s.add_argument("csv")
     s.add_argument("--top", type=int, default=25)
@@ -149,6 +199,11 @@ def main():
     q.add_argument("csv")
     q.add_argument("--stats-csv")
     q.add_argument("--top", type=int, default=12)
+    o = sub.add_parser("occupancy")
+    o.add_argument("csv")
+    r = sub.add_parser("resources")
+    r.add_argument("remarks")
+    r.add_argument("--match", nargs="*", default=[])
     t = sub.add_parser("traffic")
     t.add_argument("fetch_csv")
     t.add_argument("write_csv")
@@ -156,7 +211,8 @@ def main():
     t.add_argument("--json")
     a = ap.parse_args()
     if a.cmd == "run-batched":
-        run_batched(a.events, a.steps, a.ppb, a.nh_first_radius)
+        run_batched(a.events, a.steps, a.ppb, a.nh_first_radius, a.xcd_remap, a.pending_group, a.density_order,
+                    a.copy_mib)
     elif a.cmd == "sq":
         cnt = sq_counters(a.csv)
         dur = {}
@@ -175,6 +231,18 @@ def main():
                 rate = c.get("SQ_INSTS_VALU", 0) / dur[k]
                 line += f" {rate:13.3e} {rate / VALU_PEAK:8.3f}"
             print(line)
+    elif a.cmd == "occupancy":
+        res = occupancy(a.csv)
+        print(f"{'kernel':64s} {'OccupancyPercent':>17s} {'waves/active CU':>16s}")
+        for k, c in sorted(res.items(), key=lambda kv: kv[0]):
+            print(f"{k[:64]:64s} {c.get('OccupancyPercent', float('nan')):17.1f} "
+                  f"{c.get('MeanOccupancyPerActiveCU', float('nan')):16.2f}")
+    elif a.cmd == "resources":
+        res = resources(a.remarks, a.match)
+        print(f"{'kernel (mangled)':72s} {'VGPRs':>6s} {'SGPRs':>6s} {'LDS_B':>6s} {'scratch':>8s} {'waves/SIMD':>10s}")
+        for k, v in res.items():
+            print(f"{k[:72]:72s} {v.get('VGPRs', -1):6d} {v.get('TotalSGPRs', -1):6d} "
+                  f"{v.get('LDS Size', -1):6d} {v.get('ScratchSize', -1):8d} {v.get('Occupancy', -1):10d}")
     elif a.cmd == "stats":
         print(f"{'kernel':100s} {'calls':>6s} {'avg_us':>10s} {'share%':>7s}")
         for name, calls, avg, pct in kernel_stats(a.csv)[: a.top]:
