@@ -261,6 +261,122 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
   }
 }
 
+// Density for small calls: one wave per query.  The 64 lanes take 64
+// consecutive candidates of the query's visit order (the tiles of its search
+// box in lexicographic order, each tile's points in CSR order) and evaluate
+// their terms together; the in-radius terms are then added to the density one
+// lane after another, in lane order -- the reference's order of the sum
+// (ClusteringKernels.hpp:38-44).  With few queries the lane-per-query pass
+// leaves most of the GPU idle and its time is the longest query's chain of
+// terms; here that chain is one add per in-radius term.
+template <int D, int METRIC, int KERNEL, bool QNH, bool UNIT>
+__global__ __launch_bounds__(kThreads) void k_density_wave(const PassArgs a) {
+  __shared__ double s_exp2[KERNEL == KFLAT ? 1 : 64];
+  if constexpr (KERNEL != KFLAT) {
+    if (threadIdx.x < 64) s_exp2[threadIdx.x] = kExp2Table[threadIdx.x];
+    __syncthreads();
+  }
+  const int pos = uniform(static_cast<int>(blockIdx.x) * (kThreads / kWave) + static_cast<int>(threadIdx.x / kWave));
+  if (pos >= a.n) return;  // the whole wave
+  const int lane = __lane_id();
+  float q[D];
+  float w_self;
+  int s, pid;
+  if constexpr (QNH) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) q[d] = as_const(a.xn[d])[pos];
+    w_self = as_const(a.wn)[pos];
+    s = as_const(a.sn)[pos];
+    pid = as_const(a.pn)[pos];
+  } else {
+#pragma unroll
+    for (int d = 0; d < D; ++d) q[d] = as_const(a.xs[d])[pos];
+    w_self = as_const(a.ws)[pos];
+    s = pos;
+    pid = as_const(a.perm)[pos];
+  }
+  const int e = as_const(a.pev)[pid];
+  const EvGrid<D> g = load_event<D>(a.ev, e, a.wrap);
+  int lo[D], hi[D];
+  search_box<D>(q, a.radius, g, lo, hi);
+  constexpr bool kUsesHW = (METRIC == MEUCLID) && (KERNEL == KFLAT);
+  float rho = 0.f;
+  bool poisoned = false;
+  cptr<int32_t> OFF = as_const(a.offsets);
+  int b[D];
+  bool more = true;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    more = more && lo[d] <= hi[d];
+    b[d] = lo[d];
+  }
+  while (more) {
+    const int t = uniform(tile_index<D>(b, g));
+    const int beg = OFF[t], end = OFF[t + 1];
+    const bool nonfinite = as_const(a.tnf)[t] != 0;
+    for (int j0 = beg; j0 < end; j0 += kWave) {
+      const int j = j0 + lane;
+      const bool in = j < end;
+      float term = 0.f;
+      bool inr = false, poison = false;
+      if (in) {
+        float c[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) c[d] = a.xs[d][j];
+        const float wj = kUsesHW ? 0.f : a.ws[j];
+        if constexpr (METRIC == MEUCLID) {
+          const float s2 = squared_distance<D>(q, c);
+          inr = s2 <= a.thr;
+          if (inr) {
+            if constexpr (KERNEL == KFLAT) {
+              term = (j == s) ? w_self : a.hw[j];
+            } else {
+              float k = 1.f;
+              if (j != s) {
+                if ((UNIT || a.exp_nonpos) && s2 >= 0x1p-96f) {
+                  k = kernel_value<KERNEL, true, UNIT>(sqrt_rn_normal(s2), a.kp[0], a.kp[1], a.kp[2], a.kden_rcp,
+                                                       s_exp2);
+                } else {
+                  k = kernel_value<KERNEL>(__builtin_sqrtf(s2), a.kp[0], a.kp[1], a.kp[2], a.kden_rcp);
+                }
+              }
+              term = k * wj;
+            }
+          }
+        } else {
+          const float dist = metric_distance<D, METRIC>(q, c, a.mp);
+          inr = dist <= a.radius;
+          if (inr) {
+            const float k = (j == s) ? 1.f : kernel_value<KERNEL>(dist, a.kp[0], a.kp[1], a.kp[2], a.kden_rcp);
+            term = k * wj;
+          }
+        }
+        // out-of-radius candidates with an infinite or NaN weight add 0 * w = NaN
+        if (nonfinite && !inr) poison = !__builtin_isfinite(a.ws[j]);
+      }
+      uint64_t m = __ballot(inr);
+      while (m) {
+        const int l = __builtin_ctzll(m);
+        m &= m - 1;
+        rho = rho + readlane_f32(term, l);
+      }
+      poisoned = poisoned || __ballot(poison) != 0;
+    }
+    int d = D - 1;
+    for (; d >= 0; --d) {
+      if (++b[d] <= hi[d]) break;
+      b[d] = lo[d];
+    }
+    more = d >= 0;
+  }
+  if (poisoned) rho = __builtin_nanf("");
+  if (lane == 0) {
+    a.rho_s_out[s] = rho;
+    a.rho_out[pid] = rho;
+    a.hk_out[QNH ? pos : s] = rank_key(rho, pid);
+  }
+}
+
 template <int D, int METRIC, bool UNION>
 __global__ __launch_bounds__(kThreads) void k_nearest_higher(const PassArgs a) {
   const int s = logical_block((a.xcd >> 1) & 1) * kThreads + threadIdx.x;
@@ -343,6 +459,84 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher(const PassArgs a) {
     a.nh_out[id_i] = nh;
     a.delta_out[id_i] = delta;
     if (nh == -1) atomicAdd(&a.sc[my_e], 1);
+  }
+}
+
+// Nearest-higher by the tile walk with one wave per query (more than three
+// dimensions, other metrics, wrapped grids).  The lanes take 64 consecutive
+// candidates of the query's visit order; the batch's least distance among its
+// higher candidates within dm replaces the query's best only when strictly
+// smaller, and among the lanes at that distance the first in lane order wins --
+// together the first strict minimum in the visit order of
+// for_recursion_nearest_higher (ClusteringKernels.hpp:109-166).  Each lane's
+// distance is the metric's float value, as the reference compares it.
+template <int D, int METRIC>
+__global__ __launch_bounds__(kThreads) void k_nearest_higher_wave(const PassArgs a) {
+  const int pos = uniform(static_cast<int>(blockIdx.x) * (kThreads / kWave) + static_cast<int>(threadIdx.x / kWave));
+  if (pos >= a.n) return;  // the whole wave
+  const int lane = __lane_id();
+  float q[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) q[d] = as_const(a.xs[d])[pos];
+  const uint64_t key_i = uniform(as_const(a.hk)[pos]);
+  const uint64_t thr_i = higher_threshold(key_i);
+  const int32_t id_i = static_cast<int32_t>(key_i & 0xffffffffu);
+  const int e = as_const(a.pev)[id_i];
+  const EvGrid<D> g = load_event<D>(a.ev, e, a.wrap);
+  int lo[D], hi[D];
+  search_box<D>(q, a.radius, g, lo, hi);
+  float delta = FLT_MAX;
+  int32_t nh = -1;
+  cptr<int32_t> OFF = as_const(a.offsets);
+  int b[D];
+  bool more = true;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    more = more && lo[d] <= hi[d];
+    b[d] = lo[d];
+  }
+  while (more) {
+    const int t = uniform(tile_index<D>(b, g));
+    const int beg = OFF[t], end = OFF[t + 1];
+    for (int j0 = beg; j0 < end; j0 += kWave) {
+      const int j = j0 + lane;
+      float dist = __builtin_inff();
+      uint64_t key_j = 0;
+      if (j < end) {
+        key_j = a.hk[j];
+        if (key_j > thr_i) {
+          float c[D];
+#pragma unroll
+          for (int d = 0; d < D; ++d) c[d] = a.xs[d][j];
+          float dj;
+          if constexpr (METRIC == MEUCLID) {
+            const float s2 = squared_distance<D>(q, c);
+            dj = s2 <= a.thr ? __builtin_sqrtf(s2) : __builtin_inff();
+          } else {
+            dj = metric_distance<D, METRIC>(q, c, a.mp);
+            dj = dj <= a.radius ? dj : __builtin_inff();
+          }
+          dist = dj;
+        }
+      }
+      const float dmin = wave_min_f32(dist);
+      if (dmin < delta) {  // uniform
+        const int l = __builtin_ctzll(__ballot(dist == dmin));
+        delta = dmin;
+        nh = static_cast<int32_t>(readlane_u64(key_j, l) & 0xffffffffu);
+      }
+    }
+    int d = D - 1;
+    for (; d >= 0; --d) {
+      if (++b[d] <= hi[d]) break;
+      b[d] = lo[d];
+    }
+    more = d >= 0;
+  }
+  if (lane == 0) {
+    a.nh_out[id_i] = nh;
+    a.delta_out[id_i] = delta;
+    if (nh == -1) atomicAdd(&a.sc[e], 1);
   }
 }
 
@@ -907,6 +1101,103 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
     a.qperm = vals;
     return r2;
   };
+  // the nearest-higher pass (after either density pass)
+  // one wave per query (k_density_wave, k_nearest_higher_wave) for small calls
+  const bool wave_mode = D > p->tuning.wave_dims || (p->tuning.small_call_points > 0 && a.n <= p->tuning.small_call_points);
+  auto nearest_higher = [&]() -> clue_status {
+    a.radius = dm;
+    if constexpr (!U) {
+      if (!wave_mode) {
+        rc = order_by_box(dm, false);  // the first tile does not pay here (measured)
+        if (rc != CLUE_OK) return rc;
+      }
+    }
+    a.thr = sqrt_le_bound(dm);
+    rc = mark(p, 5, st);
+    if (rc != CLUE_OK) return rc;
+    if constexpr (METRIC == MEUCLID && D <= 3) {
+      if (blocks) {
+        a.thr1 = sqrt_le_bound(dm * p->tuning.nh_first_radius);
+        a.box1 = dm * p->tuning.nh_first_radius * (1.f + 1.f / 1024.f);
+        // second phase levels: twice the radius each time, dm last
+        a.nlvl = 0;
+        a.lvl_prev0 = std::min(a.thr1, a.thr);
+        float r = dm * p->tuning.nh_first_radius;
+        if (r < dm) {
+          while (a.nlvl < 7) {
+            r *= 2.f;
+            if (!(r < dm)) break;
+            a.lvl[a.nlvl++] = sqrt_le_bound(r);
+          }
+          a.lvl[a.nlvl++] = a.thr;
+        }
+        rc = launch_nh_maxkeys(p, a, st);
+        if (rc != CLUE_OK) return rc;
+        k_nearest_higher_blocks<D><<<grid, kThreads, 0, st>>>(a);
+        CLUE_LAUNCH_CHECK("nearest_higher_blocks");
+        rc = scan_exclusive(a.pending_flag, a.pending_flag, a.n, p->part.as<int32_t>(), st);
+        if (rc != CLUE_OK) return rc;
+        rc = launch_compact(a.pending_flag, a.n, a.pending, st);
+        if (rc != CLUE_OK) return rc;
+        a.pending_count = a.pending_flag + a.n;
+        // enough waves to fill the GPU; surplus waves find the pending list exhausted
+        // default grid of the persistent second phase: 8192 workgroups for one
+        // event of 2^20 points or more (2D 1M points: 0.51 -> 0.49 ms, 3D 4M: 7.43
+        // -> 7.31 ms), 2048 otherwise (the 4096-event batch: 7.08 ms against 7.23 ms
+        // with 8192; small served requests lose more than they gain)
+        const int pblocks = p->tuning.pending_blocks > 0 ? p->tuning.pending_blocks
+                                                         : (a.n_events == 1 && a.n >= (1 << 20) ? 8192 : 2048);
+        const int nblk = static_cast<int>(std::min<int64_t>(pblocks, div_up(a.n, kThreads / kWave)));
+        switch (p->tuning.pending_group) {
+          case 8: k_nh_pending<D, 8><<<nblk, kThreads, 0, st>>>(a); break;
+          case 16: k_nh_pending<D, 16><<<nblk, kThreads, 0, st>>>(a); break;
+          case 32: k_nh_pending<D, 32><<<nblk, kThreads, 0, st>>>(a); break;
+          default: k_nh_pending<D, 64><<<nblk, kThreads, 0, st>>>(a); break;
+        }
+        CLUE_LAUNCH_CHECK("nearest_higher_pending");
+        return CLUE_OK;
+      }
+    }
+    if (wave_mode) {
+      k_nearest_higher_wave<D, METRIC><<<div_up(a.n, kThreads / kWave), kThreads, 0, st>>>(a);
+      CLUE_LAUNCH_CHECK("nearest_higher_wave");
+      return CLUE_OK;
+    }
+    k_nearest_higher<D, METRIC, U><<<grid, kThreads, 0, st>>>(a);
+    CLUE_LAUNCH_CHECK("nearest_higher");
+    return CLUE_OK;
+  };
+  // small calls: one wave per query (k_density_wave), the same sums in the same order
+  if (wave_mode) {
+    const int wgrid = static_cast<int>(div_up(a.n, kThreads / kWave));
+    if constexpr (METRIC == MEUCLID) {
+      if (blocks) {
+        const bool unit = a.exp_unit != 0;
+        switch (kernel_type) {
+          case KFLAT: k_density_wave<D, METRIC, KFLAT, true, false><<<wgrid, kThreads, 0, st>>>(a); break;
+          case KGAUSS:
+            if (unit) k_density_wave<D, METRIC, KGAUSS, true, true><<<wgrid, kThreads, 0, st>>>(a);
+            else k_density_wave<D, METRIC, KGAUSS, true, false><<<wgrid, kThreads, 0, st>>>(a);
+            break;
+          case KEXP:
+            if (unit) k_density_wave<D, METRIC, KEXP, true, true><<<wgrid, kThreads, 0, st>>>(a);
+            else k_density_wave<D, METRIC, KEXP, true, false><<<wgrid, kThreads, 0, st>>>(a);
+            break;
+          default: return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown convolutional kernel type");
+        }
+        CLUE_LAUNCH_CHECK("density_wave");
+        return nearest_higher();
+      }
+    }
+    switch (kernel_type) {
+      case KFLAT: k_density_wave<D, METRIC, KFLAT, false, false><<<wgrid, kThreads, 0, st>>>(a); break;
+      case KGAUSS: k_density_wave<D, METRIC, KGAUSS, false, false><<<wgrid, kThreads, 0, st>>>(a); break;
+      case KEXP: k_density_wave<D, METRIC, KEXP, false, false><<<wgrid, kThreads, 0, st>>>(a); break;
+      default: return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown convolutional kernel type");
+    }
+    CLUE_LAUNCH_CHECK("density_wave");
+    return nearest_higher();
+  }
   if constexpr (!U) {
     rc = order_by_box(dc, true);
     if (rc != CLUE_OK) return rc;
@@ -943,60 +1234,7 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
     }
   }
   CLUE_LAUNCH_CHECK("density");
-  a.radius = dm;
-  if constexpr (!U) {
-    rc = order_by_box(dm, false);  // the first tile does not pay here (measured)
-    if (rc != CLUE_OK) return rc;
-  }
-  a.thr = sqrt_le_bound(dm);
-  rc = mark(p, 5, st);
-  if (rc != CLUE_OK) return rc;
-  if constexpr (METRIC == MEUCLID && D <= 3) {
-    if (blocks) {
-      a.thr1 = sqrt_le_bound(dm * p->tuning.nh_first_radius);
-      a.box1 = dm * p->tuning.nh_first_radius * (1.f + 1.f / 1024.f);
-      // second phase levels: twice the radius each time, dm last
-      a.nlvl = 0;
-      a.lvl_prev0 = std::min(a.thr1, a.thr);
-      float r = dm * p->tuning.nh_first_radius;
-      if (r < dm) {
-        while (a.nlvl < 7) {
-          r *= 2.f;
-          if (!(r < dm)) break;
-          a.lvl[a.nlvl++] = sqrt_le_bound(r);
-        }
-        a.lvl[a.nlvl++] = a.thr;
-      }
-      rc = launch_nh_maxkeys(p, a, st);
-      if (rc != CLUE_OK) return rc;
-      k_nearest_higher_blocks<D><<<grid, kThreads, 0, st>>>(a);
-      CLUE_LAUNCH_CHECK("nearest_higher_blocks");
-      rc = scan_exclusive(a.pending_flag, a.pending_flag, a.n, p->part.as<int32_t>(), st);
-      if (rc != CLUE_OK) return rc;
-      rc = launch_compact(a.pending_flag, a.n, a.pending, st);
-      if (rc != CLUE_OK) return rc;
-      a.pending_count = a.pending_flag + a.n;
-      // enough waves to fill the GPU; surplus waves find the pending list exhausted
-      // default grid of the persistent second phase: 8192 workgroups for one
-      // event of 2^20 points or more (2D 1M points: 0.51 -> 0.49 ms, 3D 4M: 7.43
-      // -> 7.31 ms), 2048 otherwise (the 4096-event batch: 7.08 ms against 7.23 ms
-      // with 8192; small served requests lose more than they gain)
-      const int pblocks = p->tuning.pending_blocks > 0 ? p->tuning.pending_blocks
-                                                       : (a.n_events == 1 && a.n >= (1 << 20) ? 8192 : 2048);
-      const int nblk = static_cast<int>(std::min<int64_t>(pblocks, div_up(a.n, kThreads / kWave)));
-      switch (p->tuning.pending_group) {
-        case 8: k_nh_pending<D, 8><<<nblk, kThreads, 0, st>>>(a); break;
-        case 16: k_nh_pending<D, 16><<<nblk, kThreads, 0, st>>>(a); break;
-        case 32: k_nh_pending<D, 32><<<nblk, kThreads, 0, st>>>(a); break;
-        default: k_nh_pending<D, 64><<<nblk, kThreads, 0, st>>>(a); break;
-      }
-      CLUE_LAUNCH_CHECK("nearest_higher_pending");
-      return CLUE_OK;
-    }
-  }
-  k_nearest_higher<D, METRIC, U><<<grid, kThreads, 0, st>>>(a);
-  CLUE_LAUNCH_CHECK("nearest_higher");
-  return CLUE_OK;
+  return nearest_higher();
 }
 
 template <int D, int METRIC>

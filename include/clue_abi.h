@@ -108,7 +108,19 @@ typedef struct {
  *  pending_group       lanes per unresolved point in that phase: 8, 16, 32, 64
  *  density_order       1: launch the density pass's query blocks heaviest first
  *                      (estimated from their tiles' sizes), 0: in Morton order,
- *                      -1: heaviest first in 1D and 2D only */
+ *                      -1: heaviest first in 1D and 2D only
+ *  small_call_points   calls of at most this many points (all events together)
+ *                      run the density pass and the tile-walk nearest-higher
+ *                      search with one wave per query, which shortens the
+ *                      longest query's chain when few queries fill the GPU;
+ *                      0: never by size
+ *  wave_dims           calls in more dimensions than this run them with one
+ *                      wave per query whatever their size (lanes walking their
+ *                      own boxes diverge there); 10: never by dimension
+ *  small_call_blocks   1: small calls (small_call_points) in 1-3 dimensions
+ *                      keep the Morton layout and block search for the
+ *                      nearest-higher pass; 0: they skip the layout and walk
+ *                      the tiles, one wave per query */
 typedef struct {
   float nh_first_radius;
   int32_t morton_bits;
@@ -116,6 +128,9 @@ typedef struct {
   int32_t pending_blocks;
   int32_t pending_group;
   int32_t density_order;
+  int32_t small_call_points;
+  int32_t wave_dims;
+  int32_t small_call_blocks;
 } clue_tuning;
 
 typedef struct clue_plan_s* clue_plan_t;

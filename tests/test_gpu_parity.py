@@ -126,7 +126,9 @@ TUNINGS = [dict(nh_first_radius=0.05, morton_bits=5),
            dict(xcd_remap=3, pending_blocks=7, pending_group=64),
            dict(xcd_remap=2, pending_blocks=0, nh_first_radius=0.01, pending_group=32, density_order=0),
            dict(density_order=1, xcd_remap=0, pending_group=16),
-           dict(density_order=1, xcd_remap=7)]
+           dict(density_order=1, xcd_remap=7),
+           dict(small_call_points=0, wave_dims=10),
+           dict(small_call_points=1 << 30, nh_first_radius=0.3)]
 
 
 @pytest.mark.parametrize("tuning", TUNINGS)
@@ -180,6 +182,8 @@ def test_known_answer_twenty_clusters(ref_data):
 def test_metrics_and_kernels(ref_data, metric, mp, kernel):
     c, w = ref_data("toyDetector_1000")
     assert_same(*run_both(c, w, 4.0, 2.5, 4.0, kernel=kernel, metric=metric, metric_params=mp))
+    assert_same(*run_both(c, w, 4.0, 2.5, 4.0, kernel=kernel, metric=metric, metric_params=mp,
+                          tuning=dict(small_call_points=0, wave_dims=10)))
 
 
 @pytest.mark.parametrize("kernel", [("gaussian", 0.0, 1.0, 1.0), ("exponential", 1.0, 1.0)])
@@ -188,10 +192,14 @@ def test_kernels_euclidean(kernel):
     assert_same(*run_both(ps.coords, ps.weights, 2.0, 8.0, 4.0, kernel=kernel))
 
 
+# small calls run the neighbour passes with one wave per query (small_call_points),
+# larger ones with one lane per query: both on every dimension
+@pytest.mark.parametrize("lanes", [False, True])
 @pytest.mark.parametrize("ndim", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
-def test_every_dimension(ndim):
+def test_every_dimension(ndim, lanes):
     ps = datasets.blobs(3000, ndim, 6, 0.0, 40.0, 1.0, 0.1, ("uniform", 0.5, 1.5), 50 + ndim)
-    assert_same(*run_both(ps.coords, ps.weights, 1.5, 3.0, 2.5, kernel=("gaussian", 0.3, 1.0, 1.0)))
+    tuning = dict(small_call_points=0, wave_dims=10) if lanes else None
+    assert_same(*run_both(ps.coords, ps.weights, 1.5, 3.0, 2.5, kernel=("gaussian", 0.3, 1.0, 1.0), tuning=tuning))
 
 
 def test_seed_capacity_truncation():
