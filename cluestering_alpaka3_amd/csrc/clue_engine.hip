@@ -1191,8 +1191,7 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
   const int tbits = bits_for(Ttot);
   int sub_bits = std::min(D == 1 ? 10 : (D == 2 ? 5 : 4), (31 - tbits) / D);
   if (p->tuning.morton_bits >= 0) sub_bits = std::min(p->tuning.morton_bits, (31 - tbits) / D);
-  const bool small_tiles = p->tuning.small_call_blocks == 0 && p->tuning.small_call_points > 0 &&
-                           n <= p->tuning.small_call_points;
+  const bool small_tiles = n <= p->tuning.tile_call_points;
   const bool blocks = md.type == MEUCLID && !wrapped_any && D <= 3 && sub_bits >= 1 && !small_tiles;
   if (!blocks) sub_bits = -1;
 
@@ -1815,8 +1814,7 @@ clue_status clue_plan_set_tuning(clue_plan_t plan, const clue_tuning* tuning) {
       (tuning->pending_group != 8 && tuning->pending_group != 16 && tuning->pending_group != 32 &&
        tuning->pending_group != 64) ||
       tuning->density_order < -1 || tuning->density_order > 1 || tuning->small_call_points < 0 ||
-      tuning->wave_dims < 0 || tuning->wave_dims > CLUE_MAX_DIM || tuning->small_call_blocks < 0 ||
-      tuning->small_call_blocks > 1)
+      tuning->wave_dims < 0 || tuning->wave_dims > CLUE_MAX_DIM || tuning->tile_call_points < 0)
     return fail(CLUE_ERR_INVALID_ARGUMENT, "invalid tuning values");
   plan->tuning = *tuning;
   return CLUE_OK;

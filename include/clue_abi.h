@@ -117,10 +117,10 @@ typedef struct {
  *  wave_dims           calls in more dimensions than this run them with one
  *                      wave per query whatever their size (lanes walking their
  *                      own boxes diverge there); 10: never by dimension
- *  small_call_blocks   1: small calls (small_call_points) in 1-3 dimensions
- *                      keep the Morton layout and block search for the
- *                      nearest-higher pass; 0: they skip the layout and walk
- *                      the tiles, one wave per query */
+ *  tile_call_points    calls of at most this many points in 1-3 dimensions skip
+ *                      the Morton layout and the block search of the
+ *                      nearest-higher pass and walk the tiles instead (one wave
+ *                      per query when small_call_points holds them); 0: never */
 typedef struct {
   float nh_first_radius;
   int32_t morton_bits;
@@ -130,7 +130,7 @@ typedef struct {
   int32_t density_order;
   int32_t small_call_points;
   int32_t wave_dims;
-  int32_t small_call_blocks;
+  int32_t tile_call_points;
 } clue_tuning;
 
 typedef struct clue_plan_s* clue_plan_t;

@@ -127,8 +127,10 @@ TUNINGS = [dict(nh_first_radius=0.05, morton_bits=5),
            dict(xcd_remap=2, pending_blocks=0, nh_first_radius=0.01, pending_group=32, density_order=0),
            dict(density_order=1, xcd_remap=0, pending_group=16),
            dict(density_order=1, xcd_remap=7),
-           dict(small_call_points=0, wave_dims=10),
-           dict(small_call_points=1 << 30, nh_first_radius=0.3)]
+           dict(small_call_points=0, wave_dims=10, tile_call_points=0),
+           dict(small_call_points=1 << 30, nh_first_radius=0.3, tile_call_points=0),
+           dict(small_call_points=1 << 30, tile_call_points=1 << 30),
+           dict(small_call_points=0, tile_call_points=1 << 30)]
 
 
 @pytest.mark.parametrize("tuning", TUNINGS)
@@ -183,7 +185,7 @@ def test_metrics_and_kernels(ref_data, metric, mp, kernel):
     c, w = ref_data("toyDetector_1000")
     assert_same(*run_both(c, w, 4.0, 2.5, 4.0, kernel=kernel, metric=metric, metric_params=mp))
     assert_same(*run_both(c, w, 4.0, 2.5, 4.0, kernel=kernel, metric=metric, metric_params=mp,
-                          tuning=dict(small_call_points=0, wave_dims=10)))
+                          tuning=dict(small_call_points=0, wave_dims=10, tile_call_points=0)))
 
 
 @pytest.mark.parametrize("kernel", [("gaussian", 0.0, 1.0, 1.0), ("exponential", 1.0, 1.0)])
@@ -198,7 +200,7 @@ def test_kernels_euclidean(kernel):
 @pytest.mark.parametrize("ndim", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
 def test_every_dimension(ndim, lanes):
     ps = datasets.blobs(3000, ndim, 6, 0.0, 40.0, 1.0, 0.1, ("uniform", 0.5, 1.5), 50 + ndim)
-    tuning = dict(small_call_points=0, wave_dims=10) if lanes else None
+    tuning = dict(small_call_points=0, wave_dims=10, tile_call_points=0) if lanes else None
     assert_same(*run_both(ps.coords, ps.weights, 1.5, 3.0, 2.5, kernel=("gaussian", 0.3, 1.0, 1.0), tuning=tuning))
 
 
