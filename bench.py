@@ -151,7 +151,8 @@ def served_bench(clue, datasets, args):
 
     ok1, pois = poisson(args.served_load, 7)
     ok2, pois_light = poisson(args.served_load / 2, 8)
-    same = same and ok1 and ok2
+    ok3, pois_low = poisson(0.1, 9)
+    same = same and ok1 and ok2 and ok3
     st = srv.stats()
     srv.close()
     pl = np.array(plain_lat) * 1e3
@@ -166,6 +167,7 @@ def served_bench(clue, datasets, args):
                              "p99": round(float(np.percentile(pl, 99)), 3)},
         "poisson": pois,
         "poisson_light": pois_light,
+        "poisson_low": pois_low,
         "labels_equal_plain": bool(same),
         "server": {"slots": args.served_slots, "budget_bytes": 32 << 30, "slot_capacity_points": st["slot_capacity_points"],
                    "device_bytes": st["device_bytes"], "batches": st["batches"],
