@@ -1809,10 +1809,10 @@ clue_status clue_plan_copy_delta(clue_plan_t plan, float* delta) {
 
 clue_status clue_plan_set_tuning(clue_plan_t plan, const clue_tuning* tuning) {
   if (!plan || !tuning) return fail(CLUE_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (!(tuning->nh_first_radius > 0.f) || tuning->morton_bits > 10 ||
+  if (!(tuning->nh_first_radius >= 0.f) || tuning->morton_bits > 10 ||
       tuning->xcd_remap < 0 || tuning->xcd_remap > 7 || tuning->pending_blocks < 0 ||
-      (tuning->pending_group != 8 && tuning->pending_group != 16 && tuning->pending_group != 32 &&
-       tuning->pending_group != 64) ||
+      (tuning->pending_group != 0 && tuning->pending_group != 8 && tuning->pending_group != 16 &&
+       tuning->pending_group != 32 && tuning->pending_group != 64) ||
       tuning->density_order < -1 || tuning->density_order > 1 || tuning->small_call_points < 0 ||
       tuning->wave_dims < 0 || tuning->wave_dims > CLUE_MAX_DIM || tuning->tile_call_points < 0)
     return fail(CLUE_ERR_INVALID_ARGUMENT, "invalid tuning values");

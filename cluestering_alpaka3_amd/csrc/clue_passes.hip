@@ -818,7 +818,7 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
 // group reduction takes the least:
 // the serial backend's first strict minimum on a grid without wrapping.  The
 // grid is persistent: every wave strides over the pending list and leaves when
-// it is exhausted.
+// it is exhausted.  SINGLE: one event, whose grid is then wave-uniform.
 template <int G>
 __device__ __forceinline__ float group_min_f32(float v) {
 #pragma unroll
@@ -829,7 +829,7 @@ __device__ __forceinline__ float group_min_f32(float v) {
   return v;
 }
 
-template <int D, int G>
+template <int D, int G, bool SINGLE>
 __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
   static_assert(kWave % G == 0, "groups must tile the wave");
   const int npend = uniform(*a.pending_count);
@@ -849,21 +849,8 @@ __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
     const uint64_t key_i = a.hk[pos];
     const uint64_t thr_i = higher_threshold(key_i);
     const int32_t id_i = static_cast<int32_t>(key_i & 0xffffffffu);
-    const int e = a.pev[id_i];
-    const EventDesc* ev = a.ev + e;
-    EvGrid<D> g;
-    g.npd = ev->npd;
-    g.tile_base = ev->tile_base;
-    g.ntiles = ev->ntiles;
-    g.begin = ev->begin;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      g.stride[d] = ev->stride[d];
-      g.mn[d] = ev->mn[d];
-      g.mx[d] = ev->mx[d];
-      g.ts[d] = ev->ts[d];
-      g.wrapped[d] = a.wrap.w[d] != 0;
-    }
+    const int e = SINGLE ? 0 : a.pev[id_i];
+    const EvGrid<D> g = load_event<D>(a.ev, e, a.wrap);
     int lo[D], hi[D];
     search_box<D>(q, a.radius, g, lo, hi);
     float delta = FLT_MAX;
@@ -873,12 +860,15 @@ __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
     float bound = a.thr;
     float lo2 = __builtin_inff();
     float group_bound = a.thr;
-    // Levels of growing radius: a level examines the candidates within its
-    // radius that lie outside the previous one (blocks entirely inside were
-    // examined before and hold no higher point); the first level that finds a
-    // higher point is the last, since everything nearer has then been examined.
-    float prev = a.lvl_prev0;
-    bool found = false;
+    // Levels of growing radius order the visits nearest first, so that the
+    // bound shrinks early.  Each block is visited once, at the first level whose
+    // radius reaches its box, and then every candidate of it within the bound is
+    // examined; blocks inside the first phase's radius hold no higher point.
+    // After a level the search is complete when the group's bound lies within
+    // the level's radius: every block not yet visited is farther than that.
+    const float prev0 = a.lvl_prev0;
+    float prev = -1.f;  // the previous level's threshold (none before the first)
+    bool done = false;
     // The box's tiles, one per lane of the group when they fit (their boxes and
     // ranges are then loaded once, together, and shuffled to the group at each
     // level); otherwise each level walks them and loads them one by one.
@@ -917,10 +907,21 @@ __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
       my_o1 = tv ? a.offsets[my_t + 1] : 0;
       my_tk = tv ? a.tmaxk[my_t] : 0;
     }
+    auto cand = [&](int t, const float* c, uint64_t key_j) {
+      const float s2 = squared_distance<D>(q, c);
+      const int64_t key = (static_cast<int64_t>(t) << 32) | static_cast<uint32_t>(key_j);
+      if (key_j > thr_i && s2 <= bound && (s2 < lo2 || key < best)) {
+        delta = __builtin_sqrtf(s2);
+        best = key;
+        float hi2;
+        sqrt_class(delta, lo2, hi2);
+        bound = hi2 < a.thr ? hi2 : a.thr;
+      }
+    };
     for (int lv = 0; lv < a.nlvl; ++lv) {
       const float cap = a.lvl[lv];
       int ti = 0;
-      bool more = active && !found;
+      bool more = active && !done;
       while (__ballot(more) != 0) {
         if (more) {  // uniform within a group
           int t, k0, k1, off0, off1;
@@ -954,10 +955,13 @@ __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
             tk = a.tmaxk[t];
           }
           const float lim_t = cap < group_bound ? cap : group_bound;
-          // tiles and blocks without a point higher than this one are skipped
-          if (tk > thr_i && lower_bound2<D>(q, tl, th) <= lim_t && upper_bound2<D>(q, tl, th) > prev) {
+          // tiles without a point higher than this one, beyond the level's
+          // radius, inside the first phase's radius or whose every block an
+          // earlier level reached are skipped
+          const float ub_t = upper_bound2<D>(q, tl, th);
+          if (tk > thr_i && lower_bound2<D>(q, tl, th) <= lim_t && ub_t > prev0 && ub_t > prev) {
             // the tile's Morton blocks: G at a time against the group's bound,
-            // then the points of each block that can still hold a better candidate
+            // then the points of each block visited at this level
             for (int kb = k0; kb < k1; kb += G) {
               const int k = kb + gl;
               const bool inb = k < k1;
@@ -969,8 +973,9 @@ __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
               }
               const uint64_t bk = inb ? a.bmaxk[k] : 0;
               const float lim_b = cap < group_bound ? cap : group_bound;
-              const bool pass = inb && bk > thr_i && lower_bound2<D>(q, bl, bh) <= lim_b &&
-                                upper_bound2<D>(q, bl, bh) > prev;
+              const float lb = lower_bound2<D>(q, bl, bh);
+              const bool pass = inb && bk > thr_i && lb <= lim_b && lb > prev &&
+                                upper_bound2<D>(q, bl, bh) > prev0;
               uint64_t gm = __ballot(pass) >> grp_shift;
               if constexpr (G < kWave) gm &= (uint64_t{1} << G) - 1;
               while (gm) {
@@ -978,21 +983,21 @@ __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
                 gm &= gm - 1;
                 const int start = off0 + (kb + kl - k0) * kBlock;
                 const int end = min(start + kBlock, off1);
-                const float lane_cap = bound < cap ? bound : cap;
-                for (int j = start + gl; j < end; j += G) {
-                  float c[D];
+                // two candidates per lane, both loads issued before either is used
+                for (int j = start + gl; j < end; j += 2 * G) {
+                  const int j2 = j + G;
+                  const bool v2 = j2 < end;
+                  const int jj = v2 ? j2 : j;
+                  float c0[D], c1[D];
 #pragma unroll
-                  for (int d = 0; d < D; ++d) c[d] = a.xn[d][j];
-                  const uint64_t key_j = a.hk[j];
-                  const float s2 = squared_distance<D>(q, c);
-                  const int64_t key = (static_cast<int64_t>(t) << 32) | static_cast<uint32_t>(key_j);
-                  if (key_j > thr_i && s2 <= lane_cap && s2 <= bound && (s2 < lo2 || key < best)) {
-                    delta = __builtin_sqrtf(s2);
-                    best = key;
-                    float hi2;
-                    sqrt_class(delta, lo2, hi2);
-                    bound = hi2 < a.thr ? hi2 : a.thr;
+                  for (int d = 0; d < D; ++d) {
+                    c0[d] = a.xn[d][j];
+                    c1[d] = a.xn[d][jj];
                   }
+                  const uint64_t h0 = a.hk[j];
+                  const uint64_t h1 = a.hk[jj];
+                  cand(t, c0, h0);
+                  if (v2) cand(t, c1, h1);
                 }
               }
               group_bound = group_min_f32<G>(bound);  // the point's best so far over its lanes
@@ -1001,9 +1006,8 @@ __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
           more = ++ti < nt;
         }
       }
-      uint64_t fm = __ballot(best != INT64_MAX) >> grp_shift;
-      if constexpr (G < kWave) fm &= (uint64_t{1} << G) - 1;
-      found = fm != 0;
+      group_bound = group_min_f32<G>(bound);
+      done = group_bound <= cap;
       prev = cap;
     }
     // lexicographic (distance, key) minimum over the group
@@ -1117,12 +1121,17 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
     if (rc != CLUE_OK) return rc;
     if constexpr (METRIC == MEUCLID && D <= 3) {
       if (blocks) {
-        a.thr1 = sqrt_le_bound(dm * p->tuning.nh_first_radius);
-        a.box1 = dm * p->tuning.nh_first_radius * (1.f + 1.f / 1024.f);
+        // automatic first radius and group: measured per dimension (3D 4M points:
+        // 3.76 ms at 0.15 dm and 16 lanes, 3.0 ms at 0.2 dm and 8 lanes; 2D 1M
+        // points: 0.47 ms at 0.15 dm and 16 lanes, 0.50 ms at 0.2 dm and 8 lanes)
+        const float r1 = p->tuning.nh_first_radius > 0.f ? p->tuning.nh_first_radius : (D == 3 ? 0.2f : 0.15f);
+        const int group = p->tuning.pending_group > 0 ? p->tuning.pending_group : (D == 3 ? 8 : 16);
+        a.thr1 = sqrt_le_bound(dm * r1);
+        a.box1 = dm * r1 * (1.f + 1.f / 1024.f);
         // second phase levels: twice the radius each time, dm last
         a.nlvl = 0;
         a.lvl_prev0 = std::min(a.thr1, a.thr);
-        float r = dm * p->tuning.nh_first_radius;
+        float r = dm * r1;
         if (r < dm) {
           while (a.nlvl < 7) {
             r *= 2.f;
@@ -1148,11 +1157,24 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
         const int pblocks = p->tuning.pending_blocks > 0 ? p->tuning.pending_blocks
                                                          : (a.n_events == 1 && a.n >= (1 << 20) ? 8192 : 2048);
         const int nblk = static_cast<int>(std::min<int64_t>(pblocks, div_up(a.n, kThreads / kWave)));
-        switch (p->tuning.pending_group) {
-          case 8: k_nh_pending<D, 8><<<nblk, kThreads, 0, st>>>(a); break;
-          case 16: k_nh_pending<D, 16><<<nblk, kThreads, 0, st>>>(a); break;
-          case 32: k_nh_pending<D, 32><<<nblk, kThreads, 0, st>>>(a); break;
-          default: k_nh_pending<D, 64><<<nblk, kThreads, 0, st>>>(a); break;
+        const bool single = a.n_events == 1;
+        switch (group) {
+          case 8:
+            if (single) k_nh_pending<D, 8, true><<<nblk, kThreads, 0, st>>>(a);
+            else k_nh_pending<D, 8, false><<<nblk, kThreads, 0, st>>>(a);
+            break;
+          case 16:
+            if (single) k_nh_pending<D, 16, true><<<nblk, kThreads, 0, st>>>(a);
+            else k_nh_pending<D, 16, false><<<nblk, kThreads, 0, st>>>(a);
+            break;
+          case 32:
+            if (single) k_nh_pending<D, 32, true><<<nblk, kThreads, 0, st>>>(a);
+            else k_nh_pending<D, 32, false><<<nblk, kThreads, 0, st>>>(a);
+            break;
+          default:
+            if (single) k_nh_pending<D, 64, true><<<nblk, kThreads, 0, st>>>(a);
+            else k_nh_pending<D, 64, false><<<nblk, kThreads, 0, st>>>(a);
+            break;
         }
         CLUE_LAUNCH_CHECK("nearest_higher_pending");
         return CLUE_OK;
