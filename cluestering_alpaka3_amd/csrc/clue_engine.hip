@@ -1316,6 +1316,7 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
   }
   a.ntiles = Ttot;
   a.xcd = p->tuning.xcd_remap;
+  a.dfilter = p->tuning.density_filter;
   a.tnf = p->tnf.as<int32_t>();
   // rank keys of the densities: Morton order for the block search, tile order for the tile walk
   a.hk_out = p->hk.as<uint64_t>();
@@ -1814,7 +1815,8 @@ clue_status clue_plan_set_tuning(clue_plan_t plan, const clue_tuning* tuning) {
       (tuning->pending_group != 0 && tuning->pending_group != 8 && tuning->pending_group != 16 &&
        tuning->pending_group != 32 && tuning->pending_group != 64) ||
       tuning->density_order < -1 || tuning->density_order > 1 || tuning->small_call_points < 0 ||
-      tuning->wave_dims < 0 || tuning->wave_dims > CLUE_MAX_DIM || tuning->tile_call_points < 0)
+      tuning->wave_dims < 0 || tuning->wave_dims > CLUE_MAX_DIM || tuning->tile_call_points < 0 ||
+      tuning->density_filter < 0 || tuning->density_filter > 1)
     return fail(CLUE_ERR_INVALID_ARGUMENT, "invalid tuning values");
   plan->tuning = *tuning;
   return CLUE_OK;

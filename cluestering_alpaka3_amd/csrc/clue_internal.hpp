@@ -168,6 +168,7 @@ struct PassArgs {
   const int32_t* qorder;     // density: query blocks in decreasing estimated cost (or null)
   const int32_t* tnf;        // density: 1 for tiles holding an infinite or NaN weight
   const int32_t* qperm;      // tile-order walks: query slots by growing search-box size (or null)
+  int dfilter;               // density: filtered union walk (tuning density_filter)
 };
 
 
@@ -277,6 +278,56 @@ __device__ __forceinline__ float wave_min_f32(float v) {
   }
   return v;
 }
+__device__ __forceinline__ float uniform_f32(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+// Number of set bits of m below this lane.
+__device__ __forceinline__ int mbcnt64(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0));
+}
+// Neighbouring floats (nextafterf towards -inf / +inf); NaN stays NaN.
+__device__ __forceinline__ float next_down(float x) {
+  if (x != x || x == -__builtin_inff()) return x;
+  if (x == 0.f) return -0x1p-149f;
+  const int b = __float_as_int(x);
+  return __int_as_float(x > 0.f ? b - 1 : b + 1);
+}
+__device__ __forceinline__ float next_up(float x) { return -next_down(-x); }
+
+// Whether every candidate within radius r of q (Euclidean, squared distance
+// <= thr) lies in q's search box [lo, hi], so that a walk may ignore which
+// tiles a lane's box holds.  A candidate c outside the box in dimension d has
+// c < m = fl(q - r) (or c > M = fl(q + r)): the bins are monotone while
+// (x - min)/tile_size stays inside the int range, as it does for every
+// candidate.  Then fl(q - c) >= fl(q - next_down(m)), and once that exceeds
+// R = max(r, 2^-40), fl((q-c)^2) and with it the squared distance exceed thr
+// (sqrtf(fl(x*x)) = |x| for |x| >= 2^-40).  Boxes touching the grid's edge, and
+// dimensions of zero extent (every bin 0, the box [0, 0]), exclude nothing.
+// Non-finite queries, wrapped dimensions and overflowing bins answer false.
+template <int D>
+__device__ __forceinline__ bool box_exact(const float* q, float r, float R, const EvGrid<D>& g, const int* lo,
+                                          const int* hi) {
+  bool ok = true;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    ok = ok && !g.wrapped[d] && __builtin_isfinite(q[d]);
+    if (g.ts[d] == 0.f) continue;
+    ok = ok && __builtin_isfinite(g.ts[d]) && g.ts[d] > 0.f;
+    if (lo[d] > 0) {
+      const float m = q[d] - r;
+      const float v = (m - g.mn[d]) / g.ts[d];
+      ok = ok && v >= -2147483648.0f && v < 2147483648.0f && (q[d] - next_down(m)) > R;
+    }
+    if (hi[d] < g.npd - 1) {
+      const float mm = q[d] + r;
+      const float v = (mm - g.mn[d]) / g.ts[d];
+      ok = ok && v >= -2147483648.0f && v < 2147483648.0f && (next_up(mm) - q[d]) > R;
+    }
+  }
+  return ok;
+}
+
 __device__ __forceinline__ float readlane_f32(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
@@ -387,7 +438,7 @@ struct clue_plan_s {
   DevBuf pending, pending_flag, qcost, qorder;
   // cluster -> points map (post-processing): sort keys and values
   DevBuf mkey, mkey2, mval, mval2;
-  clue_tuning tuning{0.f, -1, 7, 0, 0, -1, 65536, 3, 32768};
+  clue_tuning tuning{0.f, -1, 7, 0, 0, -1, 65536, 3, 32768, 1};
   int64_t ldb = 0;  // leading dimension of the block boxes
   int64_t ld = 0;   // leading dimension of xs / xn
   // pinned staging ring of the event and chunk tables (a slot is reused once its

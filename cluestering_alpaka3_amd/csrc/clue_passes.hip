@@ -110,7 +110,7 @@ __global__ __launch_bounds__(kThreads) void k_box_class(const PassArgs a, float 
 // QNH: the lanes take their queries in Morton order (spatially coherent waves,
 // so the lanes of a wave share most of their search box); otherwise in tile order.
 // UNIT: Euclidean metric, exp argument in [-110, 0] and amplitude 1 (exp_unit).
-template <int D, int METRIC, int KERNEL, bool UNION, bool QNH, bool UNIT = false>
+template <int D, int METRIC, int KERNEL, bool UNION, bool QNH, bool UNIT = false, bool FILT = false>
 __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
   // the exponential's 2^(j/64) table, read through LDS by the kernel evaluations
   __shared__ double s_exp2[KERNEL == KFLAT ? 1 : 64];
@@ -118,6 +118,14 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
     if (threadIdx.x < 64) s_exp2[threadIdx.x] = kExp2Table[threadIdx.x];
     __syncthreads();
   }
+  // the filtered union walk's per-wave ring of candidates (coordinates, value, slot)
+  constexpr bool kFilter = FILT && UNION && METRIC == MEUCLID;
+  constexpr int kRing = 128;  // entries per wave; a batch never wraps (kRing % kFB == 0)
+  constexpr int kFB = 8;      // candidates per evaluation batch
+  // kernel value 1 at distance 0, the self term's value: the self test is not needed
+  constexpr bool kSelfFree = UNIT && KERNEL == KEXP;
+  __shared__ float4 s_ring[kFilter ? kThreads / kWave : 1][kFilter ? kRing : 1];
+  __shared__ int32_t s_ringj[kFilter && D == 3 ? kThreads / kWave : 1][kFilter && D == 3 ? kRing : 1];
   const int qb = a.qorder ? a.qorder[blockIdx.x] : logical_block(a.xcd & 1);
   const int pos = qb * kThreads + threadIdx.x;
   const bool valid = pos < a.n;
@@ -218,10 +226,124 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
       if (need) eval(self_c, j, c, wj, hwj);
     }
   };
+  // out-of-radius candidates with an infinite or NaN weight add 0 * w = NaN
+  auto poison_check = [&](int t, int beg, int end, bool need) {
+    if (a.tnf[t] != 0) {
+      for (int j = beg; j < end; ++j) {
+        const float wj = W[j];
+        if (need && !__builtin_isfinite(wj)) {
+          float c[D];
+#pragma unroll
+          for (int d = 0; d < D; ++d) c[d] = X[d][j];
+          bool in;
+          if constexpr (METRIC == MEUCLID) in = squared_distance<D>(q, c) <= a.thr;
+          else in = metric_distance<D, METRIC>(q, c, a.mp) <= a.radius;
+          poisoned = poisoned || !in;
+        }
+      }
+    }
+  };
   for_each_event(valid, my_e, [&](int e, bool act) {
     const EvGrid<D> g = load_event<D>(a.ev, e, a.wrap);
     int lo[D], hi[D];
     search_box<D>(q, a.radius, g, lo, hi);
+    if constexpr (kFilter) {
+      // Filtered union walk.  The wave drops every candidate farther than R (in
+      // some dimension) from the box of its queries: no lane has it within the
+      // radius.  The survivors are compacted in the visit order into a per-wave
+      // ring in LDS (lane order = CSR order inside a chunk of a tile, tiles
+      // ascending) and evaluated kFB at a time, every lane reading the same
+      // candidate at one address.  When every lane passes box_exact, each lane's
+      // in-radius candidates all lie in its own box, so the lane adds every
+      // in-radius survivor, whatever tile it came from: exactly its box's terms,
+      // in its box's order.  Otherwise the ring is emptied at the end of each
+      // tile and only the lanes whose box holds the tile add its survivors.
+      const float R = __builtin_fmaxf(a.radius, 0x1p-40f);
+      const bool exact = __ballot(act && !box_exact<D>(q, a.radius, R, g, lo, hi)) == 0;
+      float qlo[D], qhi[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        qlo[d] = uniform_f32(wave_min_f32(act ? q[d] : __builtin_inff()));
+        qhi[d] = uniform_f32(wave_max_f32(act ? q[d] : -__builtin_inff()));
+      }
+      float4* ring = s_ring[wv];
+      int32_t* ringj = s_ringj[D <= 2 ? 0 : wv];
+      int head = 0, tail = 0;  // wave-uniform ring positions (head a multiple of kFB)
+      auto put = [&](int at, const float* c, float v, int j) {
+        float4 r4;
+        if constexpr (D == 1) r4 = make_float4(c[0], v, __int_as_float(j), 0.f);
+        else if constexpr (D == 2) r4 = make_float4(c[0], c[1], v, __int_as_float(j));
+        else r4 = make_float4(c[0], c[1], c[2], v);
+        ring[at] = r4;
+        if constexpr (D == 3) ringj[at] = j;
+      };
+      auto drain = [&](bool take) {
+        __builtin_amdgcn_wave_barrier();
+        float4 r4[kFB];
+        int jj[kFB];
+#pragma unroll
+        for (int u = 0; u < kFB; ++u) {
+          r4[u] = ring[head + u];
+          if constexpr (D == 3) jj[u] = ringj[head + u];
+        }
+        if (take) {
+#pragma unroll
+          for (int u = 0; u < kFB; ++u) {
+            float c[D];
+            float v;
+            int j;
+            if constexpr (D == 1) {
+              c[0] = r4[u].x; v = r4[u].y; j = __float_as_int(r4[u].z);
+            } else if constexpr (D == 2) {
+              c[0] = r4[u].x; c[1] = r4[u].y; v = r4[u].z; j = __float_as_int(r4[u].w);
+            } else {
+              c[0] = r4[u].x; c[1] = r4[u].y; c[2] = r4[u].z; v = r4[u].w; j = jj[u];
+            }
+            if constexpr (kSelfFree) eval(std::false_type{}, j, c, v, v);
+            else eval(std::true_type{}, j, c, v, v);
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+        head = (head + kFB) & (kRing - 1);
+      };
+      // the last batch is padded with candidates no lane adds (NaN distance, slot -2)
+      auto flush = [&](bool take) {
+        const int left = (tail - head) & (kRing - 1);
+        if (left > 0) {
+          if (lane < kFB - left) {
+            float c[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) c[d] = __builtin_nanf("");
+            put((tail + lane) & (kRing - 1), c, 0.f, -2);
+          }
+          tail = (head + kFB) & (kRing - 1);
+          drain(take);
+        }
+      };
+      visit_box<D, true>(act, lo, hi, g, as_const(a.offsets), [&](int t, int beg, int end, bool need) {
+        // lanes of the wave's other events take no part
+        const bool take = exact ? act : need;
+        for (int base = beg; base < end; base += kWave) {
+          const int j = base + lane;
+          bool pass = j < end;
+          float c[D];
+          float v = 0.f;
+#pragma unroll
+          for (int d = 0; d < D; ++d) c[d] = pass ? a.xs[d][j] : 0.f;
+          if (pass) v = kUsesHW ? a.hw[j] : a.ws[j];
+#pragma unroll
+          for (int d = 0; d < D; ++d) pass = pass && (qlo[d] - c[d] <= R) && (c[d] - qhi[d] <= R);
+          const uint64_t m = __ballot(pass);
+          if (pass) put((tail + mbcnt64(m)) & (kRing - 1), c, v, j);
+          tail = (tail + __builtin_popcountll(m)) & (kRing - 1);
+          while (((tail - head) & (kRing - 1)) >= kFB) drain(take);
+        }
+        if (!exact) flush(take);
+        poison_check(t, beg, end, need);
+      });
+      flush(act);
+      return;
+    }
     visit_box<D, UNION>(act, lo, hi, g, as_const(a.offsets), [&](int t, int beg, int end, bool need) {
       // Only the tile holding a lane's own point needs the j == i test.
       const bool self = !UNION || __ballot(s >= beg && s < end) != 0;
@@ -229,21 +351,7 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
       if constexpr (!UNION) uni = __ballot(t != __builtin_amdgcn_readfirstlane(t)) == 0;
       if (self) run(std::true_type{}, beg, end, need, uni);
       else run(std::false_type{}, beg, end, need, uni);
-      // out-of-radius candidates with an infinite or NaN weight add 0 * w = NaN
-      if (a.tnf[t] != 0) {
-        for (int j = beg; j < end; ++j) {
-          const float wj = W[j];
-          if (need && !__builtin_isfinite(wj)) {
-            float c[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) c[d] = X[d][j];
-            bool in;
-            if constexpr (METRIC == MEUCLID) in = squared_distance<D>(q, c) <= a.thr;
-            else in = metric_distance<D, METRIC>(q, c, a.mp) <= a.radius;
-            poisoned = poisoned || !in;
-          }
-        }
-      }
+      poison_check(t, beg, end, need);
     });
   });
   if (poisoned) rho = __builtin_nanf("");
@@ -1235,18 +1343,25 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
   if (blocks) {
     bool unit = false;
     if constexpr (METRIC == MEUCLID) unit = a.exp_unit != 0;
-    switch (kernel_type) {
-      case KFLAT: k_density<D, METRIC, KFLAT, U, true><<<grid, kThreads, 0, st>>>(a); break;
-      case KGAUSS:
-        if (unit) k_density<D, METRIC, KGAUSS, U, true, METRIC == MEUCLID><<<grid, kThreads, 0, st>>>(a);
-        else k_density<D, METRIC, KGAUSS, U, true><<<grid, kThreads, 0, st>>>(a);
-        break;
-      case KEXP:
-        if (unit) k_density<D, METRIC, KEXP, U, true, METRIC == MEUCLID><<<grid, kThreads, 0, st>>>(a);
-        else k_density<D, METRIC, KEXP, U, true><<<grid, kThreads, 0, st>>>(a);
-        break;
-      default: return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown convolutional kernel type");
-    }
+    auto launch = [&](auto filt_c) -> bool {
+      constexpr bool F = decltype(filt_c)::value;
+      switch (kernel_type) {
+        case KFLAT: k_density<D, METRIC, KFLAT, U, true, false, F><<<grid, kThreads, 0, st>>>(a); return true;
+        case KGAUSS:
+          if (unit) k_density<D, METRIC, KGAUSS, U, true, METRIC == MEUCLID, F><<<grid, kThreads, 0, st>>>(a);
+          else k_density<D, METRIC, KGAUSS, U, true, false, F><<<grid, kThreads, 0, st>>>(a);
+          return true;
+        case KEXP:
+          if (unit) k_density<D, METRIC, KEXP, U, true, METRIC == MEUCLID, F><<<grid, kThreads, 0, st>>>(a);
+          else k_density<D, METRIC, KEXP, U, true, false, F><<<grid, kThreads, 0, st>>>(a);
+          return true;
+      }
+      return false;
+    };
+    bool ok;
+    if constexpr (U && METRIC == MEUCLID) ok = a.dfilter ? launch(std::true_type{}) : launch(std::false_type{});
+    else ok = launch(std::false_type{});
+    if (!ok) return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown convolutional kernel type");
   } else {
     switch (kernel_type) {
       case KFLAT: k_density<D, METRIC, KFLAT, U, false><<<grid, kThreads, 0, st>>>(a); break;

@@ -122,7 +122,11 @@ typedef struct {
  *  tile_call_points    calls of at most this many points in 1-3 dimensions skip
  *                      the Morton layout and the block search of the
  *                      nearest-higher pass and walk the tiles instead (one wave
- *                      per query when small_call_points holds them); 0: never */
+ *                      per query when small_call_points holds them); 0: never
+ *  density_filter      1: the density pass's union walk (1-3 dimensions,
+ *                      Euclidean) drops the candidates beyond the radius from
+ *                      the box of the wave's queries before evaluating any;
+ *                      0: every candidate of the union is evaluated */
 typedef struct {
   float nh_first_radius;
   int32_t morton_bits;
@@ -133,6 +137,7 @@ typedef struct {
   int32_t small_call_points;
   int32_t wave_dims;
   int32_t tile_call_points;
+  int32_t density_filter;
 } clue_tuning;
 
 typedef struct clue_plan_s* clue_plan_t;

@@ -130,7 +130,8 @@ TUNINGS = [dict(nh_first_radius=0.05, morton_bits=5),
            dict(small_call_points=0, wave_dims=10, tile_call_points=0),
            dict(small_call_points=1 << 30, nh_first_radius=0.3, tile_call_points=0),
            dict(small_call_points=1 << 30, tile_call_points=1 << 30),
-           dict(small_call_points=0, tile_call_points=1 << 30)]
+           dict(small_call_points=0, tile_call_points=1 << 30),
+           dict(density_filter=0, density_order=1)]
 
 
 @pytest.mark.parametrize("tuning", TUNINGS)
@@ -248,6 +249,27 @@ def test_wrapped_coordinates(ref_data):
     assert_same(*run_both(c, w, 0.2, 5.0, 0.2, wrapped=[0, 1]))
     ps = datasets.blobs(5000, 2, 8, 0.0, 50.0, 1.0, 0.1, ("const", 1.0), 4242)
     assert_same(*run_both(ps.coords, ps.weights, 1.0, 3.0, 2.0, wrapped=[1, 1]))
+
+
+def test_in_radius_outside_box():
+    # q = 0.75 and c = -0.25 - 2^-25 are exactly dc = 1 apart after rounding
+    # (fl(q - c) = 1), but c's tile lies below q's search box: its lower edge
+    # fl(q - dc) = -0.25 is a tile boundary (x tiles of 0.5 from -0.75).  The
+    # reference never visits c for q; the density's filtered walk must not add it.
+    f32 = np.float32
+    q, c = f32(0.75), f32(-0.25) - f32(2.0 ** -25)
+    x = np.array([q, c, -0.75, 0.0, 0.5, -0.5, 0.25, 0.7, -0.7], f32)
+    y = np.array([0.0, 0.0, 1.0, 2.0, 0.5, 1.5, 1.0, 2.0, 0.2], f32)
+    assert np.sqrt(f32(q - c) * f32(q - c)) <= f32(1.0)
+    ts = (x.max() - x.min()) / f32(3)
+    assert int((f32(q - f32(1.0)) - x.min()) / ts) == 1 and int((c - x.min()) / ts) == 0
+    coords = np.stack([x, y])
+    w = np.ones(9, f32)
+    for tuning in (None, dict(small_call_points=0, wave_dims=10, tile_call_points=0),
+                   dict(small_call_points=0, wave_dims=10, tile_call_points=0, density_filter=0)):
+        got, ref, *rest = run_both(coords, w, 1.0, 0.5, 1.0, ppb=1, tuning=tuning)
+        assert ref.grid["npd"] == 3 and ref.rho[0] == f32(1.0) + f32(0.5)  # itself and (0.5, 0.5)
+        assert_same(got, ref, *rest)
 
 
 def test_edge_cases():
