@@ -168,17 +168,19 @@ __device__ __forceinline__ float engine_expf_nonpos(float x, const double* table
 }
 
 // The correctly rounded float exp of x in [-14, 0], a shorter evaluation for
-// the kernels' in-radius terms once the host has bounded their argument: the
-// reduction with one constant (ln2/64 rounded to double: |k| <= 1293 keeps its
-// error below 2^-49 of the result) and exp(r) - 1 as
-// r + r^2 (1/2 + r/6 + r^2 (1 + r/5) / 24), whose additive constants 0.5 and 1.0
-// are inline operands (no register copies).  Checked on every float in
-// [-14, 0] against long double rounded once to float: the same bits as
-// engine_expf, the correctly rounded value.
+// the kernels' in-radius terms once the host has bounded their argument: k is
+// x*64/ln2 rounded to an integer by adding 1.5*2^52 in the fused multiply-add
+// (k is then the low word of the sum, no conversion), the reduction uses one
+// constant (ln2/64 rounded to double: |k| <= 1293 keeps its error below 2^-49
+// of the result) and exp(r) - 1 is r + r^2 (1/2 + r/6 + r^2 (1 + r/5) / 24),
+// whose additive constants 0.5 and 1.0 are inline operands (no register
+// copies).  Checked on every float in [-14, 0] against long double rounded once
+// to float: the same bits as engine_expf, the correctly rounded value.
 __device__ __forceinline__ float engine_expf_unit(float x, const double* table = kExp2Table) {
   const double xd = static_cast<double>(x);
-  const double kd = __builtin_rint(xd * 0x1.71547652b82fep+6);
-  const int k = static_cast<int>(kd);
+  const double sh = __builtin_fma(xd, 0x1.71547652b82fep+6, 0x1.8p+52);
+  const int k = static_cast<int>(static_cast<uint32_t>(__builtin_bit_cast(uint64_t, sh)));
+  const double kd = sh - 0x1.8p+52;
   const double r = __builtin_fma(-kd, 0x1.62e42fefa39efp-7, xd);
   const double h = __builtin_fma(r, 1.0 / 6.0, 0.5);
   const double q = __builtin_fma(r, 0.2, 1.0) * (1.0 / 24.0);
@@ -216,12 +218,17 @@ enum { KFLAT = 0, KGAUSS = 1, KEXP = 2 };
 // rcp_den: 1 / ((2*sigma)*sigma) rounded to double (Gaussian only).
 // NONPOS: the exp argument is known to be <= 0 and not NaN (checked on the host).
 // UNIT: also the amplitude is exactly 1 (amplitude * e == e) and the argument is
-// known to lie in [-14, 0] for every in-radius distance (checked on the host).
-template <int KERNEL, bool NONPOS = false, bool UNIT = false>
+// known to lie in [-14, 0] for every in-radius distance (checked on the host);
+// 2 (Gaussian): also the mean is 0 and (2 sigma) sigma a power of two.
+template <int KERNEL, bool NONPOS = false, int UNIT = 0>
 __device__ __forceinline__ float kernel_value(float dist, float p0, float p1, float p2, double rcp_den,
                                               const double* table = kExp2Table) {
   if constexpr (KERNEL == KFLAT) {
     return p0;
+  } else if constexpr (KERNEL == KGAUSS && UNIT == 2) {
+    // mean 0 and (2 sigma) sigma = 2^e: the quotient -(d*d) / 2^e is the exact
+    // scaling of d*d by -2^-e (rcp_den), so one float product gives its bits
+    return engine_expf_unit((dist * dist) * static_cast<float>(-rcp_den), table);
   } else if constexpr (KERNEL == KGAUSS) {
     const float a = dist - p0;
     const float num = (-a) * a;  // -(d-mu)*(d-mu), divided by (2*sigma)*sigma

@@ -1292,6 +1292,13 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
     }
     const float amp = kd.type == CLUE_KERNEL_GAUSSIAN ? kd.params[2] : kd.params[1];
     a.exp_unit = amp == 1.0f && lowest * 1.001 <= 14.0;
+    if (a.exp_unit && kd.type == CLUE_KERNEL_GAUSSIAN && kd.params[0] == 0.f) {
+      // (2 sigma) sigma = 2^e with 2^-e a float: -(d*d) / den is d*d scaled exactly
+      const float den = (2.f * kd.params[1]) * kd.params[1];
+      int e = 0;
+      const float mant = std::frexp(den, &e);  // den = mant * 2^e, mant in [0.5, 1)
+      if (std::isfinite(den) && den > 0.f && mant == 0.5f && e - 1 <= 149 && e - 1 >= -127) a.exp_unit = 2;
+    }
   }
   for (int d = 0; d < CLUE_MAX_DIM; ++d) a.mp[d] = md.params[d];
   if (blocks) {

@@ -164,7 +164,8 @@ struct PassArgs {
   float* delta_out;          // nearest-higher distance: original order
   int xcd;                   // bit 0: XCD-aware block order of density, bit 1: of nearest-higher
   int exp_nonpos;            // the kernel's exp argument is always <= 0 and never NaN
-  int exp_unit;              // and >= -14 for in-radius distances, with amplitude exactly 1
+  int exp_unit;              // and >= -14 for in-radius distances, with amplitude exactly 1;
+                             // 2: a Gaussian of mean 0 and power-of-two (2 sigma) sigma besides
   const int32_t* qorder;     // density: query blocks in decreasing estimated cost (or null)
   const int32_t* tnf;        // density: 1 for tiles holding an infinite or NaN weight
   const int32_t* qperm;      // tile-order walks: query slots by growing search-box size (or null)

@@ -109,8 +109,9 @@ __global__ __launch_bounds__(kThreads) void k_box_class(const PassArgs a, float 
 
 // QNH: the lanes take their queries in Morton order (spatially coherent waves,
 // so the lanes of a wave share most of their search box); otherwise in tile order.
-// UNIT: Euclidean metric, exp argument in [-110, 0] and amplitude 1 (exp_unit).
-template <int D, int METRIC, int KERNEL, bool UNION, bool QNH, bool UNIT = false, bool FILT = false>
+// UNIT: Euclidean metric, exp argument in [-14, 0] and amplitude 1 (exp_unit);
+// 2: also a Gaussian of mean 0 and power-of-two (2 sigma) sigma.
+template <int D, int METRIC, int KERNEL, bool UNION, bool QNH, int UNIT = 0, bool FILT = false>
 __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
   // the exponential's 2^(j/64) table, read through LDS by the kernel evaluations
   __shared__ double s_exp2[KERNEL == KFLAT ? 1 : 64];
@@ -123,7 +124,7 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
   constexpr int kRing = 128;  // entries per wave; a batch never wraps (kRing % kFB == 0)
   constexpr int kFB = 8;      // candidates per evaluation batch
   // kernel value 1 at distance 0, the self term's value: the self test is not needed
-  constexpr bool kSelfFree = UNIT && KERNEL == KEXP;
+  constexpr bool kSelfFree = (UNIT != 0 && KERNEL == KEXP) || (UNIT == 2 && KERNEL == KGAUSS);
   __shared__ float4 s_ring[kFilter ? kThreads / kWave : 1][kFilter ? kRing : 1];
   __shared__ int32_t s_ringj[kFilter && D == 3 ? kThreads / kWave : 1][kFilter && D == 3 ? kRing : 1];
   const int qb = a.qorder ? a.qorder[blockIdx.x] : logical_block(a.xcd & 1);
@@ -175,7 +176,7 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
       } else {
         if (s2 <= a.thr) {
           float k = 1.f;
-          if (j != s) {
+          if (!SELF || j != s) {
             if ((UNIT || a.exp_nonpos) && s2 >= 0x1p-96f) {
               k = kernel_value<KERNEL, true, UNIT>(sqrt_rn_normal(s2), a.kp[0], a.kp[1], a.kp[2], a.kden_rcp,
                                                    s_exp2);
@@ -326,11 +327,11 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
         for (int base = beg; base < end; base += kWave) {
           const int j = base + lane;
           bool pass = j < end;
+          const int jl = pass ? j : end - 1;  // loads stay inside the tile
           float c[D];
-          float v = 0.f;
 #pragma unroll
-          for (int d = 0; d < D; ++d) c[d] = pass ? a.xs[d][j] : 0.f;
-          if (pass) v = kUsesHW ? a.hw[j] : a.ws[j];
+          for (int d = 0; d < D; ++d) c[d] = a.xs[d][jl];
+          const float v = kUsesHW ? a.hw[jl] : a.ws[jl];
 #pragma unroll
           for (int d = 0; d < D; ++d) pass = pass && (qlo[d] - c[d] <= R) && (c[d] - qhi[d] <= R);
           const uint64_t m = __ballot(pass);
@@ -377,7 +378,7 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
 // (ClusteringKernels.hpp:38-44).  With few queries the lane-per-query pass
 // leaves most of the GPU idle and its time is the longest query's chain of
 // terms; here that chain is one add per in-radius term.
-template <int D, int METRIC, int KERNEL, bool QNH, bool UNIT>
+template <int D, int METRIC, int KERNEL, bool QNH, int UNIT>
 __global__ __launch_bounds__(kThreads) void k_density_wave(const PassArgs a) {
   __shared__ double s_exp2[KERNEL == KFLAT ? 1 : 64];
   if constexpr (KERNEL != KFLAT) {
@@ -1302,16 +1303,17 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
     const int wgrid = static_cast<int>(div_up(a.n, kThreads / kWave));
     if constexpr (METRIC == MEUCLID) {
       if (blocks) {
-        const bool unit = a.exp_unit != 0;
+        const int unit = a.exp_unit;
         switch (kernel_type) {
-          case KFLAT: k_density_wave<D, METRIC, KFLAT, true, false><<<wgrid, kThreads, 0, st>>>(a); break;
+          case KFLAT: k_density_wave<D, METRIC, KFLAT, true, 0><<<wgrid, kThreads, 0, st>>>(a); break;
           case KGAUSS:
-            if (unit) k_density_wave<D, METRIC, KGAUSS, true, true><<<wgrid, kThreads, 0, st>>>(a);
-            else k_density_wave<D, METRIC, KGAUSS, true, false><<<wgrid, kThreads, 0, st>>>(a);
+            if (unit == 2) k_density_wave<D, METRIC, KGAUSS, true, 2><<<wgrid, kThreads, 0, st>>>(a);
+            else if (unit) k_density_wave<D, METRIC, KGAUSS, true, 1><<<wgrid, kThreads, 0, st>>>(a);
+            else k_density_wave<D, METRIC, KGAUSS, true, 0><<<wgrid, kThreads, 0, st>>>(a);
             break;
           case KEXP:
-            if (unit) k_density_wave<D, METRIC, KEXP, true, true><<<wgrid, kThreads, 0, st>>>(a);
-            else k_density_wave<D, METRIC, KEXP, true, false><<<wgrid, kThreads, 0, st>>>(a);
+            if (unit) k_density_wave<D, METRIC, KEXP, true, 1><<<wgrid, kThreads, 0, st>>>(a);
+            else k_density_wave<D, METRIC, KEXP, true, 0><<<wgrid, kThreads, 0, st>>>(a);
             break;
           default: return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown convolutional kernel type");
         }
@@ -1320,9 +1322,9 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
       }
     }
     switch (kernel_type) {
-      case KFLAT: k_density_wave<D, METRIC, KFLAT, false, false><<<wgrid, kThreads, 0, st>>>(a); break;
-      case KGAUSS: k_density_wave<D, METRIC, KGAUSS, false, false><<<wgrid, kThreads, 0, st>>>(a); break;
-      case KEXP: k_density_wave<D, METRIC, KEXP, false, false><<<wgrid, kThreads, 0, st>>>(a); break;
+      case KFLAT: k_density_wave<D, METRIC, KFLAT, false, 0><<<wgrid, kThreads, 0, st>>>(a); break;
+      case KGAUSS: k_density_wave<D, METRIC, KGAUSS, false, 0><<<wgrid, kThreads, 0, st>>>(a); break;
+      case KEXP: k_density_wave<D, METRIC, KEXP, false, 0><<<wgrid, kThreads, 0, st>>>(a); break;
       default: return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown convolutional kernel type");
     }
     CLUE_LAUNCH_CHECK("density_wave");
@@ -1341,19 +1343,21 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
     a.qorder = p->qorder.as<int32_t>();
   }
   if (blocks) {
-    bool unit = false;
-    if constexpr (METRIC == MEUCLID) unit = a.exp_unit != 0;
+    int unit = 0;
+    if constexpr (METRIC == MEUCLID) unit = a.exp_unit;
+    constexpr int kU1 = METRIC == MEUCLID ? 1 : 0, kU2 = METRIC == MEUCLID ? 2 : 0;
     auto launch = [&](auto filt_c) -> bool {
       constexpr bool F = decltype(filt_c)::value;
       switch (kernel_type) {
-        case KFLAT: k_density<D, METRIC, KFLAT, U, true, false, F><<<grid, kThreads, 0, st>>>(a); return true;
+        case KFLAT: k_density<D, METRIC, KFLAT, U, true, 0, F><<<grid, kThreads, 0, st>>>(a); return true;
         case KGAUSS:
-          if (unit) k_density<D, METRIC, KGAUSS, U, true, METRIC == MEUCLID, F><<<grid, kThreads, 0, st>>>(a);
-          else k_density<D, METRIC, KGAUSS, U, true, false, F><<<grid, kThreads, 0, st>>>(a);
+          if (unit == 2) k_density<D, METRIC, KGAUSS, U, true, kU2, F><<<grid, kThreads, 0, st>>>(a);
+          else if (unit) k_density<D, METRIC, KGAUSS, U, true, kU1, F><<<grid, kThreads, 0, st>>>(a);
+          else k_density<D, METRIC, KGAUSS, U, true, 0, F><<<grid, kThreads, 0, st>>>(a);
           return true;
         case KEXP:
-          if (unit) k_density<D, METRIC, KEXP, U, true, METRIC == MEUCLID, F><<<grid, kThreads, 0, st>>>(a);
-          else k_density<D, METRIC, KEXP, U, true, false, F><<<grid, kThreads, 0, st>>>(a);
+          if (unit) k_density<D, METRIC, KEXP, U, true, kU1, F><<<grid, kThreads, 0, st>>>(a);
+          else k_density<D, METRIC, KEXP, U, true, 0, F><<<grid, kThreads, 0, st>>>(a);
           return true;
       }
       return false;

@@ -14,13 +14,17 @@ float oracle_expf_engine(float x);
 float oracle_expf_longdouble(float x);
 
 /* The GPU density pass's evaluation for arguments in [-14, 0] (engine_expf_unit
- * in clue_device.hpp), restated operation by operation: one-constant reduction,
- * exp(r) - 1 = r + r^2 (1/2 + r/6 + r^2 (1 + r/5) / 24). */
+ * in clue_device.hpp), restated operation by operation: k = x*64/ln2 rounded to
+ * an integer by adding 1.5*2^52 in one fused multiply-add (k is the low word),
+ * one-constant reduction, exp(r) - 1 = r + r^2 (1/2 + r/6 + r^2 (1 + r/5) / 24). */
 static double unit_table[64];
 static float expf_unit(float x) {
   const double xd = (double)x;
-  const double kd = rint(xd * 0x1.71547652b82fep+6);
-  const int k = (int)kd;
+  const double sh = fma(xd, 0x1.71547652b82fep+6, 0x1.8p+52);
+  uint64_t bits;
+  memcpy(&bits, &sh, 8);
+  const int k = (int)(uint32_t)bits;
+  const double kd = sh - 0x1.8p+52;
   const double r = fma(-kd, 0x1.62e42fefa39efp-7, xd);
   const double h = fma(r, 1.0 / 6.0, 0.5);
   const double q = fma(r, 0.2, 1.0) * (1.0 / 24.0);
