@@ -530,3 +530,17 @@ def test_periodic_known_answer(ref_data):
     r = oracle.make_clusters(c, w, p)
     assert r.stats["n_stored_seeds"] == 1
     assert int(r.cluster.max()) + 1 == 1
+
+
+def test_engine_exp_exhaustive():
+    # every float of [-110, 89] (the engine exp) and of [-14, 0] (the density
+    # pass's unit-amplitude evaluation, restated in the checker operation by
+    # operation) against long double expl rounded once to float
+    import os
+    import subprocess
+
+    oracle_dir = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
+    r = subprocess.run(["make", "-C", oracle_dir, "-s", "check-exp"], capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = [l for l in r.stdout.splitlines() if l.startswith("exhaustive:")]
+    assert len(lines) == 2 and all(l.endswith(", 0 mismatches") for l in lines), r.stdout
