@@ -72,7 +72,8 @@ class ClueTuning(ctypes.Structure):
                 ("xcd_remap", ctypes.c_int32), ("pending_blocks", ctypes.c_int32),
                 ("pending_group", ctypes.c_int32), ("density_order", ctypes.c_int32),
                 ("small_call_points", ctypes.c_int32), ("wave_dims", ctypes.c_int32),
-                ("tile_call_points", ctypes.c_int32), ("density_filter", ctypes.c_int32)]
+                ("tile_call_points", ctypes.c_int32), ("density_filter", ctypes.c_int32),
+                ("fused_event_points", ctypes.c_int32)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -1025,6 +1025,7 @@ struct BatchCall {
   float* rho_out = nullptr;
   int32_t* nh_out = nullptr;
   bool persistent = false;  // the plan's grow-only seed capacity applies (plain calls)
+  int32_t max_event_n = 0, max_event_tiles = 0;
 };
 
 clue_status describe_batch(const clue_params* prm, int32_t D, int32_t n_events, const int64_t* event_offsets,
@@ -1069,6 +1070,8 @@ clue_status describe_batch(const clue_params* prm, int32_t D, int32_t n_events, 
     if (npd <= 0 || T <= 0) return fail(CLUE_ERR_INVALID_ARGUMENT, "degenerate tile grid");
     total_tiles += T;
     n_chunks64 += div_up(ne, kChunk);
+    bc.max_event_n = std::max(bc.max_event_n, static_cast<int32_t>(ne));
+    bc.max_event_tiles = std::max(bc.max_event_tiles, T);
   }
   if (total_tiles >= (int64_t{1} << 30))
     return fail(CLUE_ERR_UNSUPPORTED, "more than 2^30 tiles in one call");
@@ -1178,14 +1181,6 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
   int32_t* seed_buf = bc.is_seed_out ? bc.is_seed_out : p->seed_tmp.as<int32_t>();
   int32_t* counters = p->counters.as<int32_t>();
 
-  // ---- bounding boxes and tile sizes
-  CLUE_MARK(0);
-  k_bbox_partial<<<n_chunks, kThreads, 0, st>>>(c, p->chunks.as<Chunk>(), D, p->bbox_part.as<float>());
-  CLUE_LAUNCH_CHECK("bbox_partial");
-  k_bbox_final<<<n_events, kWave * D, 0, st>>>(p->bbox_part.as<float>(), D, n_events,
-                                                p->events.as<EventDesc>());
-  CLUE_LAUNCH_CHECK("bbox_final");
-
   // The block search of the nearest-higher pass needs the Euclidean metric on a
   // grid without wrapping; other cases run the tile walk for both passes.
   const int tbits = bits_for(Ttot);
@@ -1194,6 +1189,61 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
   const bool small_tiles = n <= p->tuning.tile_call_points;
   const bool blocks = md.type == MEUCLID && !wrapped_any && D <= 3 && sub_bits >= 1 && !small_tiles;
   if (!blocks) sub_bits = -1;
+  // small calls: each event's layout and labels in one workgroup (two launches
+  // in place of about twenty)
+  const bool fused = small_tiles && bc.max_event_n <= p->tuning.fused_event_points &&
+                     bc.max_event_n <= kFusedMaxPoints && bc.max_event_tiles <= kFusedMaxTiles;
+  const float h = kd.type == CLUE_KERNEL_FLAT ? kd.params[0] : 0.f;
+  FusedArgs fa{};
+  if (fused) {
+    fa.c = c;
+    fa.w = weights;
+    fa.ev = p->events.as<EventDesc>();
+    fa.wrap = wrap;
+    fa.pev = p->pev.as<int32_t>();
+    fa.perm = p->perm.as<int32_t>();
+    fa.offsets = p->offsets.as<int32_t>();
+    fa.xs = p->xs.as<float>();
+    fa.ld = p->ld;
+    fa.ws = p->ws.as<float>();
+    fa.hw = p->hw.as<float>();
+    fa.h = h;
+    fa.tnf = p->tnf.as<int32_t>();
+    fa.sc = p->sc.as<int32_t>();
+    fa.counters = counters;
+    fa.clear_cap = p->reset_capacity && persistent_capacity ? 1 : 0;
+    fa.rho = rho_buf;
+    fa.delta = p->delta.as<float>();
+    fa.nh = nh_buf;
+    fa.is_seed = seed_buf;
+    fa.cluster = cluster_out;
+    fa.seed_dc = prm->seed_dc;
+    fa.rhoc = prm->rhoc;
+    for (int d = 0; d < CLUE_MAX_DIM; ++d) fa.mp[d] = md.params[d];
+    fa.unbounded = prm->seed_mode == CLUE_SEEDS_UNBOUNDED ? 1 : 0;
+    fa.persistent = persistent_capacity ? 1 : 0;
+  }
+  int32_t* sorted_keys = nullptr;
+  int32_t* perm = nullptr;     // slot -> point id
+  int32_t* mkeys = nullptr;
+  int32_t* slot_of = nullptr;  // Morton position -> tile-order slot
+  if (fused) {
+    CLUE_MARK(0);
+    rc = dispatch_dim(D, [&](auto dc_) { return launch_fused_layout<decltype(dc_)::value>(fa, n_events, st); });
+    if (rc != CLUE_OK) return rc;
+    CLUE_MARK(1);
+    CLUE_MARK(2);
+    CLUE_MARK(3);
+    perm = p->perm.as<int32_t>();
+    p->last_perm = perm;
+  } else {
+  // ---- bounding boxes and tile sizes
+  CLUE_MARK(0);
+  k_bbox_partial<<<n_chunks, kThreads, 0, st>>>(c, p->chunks.as<Chunk>(), D, p->bbox_part.as<float>());
+  CLUE_LAUNCH_CHECK("bbox_partial");
+  k_bbox_final<<<n_events, kWave * D, 0, st>>>(p->bbox_part.as<float>(), D, n_events,
+                                                p->events.as<EventDesc>());
+  CLUE_LAUNCH_CHECK("bbox_final");
 
   // ---- tile ids + histogram (+ Morton keys, point -> event)
   CLUE_MARK(1);
@@ -1204,8 +1254,6 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
 
   // ---- stable counting sort by tile id: the reference's CSR order
   CLUE_MARK(2);
-  int32_t* sorted_keys = nullptr;
-  int32_t* perm = nullptr;  // slot -> point id
   rc = radix_sort(p, st, n, tbits, p->key.as<int32_t>(), p->key2.as<int32_t>(), p->perm.as<int32_t>(),
                   p->perm2.as<int32_t>(), true, &sorted_keys, &perm);
   if (rc != CLUE_OK) return rc;
@@ -1217,7 +1265,6 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
   CLUE_LAUNCH_CHECK("tile_offsets");
 
   // ---- tile-ordered SoA (+ Morton-ordered SoA and block boxes)
-  const float h = kd.type == CLUE_KERNEL_FLAT ? kd.params[0] : 0.f;
   rc = dispatch_dim(D, [&](auto dc_) {
     return launch_gather<decltype(dc_)::value>(p, perm, n, h, blocks, st);
   });
@@ -1226,8 +1273,6 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
   k_tile_nonfinite<<<div_up(n, kThreads), kThreads, 0, st>>>(p->ws.as<float>(), sorted_keys, n,
                                                               p->tnf.as<int32_t>());
   CLUE_LAUNCH_CHECK("tile_nonfinite");
-  int32_t* mkeys = nullptr;
-  int32_t* slot_of = nullptr;  // Morton position -> tile-order slot
   if (blocks) {
     // Morton layout: the tile-ordered slots sorted by (tile id, Morton code inside the tile)
     rc = radix_sort(p, st, n, tbits + sub_bits * D, p->kn2.as<int32_t>(), p->kn.as<int32_t>(),
@@ -1254,10 +1299,11 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
         p->tbox_lo.as<float>(), p->tbox_hi.as<float>());
     CLUE_LAUNCH_CHECK("tile_boxes");
   }
-
   // ---- density + nearest-higher
   k_fill_i32<<<div_up(n_events, kThreads), kThreads, 0, st>>>(p->sc.as<int32_t>(), 0, n_events);
   CLUE_LAUNCH_CHECK("fill");
+  }  // the layout of calls not fused
+
   PassArgs a{};
   for (int d = 0; d < D; ++d) a.xs[d] = p->xs.as<float>() + d * p->ld;
   a.ws = p->ws.as<float>();
@@ -1340,6 +1386,21 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
 
   // ---- seeds, ranks, labels
   CLUE_MARK(6);
+  if (fused) {
+    rc = dispatch_dim(D, [&](auto dc_) {
+      return dispatch_fused_labels<decltype(dc_)::value>(md.type, fa, n_events, st);
+    });
+    if (rc != CLUE_OK) return rc;
+    if (persistent_capacity) p->reset_capacity = false;
+    CLUE_MARK(7);
+    CLUE_MARK(8);
+    p->has_run = true;
+    p->last_n = n;
+    p->last_d = D;
+    p->last_T = Ttot;
+    p->last_events = n_events;
+    return CLUE_OK;
+  }
   ClassifyArgs ca{};
   ca.c = c;
   ca.rho = rho_buf;
@@ -1823,7 +1884,8 @@ clue_status clue_plan_set_tuning(clue_plan_t plan, const clue_tuning* tuning) {
        tuning->pending_group != 32 && tuning->pending_group != 64) ||
       tuning->density_order < -1 || tuning->density_order > 1 || tuning->small_call_points < 0 ||
       tuning->wave_dims < 0 || tuning->wave_dims > CLUE_MAX_DIM || tuning->tile_call_points < 0 ||
-      tuning->density_filter < 0 || tuning->density_filter > 1)
+      tuning->density_filter < 0 || tuning->density_filter > 1 || tuning->fused_event_points < 0 ||
+      tuning->fused_event_points > kFusedMaxPoints)
     return fail(CLUE_ERR_INVALID_ARGUMENT, "invalid tuning values");
   plan->tuning = *tuning;
   return CLUE_OK;

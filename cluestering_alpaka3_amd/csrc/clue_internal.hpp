@@ -369,6 +369,58 @@ struct ClassifyArgs {
 };
 
 // ===========================================================================
+// Small calls fused per event: one workgroup of kFusedThreads builds an event's
+// layout (bounding box, tile ids, stable counting sort into the tile CSR,
+// tile-ordered copies), another its seeds, seed ranks, capacity and labels.
+// ===========================================================================
+constexpr int kFusedThreads = 1024;
+constexpr int kFusedWaves = kFusedThreads / kWave;
+constexpr int kFusedMaxPoints = 32768;  // per event: local indices fit 16 bits
+constexpr int kFusedMaxTiles = 2048;    // per event: 16 per-wave counters per tile in LDS
+
+struct FusedArgs {
+  Coords c;            // caller coordinates, original order
+  const float* w;      // caller weights
+  EventDesc* ev;       // per-event grids: extremes and tile sizes are written here
+  Wrap wrap;
+  int32_t* pev;        // point -> event
+  int32_t* perm;       // slot -> point id
+  int32_t* offsets;    // tile CSR over global tile ids
+  float* xs;           // tile-ordered coordinates, leading dimension ld
+  int64_t ld;
+  float* ws;           // tile-ordered weights
+  float* hw;           // tile-ordered h * w (flat kernel)
+  float h;
+  int32_t* tnf;        // tiles holding an infinite or NaN weight
+  int32_t* sc;         // per event: points without a nearest-higher (cleared by the layout)
+  int32_t* counters;   // call totals (cleared by the layout: [0], [1] when clear_cap, [2, 11))
+  int clear_cap;
+  // seeds and labels
+  const float* rho;    // original order
+  const float* delta;  // nearest-higher distance, original order
+  int32_t* nh;         // in: nearest-higher (call-wide ids); out: local to the event, -1 for seeds
+  int32_t* is_seed;
+  int32_t* cluster;
+  float seed_dc;
+  float rhoc;
+  float mp[kMaxDim];
+  int unbounded;       // every seed stored (no capacity truncation)
+  int persistent;      // the plan's grow-only capacity applies (plain calls)
+};
+
+// Lanes of the wave holding the same key (< 2^bits) as this one, among the
+// lanes with `in` set (none for a lane without).
+__device__ __forceinline__ uint64_t match_peers(int key, bool in, int bits) {
+  uint64_t peers = __ballot(in);
+  for (int b = 0; b < bits; ++b) {
+    const bool bit = (key >> b) & 1;
+    const uint64_t m = __ballot(bit);
+    peers &= bit ? m : ~m;
+  }
+  return in ? peers : 0;
+}
+
+// ===========================================================================
 // Host side
 // ===========================================================================
 extern thread_local std::string g_error;
@@ -439,7 +491,7 @@ struct clue_plan_s {
   DevBuf pending, pending_flag, qcost, qorder;
   // cluster -> points map (post-processing): sort keys and values
   DevBuf mkey, mkey2, mval, mval2;
-  clue_tuning tuning{0.f, -1, 7, 0, 0, -1, 65536, 3, 32768, 1};
+  clue_tuning tuning{0.f, -1, 7, 0, 0, -1, 65536, 3, 32768, 1, 12288};
   int64_t ldb = 0;  // leading dimension of the block boxes
   int64_t ld = 0;   // leading dimension of xs / xn
   // pinned staging ring of the event and chunk tables (a slot is reused once its
@@ -505,6 +557,10 @@ clue_status dispatch_metric_passes(int metric, const PassArgs& a, int kt, float 
                                    hipStream_t st, clue_plan_s* p, bool blocks);
 template <int D>
 clue_status dispatch_metric_classify(int metric, const ClassifyArgs& a, hipStream_t st);
+template <int D>
+clue_status launch_fused_layout(const FusedArgs& a, int32_t n_events, hipStream_t st);
+template <int D>
+clue_status dispatch_fused_labels(int metric, const FusedArgs& a, int32_t n_events, hipStream_t st);
 
 template <typename F>
 clue_status dispatch_dim(int D, F&& f) {

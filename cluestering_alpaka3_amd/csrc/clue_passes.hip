@@ -1167,6 +1167,240 @@ __global__ __launch_bounds__(kThreads) void k_classify(const ClassifyArgs a) {
 }
 
 // ===========================================================================
+// Small calls fused per event (one workgroup each).
+// ===========================================================================
+// Layout of one event: bounding box and tile sizes (k_bbox_partial,
+// k_bbox_final), tile ids (k_bin), the stable counting sort by tile id into the
+// CSR -- ascending point id inside a tile, the serial fill order -- and the
+// tile-ordered copies (k_radix_*, k_tile_offsets, k_gather, k_tile_nonfinite).
+// Each wave owns a contiguous range of the event's points: it counts them per
+// tile, the bases of a tile's waves follow in wave order after the scanned tile
+// offsets, and each wave ranks its points in order, 64 at a time, equal tiles by
+// lane order (ballot match).
+template <int D>
+__global__ __launch_bounds__(kFusedThreads) void k_fused_layout(const FusedArgs a) {
+  __shared__ uint16_t s_key[kFusedMaxPoints];
+  __shared__ uint16_t s_cnt[kFusedWaves][kFusedMaxTiles];
+  __shared__ float s_lo[kFusedWaves][D], s_hi[kFusedWaves][D];
+  __shared__ float s_mn[D], s_mx[D], s_ts[D];
+  __shared__ int s_scan[kFusedWaves];
+  const int e = blockIdx.x;
+  const int tid = threadIdx.x, lane = __lane_id(), w = tid / kWave;
+  const int begin = a.ev[e].begin, n = a.ev[e].n, T = a.ev[e].ntiles, npd = a.ev[e].npd;
+  const int tb = a.ev[e].tile_base;
+  int stride[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) stride[d] = a.ev[e].stride[d];
+  if (e == 0 && tid < 11 && (tid != 1 || a.clear_cap)) a.counters[tid] = 0;
+  if (tid == 0) a.sc[e] = 0;
+  // bounding box: NaN is never selected
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    float lo = __builtin_inff(), hi = -__builtin_inff();
+    for (int i = tid; i < n; i += kFusedThreads) {
+      const float v = a.c.p[d][begin + i];
+      lo = v < lo ? v : lo;
+      hi = hi < v ? v : hi;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const float olo = __shfl_xor(lo, off), ohi = __shfl_xor(hi, off);
+      lo = olo < lo ? olo : lo;
+      hi = hi < ohi ? ohi : hi;
+    }
+    if (lane == 0) {
+      s_lo[w][d] = lo;
+      s_hi[w][d] = hi;
+    }
+  }
+  __syncthreads();
+  if (tid < D) {
+    float lo = s_lo[0][tid], hi = s_hi[0][tid];
+    for (int k = 1; k < kFusedWaves; ++k) {
+      lo = s_lo[k][tid] < lo ? s_lo[k][tid] : lo;
+      hi = hi < s_hi[k][tid] ? s_hi[k][tid] : hi;
+    }
+    const float ts = (hi - lo) / static_cast<float>(npd);  // ComputeTiles.hpp:21-26
+    s_mn[tid] = lo;
+    s_mx[tid] = hi;
+    s_ts[tid] = ts;
+    a.ev[e].mn[tid] = lo;
+    a.ev[e].mx[tid] = hi;
+    a.ev[e].ts[tid] = ts;
+  }
+  for (int t = tid; t < kFusedWaves * T; t += kFusedThreads) s_cnt[t / T][t % T] = 0;
+  __syncthreads();
+  // tile ids (local to the event)
+  for (int i = tid; i < n; i += kFusedThreads) {
+    int32_t t = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+      t += stride[d] * tile_bin(a.c.p[d][begin + i], s_mn[d], s_mx[d], s_ts[d], npd, a.wrap.w[d] != 0);
+    s_key[i] = static_cast<uint16_t>(t);
+    a.pev[begin + i] = e;
+  }
+  __syncthreads();
+  int kbits = 0;
+  while ((1 << kbits) < T) ++kbits;
+  const int span = (n + kFusedWaves - 1) / kFusedWaves;
+  const int r0 = w * span, r1 = min(r0 + span, n);
+  // per-wave counts: one lane per distinct tile of each 64 points adds their number
+  for (int base = r0; base < r1; base += kWave) {
+    const int i = base + lane;
+    const bool in = i < r1;
+    const int k = in ? s_key[i] : 0;
+    const uint64_t peers = match_peers(k, in, kbits);
+    if (in && mbcnt64(peers) == 0) s_cnt[w][k] = static_cast<uint16_t>(s_cnt[w][k] + __builtin_popcountll(peers));
+  }
+  __syncthreads();
+  // tile offsets (two tiles per thread: tid and tid + kFusedThreads), then the
+  // base of each wave inside each tile
+  const int t0 = tid, t1 = tid + kFusedThreads;
+  int c0 = 0, c1 = 0;
+  if (t0 < T)
+    for (int k = 0; k < kFusedWaves; ++k) c0 += s_cnt[k][t0];
+  if (t1 < T)
+    for (int k = 0; k < kFusedWaves; ++k) c1 += s_cnt[k][t1];
+  int total0, total1;
+  const int x0 = block_incl_scan<kFusedThreads>(c0, s_scan, total0) - c0;
+  const int x1 = total0 + block_incl_scan<kFusedThreads>(c1, s_scan, total1) - c1;
+  if (t0 < T) {
+    int run = x0;
+    for (int k = 0; k < kFusedWaves; ++k) {
+      const int c = s_cnt[k][t0];
+      s_cnt[k][t0] = static_cast<uint16_t>(run);
+      run += c;
+    }
+    a.offsets[tb + t0] = begin + x0;
+    a.tnf[tb + t0] = 0;
+  }
+  if (t1 < T) {
+    int run = x1;
+    for (int k = 0; k < kFusedWaves; ++k) {
+      const int c = s_cnt[k][t1];
+      s_cnt[k][t1] = static_cast<uint16_t>(run);
+      run += c;
+    }
+    a.offsets[tb + t1] = begin + x1;
+    a.tnf[tb + t1] = 0;
+  }
+  if (tid == 0) a.offsets[tb + T] = begin + n;
+  __syncthreads();
+  // stable scatter: every lane reads its tile's base before the lowest lane of
+  // the tile's peers moves it on
+  for (int base = r0; base < r1; base += kWave) {
+    const int i = base + lane;
+    const bool in = i < r1;
+    const int k = in ? s_key[i] : 0;
+    const uint64_t peers = match_peers(k, in, kbits);
+    const int before = mbcnt64(peers);
+    const int rb = in ? s_cnt[w][k] : 0;
+    __builtin_amdgcn_wave_barrier();
+    if (in && before == 0) s_cnt[w][k] = static_cast<uint16_t>(rb + __builtin_popcountll(peers));
+    __builtin_amdgcn_wave_barrier();
+    if (in) {
+      const int slot = begin + rb + before;
+      const int gi = begin + i;
+      a.perm[slot] = gi;
+#pragma unroll
+      for (int d = 0; d < D; ++d) a.xs[d * a.ld + slot] = a.c.p[d][gi];
+      const float wv = a.w[gi];
+      a.ws[slot] = wv;
+      a.hw[slot] = a.h * wv;  // FlatKernel term k*w with k = h
+      if (!__builtin_isfinite(wv)) a.tnf[tb + k] = 1;
+    }
+  }
+}
+
+// Seeds, seed ranks, capacity and labels of one event (k_classify,
+// scan_exclusive of the flags, k_capacity, k_jump, k_labels): roots by pointer
+// jumping in LDS, every step replacing a parent by one of its ancestors.
+template <int D, int METRIC>
+__global__ __launch_bounds__(kFusedThreads) void k_fused_labels(const FusedArgs a) {
+  __shared__ int16_t s_root[kFusedMaxPoints];
+  __shared__ int16_t s_rank[kFusedMaxPoints];  // local seed rank, -1 for other points
+  __shared__ int s_scan[kFusedWaves];
+  __shared__ int s_used;
+  const int e = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int begin = a.ev[e].begin, n = a.ev[e].n;
+  for (int i = tid; i < n; i += kFusedThreads) {
+    const int gi = begin + i;
+    const int32_t nhv = a.nh[gi];
+    float dist;
+    if (nhv == -1) {
+      // metric(point_i, PointsView[-1]) with the all-FLT_MAX sentinel (PointsCommon.hpp:75-81)
+      float x[D], m[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        x[d] = a.c.p[d][gi];
+        m[d] = FLT_MAX;
+      }
+      dist = metric_distance<D, METRIC>(x, m, a.mp);
+    } else {
+      dist = a.delta[gi];
+    }
+    const bool seed = (dist > a.seed_dc) && (a.rho[gi] >= a.rhoc);  // ClusteringKernels.hpp:233
+    a.is_seed[gi] = seed ? 1 : 0;
+    s_root[i] = static_cast<int16_t>((seed || nhv == -1) ? i : nhv - begin);
+    s_rank[i] = seed ? 1 : 0;
+    a.nh[gi] = (seed || nhv < 0) ? -1 : nhv - begin;
+  }
+  __syncthreads();
+  // seed ranks in point order (each thread 32 consecutive points)
+  const int c0 = tid * (kFusedMaxPoints / kFusedThreads);
+  int cnt = 0;
+  for (int k = 0; k < kFusedMaxPoints / kFusedThreads; ++k)
+    if (c0 + k < n) cnt += s_rank[c0 + k];
+  int ns;
+  int r = block_incl_scan<kFusedThreads>(cnt, s_scan, ns) - cnt;
+  for (int k = 0; k < kFusedMaxPoints / kFusedThreads; ++k) {
+    if (c0 + k < n) {
+      const bool sd = s_rank[c0 + k] != 0;
+      s_rank[c0 + k] = static_cast<int16_t>(sd ? r : -1);
+      r += sd ? 1 : 0;
+    }
+  }
+  if (tid == 0) {
+    const int32_t cand = a.sc[e];
+    int32_t cap = cand;  // SetupSeeds.hpp:14-18
+    if (a.persistent) {
+      cap = a.counters[1] > cand ? a.counters[1] : cand;
+      a.counters[1] = cap;
+    }
+    const int32_t used = a.unbounded ? ns : cap;
+    const int32_t stored = ns < used ? ns : used;
+    s_used = used;
+    atomicAdd(&a.counters[0], cand);
+    atomicAdd(&a.counters[2], used);
+    atomicAdd(&a.counters[3], ns);
+    atomicAdd(&a.counters[4], stored);
+    atomicAdd(&a.counters[5], ns - stored);
+  }
+  __syncthreads();
+  // each round halves every distance to a root: 16 rounds reach it from any
+  // depth below 2^15 (the nearest-higher graph is acyclic, a strict order on
+  // (density, id)); the bound only keeps a corrupt input from looping
+  for (int round = 0; round < 20; ++round) {
+    bool changed = false;
+    for (int i = tid; i < n; i += kFusedThreads) {
+      const int rt = s_root[i];
+      const int up = s_root[rt];
+      if (up != rt) {
+        s_root[i] = static_cast<int16_t>(up);
+        changed = true;
+      }
+    }
+    if (!__syncthreads_or(changed)) break;
+  }
+  const int used = s_used;
+  for (int i = tid; i < n; i += kFusedThreads) {
+    const int rk = s_rank[s_root[i]];
+    a.cluster[begin + i] = (rk >= 0 && rk < used) ? rk : -1;  // ClusteringKernels.hpp:252-279
+  }
+}
+
+// ===========================================================================
 // Launches
 // ===========================================================================
 template <int D>
@@ -1386,6 +1620,28 @@ clue_status launch_classify(const ClassifyArgs& a, hipStream_t st) {
 }
 
 template <int D>
+clue_status launch_fused_layout(const FusedArgs& a, int32_t n_events, hipStream_t st) {
+  k_fused_layout<D><<<n_events, kFusedThreads, 0, st>>>(a);
+  CLUE_LAUNCH_CHECK("fused_layout");
+  return CLUE_OK;
+}
+
+template <int D>
+clue_status dispatch_fused_labels(int metric, const FusedArgs& a, int32_t n_events, hipStream_t st) {
+  switch (metric) {
+    case MEUCLID: k_fused_labels<D, MEUCLID><<<n_events, kFusedThreads, 0, st>>>(a); break;
+    case MWEUCLID: k_fused_labels<D, MWEUCLID><<<n_events, kFusedThreads, 0, st>>>(a); break;
+    case MPERIODIC: k_fused_labels<D, MPERIODIC><<<n_events, kFusedThreads, 0, st>>>(a); break;
+    case MMANHATTAN: k_fused_labels<D, MMANHATTAN><<<n_events, kFusedThreads, 0, st>>>(a); break;
+    case MCHEBYSHEV: k_fused_labels<D, MCHEBYSHEV><<<n_events, kFusedThreads, 0, st>>>(a); break;
+    case MWCHEBYSHEV: k_fused_labels<D, MWCHEBYSHEV><<<n_events, kFusedThreads, 0, st>>>(a); break;
+    default: return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown distance metric type");
+  }
+  CLUE_LAUNCH_CHECK("fused_labels");
+  return CLUE_OK;
+}
+
+template <int D>
 clue_status dispatch_metric_passes(int metric, const PassArgs& a, int kt, float dc, float dm,
                                    hipStream_t st, clue_plan_s* p, bool blocks) {
   switch (metric) {
@@ -1417,6 +1673,8 @@ template clue_status launch_bin<CLUE_DIM>(const Coords&, const float*, int32_t, 
 template clue_status launch_gather<CLUE_DIM>(clue_plan_s*, const int32_t*, int32_t, float, bool, hipStream_t);
 template clue_status dispatch_metric_passes<CLUE_DIM>(int, const PassArgs&, int, float, float, hipStream_t,
                                                       clue_plan_s*, bool);
+template clue_status launch_fused_layout<CLUE_DIM>(const FusedArgs&, int32_t, hipStream_t);
+template clue_status dispatch_fused_labels<CLUE_DIM>(int, const FusedArgs&, int32_t, hipStream_t);
 template clue_status dispatch_metric_classify<CLUE_DIM>(int, const ClassifyArgs&, hipStream_t);
 
 }  // namespace clue_hip

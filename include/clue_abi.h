@@ -126,7 +126,13 @@ typedef struct {
  *  density_filter      1: the density pass's union walk (1-3 dimensions,
  *                      Euclidean) drops the candidates beyond the radius from
  *                      the box of the wave's queries before evaluating any;
- *                      0: every candidate of the union is evaluated */
+ *                      0: every candidate of the union is evaluated
+ *  fused_event_points  calls that skip the Morton layout (tile_call_points)
+ *                      and whose events all have at most this many points
+ *                      (and 2,048 tiles) build each event's layout, and later
+ *                      its seeds and labels, in one workgroup per event (two
+ *                      launches in place of about twenty); at most 32,768;
+ *                      0: never */
 typedef struct {
   float nh_first_radius;
   int32_t morton_bits;
@@ -138,6 +144,7 @@ typedef struct {
   int32_t wave_dims;
   int32_t tile_call_points;
   int32_t density_filter;
+  int32_t fused_event_points;
 } clue_tuning;
 
 typedef struct clue_plan_s* clue_plan_t;

@@ -131,7 +131,8 @@ TUNINGS = [dict(nh_first_radius=0.05, morton_bits=5),
            dict(small_call_points=1 << 30, nh_first_radius=0.3, tile_call_points=0),
            dict(small_call_points=1 << 30, tile_call_points=1 << 30),
            dict(small_call_points=0, tile_call_points=1 << 30),
-           dict(density_filter=0, density_order=1)]
+           dict(density_filter=0, density_order=1),
+           dict(fused_event_points=0)]
 
 
 @pytest.mark.parametrize("tuning", TUNINGS)
@@ -270,6 +271,34 @@ def test_in_radius_outside_box():
         got, ref, *rest = run_both(coords, w, 1.0, 0.5, 1.0, ppb=1, tuning=tuning)
         assert ref.grid["npd"] == 3 and ref.rho[0] == f32(1.0) + f32(0.5)  # itself and (0.5, 0.5)
         assert_same(got, ref, *rest)
+
+
+@pytest.mark.parametrize("fused", [32768, 0])
+def test_small_calls_fused_and_not(ref_data, fused):
+    # calls that skip the Morton layout: each event's layout and labels in one
+    # workgroup (up to fused_event_points points per event) or through the
+    # call-wide kernels
+    t = dict(fused_event_points=fused)
+    c, w = ref_data("toyDetector_1000")
+    for kernel in (("flat", 0.5), ("gaussian", 0.0, 1.0, 1.0), ("exponential", 1.0, 1.0)):
+        assert_same(*run_both(c, w, 4.0, 2.5, 4.0, kernel=kernel, tuning=t))
+    for ndim in (1, 3, 7, 10):
+        ps = datasets.blobs(6000, ndim, 5, 0.0, 30.0, 1.0, 0.1, ("uniform", 0.5, 1.5), 90 + ndim)
+        assert_same(*run_both(ps.coords, ps.weights, 1.5, 3.0, 2.5, kernel=("exponential", 0.7, 1.0), tuning=t))
+    # largest fused event (32,768 points) and a grid near the 2,048-tile limit
+    ps = datasets.workload_scaled("2d_1m_gauss", 32768, seed=5)
+    assert_same(*run_both(ps.coords, ps.weights, 2.0, 8.0, 4.0, kernel=("gaussian", 0.0, 1.0, 1.0), tuning=t))
+    ps = datasets.blobs(30000, 2, 40, 0.0, 300.0, 1.5, 0.2, ("const", 1.0), 17)
+    got, ref, *rest = run_both(ps.coords, ps.weights, 1.3, 2.0, 2.6, ppb=16, tuning=t)
+    assert ref.grid["ntiles"] <= 2048
+    assert_same(got, ref, *rest)
+    # wrapped coordinates and seeds beyond the capacity
+    ps = datasets.blobs(5000, 2, 8, 0.0, 50.0, 1.0, 0.1, ("const", 1.0), 4242)
+    assert_same(*run_both(ps.coords, ps.weights, 1.0, 3.0, 2.0, wrapped=[1, 1], tuning=t))
+    ps = datasets.blobs(20000, 2, 60, 0.0, 200.0, 1.0, 0.0, ("const", 1.0), 99)
+    got, ref, *rest = run_both(ps.coords, ps.weights, 0.6, 1.0, 5.0, seed_dc=0.6, tuning=t)
+    assert ref.stats["n_dropped_seeds"] > 0
+    assert_same(got, ref, *rest)
 
 
 def test_edge_cases():
