@@ -88,7 +88,7 @@ class ClueGridInfo(ctypes.Structure):
 class ClueServerConfig(ctypes.Structure):
     _fields_ = [("device_budget_bytes", ctypes.c_size_t), ("slots", ctypes.c_int32),
                 ("max_batch_points", ctypes.c_int32), ("max_batch_events", ctypes.c_int32),
-                ("max_dim", ctypes.c_int32)]
+                ("max_dim", ctypes.c_int32), ("small_batch_points", ctypes.c_int32)]
 
 
 class ClueRequest(ctypes.Structure):
@@ -523,8 +523,9 @@ class Server:
             self._keep = None
             return Result(*self.outputs)
 
-    def __init__(self, device=0, budget_bytes=0, slots=0, max_batch_points=0, max_batch_events=0, max_dim=0):
-        cfg = ClueServerConfig(budget_bytes, slots, max_batch_points, max_batch_events, max_dim)
+    def __init__(self, device=0, budget_bytes=0, slots=0, max_batch_points=0, max_batch_events=0, max_dim=0,
+                 small_batch_points=0):
+        cfg = ClueServerConfig(budget_bytes, slots, max_batch_points, max_batch_events, max_dim, small_batch_points)
         h = _vp()
         check(lib().clue_server_create(device, ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h.value

@@ -12,7 +12,10 @@
 // request fixes the batch's class (dimension, parameters, kernel, metric,
 // wrapping) and every pending request of that class is taken in arrival order
 // while the batch holds at most half the slot's capacity (a single request may
-// fill the slot).  Requests that arrive while slots are busy wait in
+// fill the slot).  With small_batch_points set, small requests (at most that
+// many points) form batches of their own of at most that many points (more when
+// many of them wait), so that a small request never waits for a large one of its
+// batch, and one slot is kept for them while the others hold large batches.  Requests that arrive while slots are busy wait in
 // the queue and join the next batch (continuous batching); the slots' streams
 // overlap one batch's copies with another's kernels.  A waiter thread per slot
 // blocks on the slot's completion event, scatters the outputs to the callers'
@@ -79,6 +82,7 @@ struct Slot {
   std::vector<int64_t> offsets;
   bool busy = false;
   bool launched = false;
+  bool large = false;  // the batch holds more than small_batch_points points
   std::condition_variable cv;
   std::thread waiter;
 };
@@ -192,26 +196,55 @@ void scheduler_loop(clue_server_s* s) {
   std::unique_lock<std::mutex> lk(s->mu);
   while (true) {
     Slot* slot = nullptr;
+    bool small_only = false;
+    const int64_t small = s->cfg.small_batch_points;
+    const int large_slots = s->slots.size() > 1 ? static_cast<int>(s->slots.size()) - 1 : 1;
     s->cv_sched.wait(lk, [&] {
       slot = nullptr;
-      for (auto& sl : s->slots)
+      small_only = false;
+      int large_busy = 0;
+      for (auto& sl : s->slots) {
         if (!sl->busy) {
-          slot = sl.get();
-          break;
+          if (!slot) slot = sl.get();
+        } else if (sl->large) {
+          ++large_busy;
         }
-      return (slot && !s->pending.empty()) || (s->stop && s->pending.empty());
+      }
+      if (s->pending.empty()) return s->stop;
+      if (!slot) return false;
+      if (small <= 0 || large_busy < large_slots) return true;
+      // every other slot holds a large batch: this one is kept for small requests
+      small_only = true;
+      for (auto& r : s->pending)
+        if (r->req.n <= small) return true;
+      return false;
     });
     if (s->pending.empty()) break;  // stopping, nothing left to launch
-    // batch: the oldest request's class, in arrival order, up to half a slot's
-    // capacity (a single request may fill the slot): under load, several
-    // batches then share the GPU, and one batch's copies overlap another's kernels
-    const clue_request first = s->pending.front()->req;
-    const int64_t target = s->cap / 2;
+    // batch: the oldest request's class (the oldest small request's when only
+    // small ones may start), in arrival order, up to half a slot's capacity (a
+    // single request may fill the slot) or small_batch_points for small
+    // requests: under load, several batches then share the GPU, and one batch's
+    // copies overlap another's kernels
+    auto first_it = s->pending.begin();
+    if (small_only)
+      while ((*first_it)->req.n > small) ++first_it;
+    const clue_request first = (*first_it)->req;
+    const bool small_first = small > 0 && first.n <= small;
+    int64_t target = small_first ? small : s->cap / 2;
+    if (small_first && !small_only) {
+      // many small requests waiting (saturation): one large batch of them costs less
+      int64_t waiting = 0;
+      for (auto& r : s->pending)
+        if (r->req.n <= small && same_class(r->req, first)) waiting += r->req.n;
+      if (waiting > small * static_cast<int64_t>(s->slots.size())) target = s->cap / 2;
+    }
     int64_t pts = 0;
     for (auto it = s->pending.begin(); it != s->pending.end();) {
       if (static_cast<int32_t>(slot->batch.size()) >= s->cfg.max_batch_events) break;
       const int64_t n = (*it)->req.n;
-      if (same_class((*it)->req, first) && pts + n <= s->cap && (slot->batch.empty() || pts + n <= target)) {
+      const bool size_ok = small <= 0 || ((n <= small) == small_first);
+      if (same_class((*it)->req, first) && size_ok && pts + n <= s->cap &&
+          (slot->batch.empty() || pts + n <= target)) {
         pts += (*it)->req.n;
         slot->batch.push_back(*it);
         it = s->pending.erase(it);
@@ -220,6 +253,7 @@ void scheduler_loop(clue_server_s* s) {
       }
     }
     slot->busy = true;
+    slot->large = small <= 0 || pts > small;
     s->stats.batches += 1;
     s->stats.max_batch_events = std::max<int64_t>(s->stats.max_batch_events, slot->batch.size());
     s->stats.max_batch_points = std::max<int64_t>(s->stats.max_batch_points, pts);

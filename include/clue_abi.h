@@ -301,6 +301,11 @@ typedef struct {
   int32_t max_batch_points;   /* per slot before the budget cap, 0: 8 Mi */
   int32_t max_batch_events;   /* 0: 4096 */
   int32_t max_dim;            /* largest dimension served, 0: 3 */
+  int32_t small_batch_points; /* > 0: requests of at most this many points are
+                                 batched only with each other, up to this many
+                                 points in all (unless many wait), so that a small
+                                 request never waits for a large one of its batch,
+                                 and one slot is kept for them; 0: one size class */
 } clue_server_config;
 
 /* One event.  coords: host (ndim, n) SoA, weights: host n floats; outputs:

@@ -78,3 +78,24 @@ def test_budget_caps_slot_capacity():
                                           clue.cluster_points(q.coords, q.weights, 1.0, 2.0, 2.0).cluster)
     finally:
         srv.close()
+
+
+@pytest.mark.parametrize("slots", [1, 3])
+def test_small_request_class(slots):
+    # small_batch_points: small requests batch only with each other (one slot kept
+    # for them); results still equal the requests alone
+    events = [datasets.served_event(k, 13, lo=500, hi=80000) for k in range(24)]
+    srv = clue.Server(0, slots=slots, small_batch_points=16384)
+    try:
+        tickets = [srv.submit(ps.coords, ps.weights, p["dc"], p["rhoc"], p["dm"], kernel=p["kernel"])
+                   for ps, p in events]
+        results = [t.result() for t in tickets]
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert st["events_done"] == len(events)
+    for k, ((ps, p), got) in enumerate(zip(events, results)):
+        ref = alone(ps, p)
+        np.testing.assert_array_equal(got.cluster, ref.cluster, err_msg=f"event {k}")
+        np.testing.assert_array_equal(got.rho.view(np.int32), ref.rho.view(np.int32), err_msg=f"event {k}")
+        np.testing.assert_array_equal(got.nearest_higher, ref.nearest_higher, err_msg=f"event {k}")
