@@ -50,6 +50,48 @@ __global__ __launch_bounds__(kThreads) void k_scan_reduce(const int32_t* in, int
   if (threadIdx.x == 0) part[blockIdx.x] = total;
 }
 
+// Short arrays (at most kScanSmall entries): one workgroup scans them and writes
+// out[n] (one launch instead of two; for longer arrays one workgroup is slower
+// than the two launches over many blocks).
+constexpr int kScanSmallItems = 16;
+constexpr int kScanSmallTile = 1024 * kScanSmallItems;
+constexpr int64_t kScanSmall = kScanSmallTile;
+__global__ __launch_bounds__(1024) void k_scan_small(const int32_t* in, int64_t n, int32_t* out) {
+  __shared__ int32_t tile[kScanSmallTile];
+  __shared__ int smem[1024 / kWave];
+  const int tid = threadIdx.x;
+  int carry = 0;
+  for (int64_t base = 0; base < n; base += kScanSmallTile) {
+#pragma unroll
+    for (int k = 0; k < kScanSmallItems; ++k) {
+      const int64_t idx = base + k * 1024 + tid;
+      tile[k * 1024 + tid] = idx < n ? in[idx] : 0;
+    }
+    __syncthreads();
+    int local = 0;
+#pragma unroll
+    for (int k = 0; k < kScanSmallItems; ++k) local += tile[tid * kScanSmallItems + k];
+    int total;
+    const int incl = block_incl_scan<1024>(local, smem, total);
+    int prefix = carry + incl - local;
+#pragma unroll
+    for (int k = 0; k < kScanSmallItems; ++k) {
+      const int v = tile[tid * kScanSmallItems + k];
+      tile[tid * kScanSmallItems + k] = prefix;
+      prefix += v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScanSmallItems; ++k) {
+      const int64_t idx = base + k * 1024 + tid;
+      if (idx < n) out[idx] = tile[k * 1024 + tid];
+    }
+    carry += total;
+    __syncthreads();
+  }
+  if (tid == 0) out[n] = carry;
+}
+
 __global__ __launch_bounds__(1024) void k_scan_partials(int32_t* part, int nb, int32_t* total_out) {
   __shared__ int smem[1024 / kWave];
   int carry = 0;
@@ -497,55 +539,55 @@ __global__ __launch_bounds__(kThreads) void k_query_block_cost(const int32_t* kn
 }
 
 
-// Query blocks in decreasing cost (ties by index), one workgroup bitonic sort in
-// LDS: heavy blocks start first, so the light ones fill in behind them and the
-// pass does not end on a few late heavy blocks.  With xcd set, the blocks are
-// first split into 8 contiguous ranges of the Morton order, one per XCD (the
-// dispatch gives workgroup b to XCD b % 8), and ordered by cost inside each
-// range: neighbouring queries -- which read the same tiles -- share one L2.
+// Query blocks in decreasing cost, one workgroup: a counting sort on the cost
+// quantised to 1/8 of an octave (heavy blocks start first, so the light ones
+// fill in behind them and the pass does not end on a few late heavy blocks; the
+// order inside a bucket is free -- the results do not depend on the launch
+// order).  With xcd set, the blocks are first split into 8 contiguous ranges of
+// the Morton order, one per XCD (the dispatch gives workgroup b to XCD b % 8),
+// and ordered by cost inside each range: neighbouring queries -- which read the
+// same tiles -- share one L2.
+__device__ __forceinline__ int cost_bucket(int32_t cost) {
+  const uint32_t c = min(static_cast<uint32_t>(cost > 0 ? cost : 0), 0xffffffu);
+  if (c == 0) return 255;
+  const int l = 31 - __builtin_clz(c);                              // floor(log2 c) <= 23
+  const int frac = static_cast<int>((c << (31 - l)) >> 28) & 7;   // next three bits
+  return 255 - (l * 8 + frac);                                      // 64..255, heaviest first
+}
+
 __global__ __launch_bounds__(1024) void k_query_block_order(const int32_t* cost, int32_t nb, int xcd,
                                                             int32_t* order) {
-  __shared__ uint64_t key[kOrderMax];
-  int m = 1;
-  while (m < nb) m <<= 1;
-  const int q = nb >> 3, r = nb & 7;
-  for (int i = threadIdx.x; i < m; i += 1024) {
-    // larger cost first: sort ascending on (range, ~cost, index)
-    const uint32_t c = i < nb ? min(static_cast<uint32_t>(cost[i]), 0xffffffu) : 0u;
-    // XCD of a block: its chunk of the Morton order (32 chunks) modulo 8, so each
-    // XCD holds four separate eighths-of-a-quarter of the domain (their costs even
-    // out); a group's size may miss its XCD's share by a chunk, which only moves
-    // those blocks to a neighbouring XCD
-    uint64_t range = 0;
-    const int chunk = (nb + 31) / 32;
-    if (xcd) range = (i / chunk) & 7;
-    key[i] = i < nb ? ((range << 56) | (static_cast<uint64_t>(0xffffffu - c) << 32) | static_cast<uint32_t>(i))
-                    : ~uint64_t{0};
-  }
+  __shared__ int32_t cnt[8 * 256];
+  __shared__ int32_t sorted[kOrderMax];
+  __shared__ int scan_part[1024 / kWave];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < 8 * 256; i += 1024) cnt[i] = 0;
   __syncthreads();
-  for (int k = 2; k <= m; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      // one compare-exchange per pair (i, i + j), i with bit j clear
-      for (int pr = threadIdx.x; pr < (m >> 1); pr += 1024) {
-        const int i = ((pr & ~(j - 1)) << 1) | (pr & (j - 1));
-        const int l = i + j;
-        const uint64_t a = key[i], b = key[l];
-        const bool up = (i & k) == 0;
-        if ((a > b) == up) {
-          key[i] = b;
-          key[l] = a;
-        }
-      }
-      __syncthreads();
-    }
-  }
-  for (int b = threadIdx.x; b < nb; b += 1024) {
+  // XCD of a block: its chunk of the Morton order (32 chunks) modulo 8, so each
+  // XCD holds four separate eighths-of-a-quarter of the domain (their costs even
+  // out); a group's size may miss its XCD's share by a chunk, which only moves
+  // those blocks to a neighbouring XCD
+  const int chunk = (nb + 31) / 32;
+  auto bucket_of = [&](int i) { return (xcd ? ((i / chunk) & 7) * 256 : 0) + cost_bucket(cost[i]); };
+  for (int i = tid; i < nb; i += 1024) atomicAdd(&cnt[bucket_of(i)], 1);
+  __syncthreads();
+  // exclusive scan of the 2048 bucket counts, two per thread
+  const int c0 = cnt[2 * tid], c1 = cnt[2 * tid + 1];
+  int total;
+  const int incl = block_incl_scan<1024>(c0 + c1, scan_part, total);
+  cnt[2 * tid] = incl - c0 - c1;
+  cnt[2 * tid + 1] = incl - c1;
+  __syncthreads();
+  for (int i = tid; i < nb; i += 1024) sorted[atomicAdd(&cnt[bucket_of(i)], 1)] = i;
+  __syncthreads();
+  const int q = nb >> 3, r = nb & 7;
+  for (int b = tid; b < nb; b += 1024) {
     int src = b;
     if (xcd) {  // workgroup b runs on XCD b % 8: the (b / 8)-th block of that XCD's range
       const int x = b & 7, k = b >> 3;
       src = (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
     }
-    order[b] = static_cast<int32_t>(key[src] & 0xffffffffu);
+    order[b] = sorted[src];
   }
 }
 
@@ -618,29 +660,30 @@ __global__ __launch_bounds__(kThreads) void k_labels(int32_t* root, int32_t n, i
                                                       const int32_t* rank_base, int32_t* cluster,
                                                       int32_t* nh_out, int32_t* flags, int round) {
   if (round > 0 && flags[round - 1] == 0) return;  // every thread finished before
-  const int i = blockIdx.x * kThreads + threadIdx.x;
-  if (i >= n) return;
-  const int e = pev[i];
-  if (round == 0) {
-    const int32_t nhv = nh_out[i];
-    if (nhv >= 0) nh_out[i] = nhv - ev[e].begin;
-  }
-  int32_t r = root[i];
-  bool at_root = false;
-  for (int h = 0; h < hops; ++h) {
-    const int32_t rr = root[r];
-    if (rr == r) {
-      at_root = true;
-      break;
+  // later rounds run on a smaller grid, each thread striding over the points
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    const int e = pev[i];
+    if (round == 0) {
+      const int32_t nhv = nh_out[i];
+      if (nhv >= 0) nh_out[i] = nhv - ev[e].begin;
     }
-    r = rr;
-  }
-  root[i] = r;
-  if (at_root) {
-    const int32_t local = rank[r] - rank_base[e];
-    cluster[i] = (is_seed[r] && local < cap_used[e]) ? local : -1;
-  } else {
-    flags[round] = 1;
+    int32_t r = root[i];
+    bool at_root = false;
+    for (int h = 0; h < hops; ++h) {
+      const int32_t rr = root[r];
+      if (rr == r) {
+        at_root = true;
+        break;
+      }
+      r = rr;
+    }
+    root[i] = r;
+    if (at_root) {
+      const int32_t local = rank[r] - rank_base[e];
+      cluster[i] = (is_seed[r] && local < cap_used[e]) ? local : -1;
+    } else {
+      flags[round] = 1;
+    }
   }
 }
 
@@ -891,6 +934,11 @@ clue_status scan_exclusive(const int32_t* in, int32_t* out, int64_t n, int32_t* 
   if (nb == 0) {
     k_fill_i32<<<1, 64, 0, st>>>(out, 0, 1);
     CLUE_LAUNCH_CHECK("fill");
+    return CLUE_OK;
+  }
+  if (n <= kScanSmall) {
+    k_scan_small<<<1, 1024, 0, st>>>(in, n, out);
+    CLUE_LAUNCH_CHECK("scan_small");
     return CLUE_OK;
   }
   k_scan_reduce<<<nb, kThreads, 0, st>>>(in, n, part);
@@ -1440,7 +1488,8 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
   int rounds = 1;
   for (int64_t span = hops; span < n; span *= hops) ++rounds;
   for (int r = 0; r < rounds; ++r) {
-    k_labels<<<div_up(n, kThreads), kThreads, 0, st>>>(
+    const int64_t lgrid = r == 0 ? div_up(n, kThreads) : std::min<int64_t>(div_up(n, kThreads), 1024);
+    k_labels<<<lgrid, kThreads, 0, st>>>(
         p->root.as<int32_t>(), n, hops, seed_buf, p->flag.as<int32_t>(), p->pev.as<int32_t>(),
         p->events.as<EventDesc>(), p->cap_used.as<int32_t>(), p->rank_base.as<int32_t>(), cluster_out,
         nh_buf, counters + 8, r);
