@@ -339,6 +339,14 @@ __device__ __forceinline__ void sqrt_class(float d, float& lo, float& hi) {
   lo = l;
 }
 
+// Every squared distance x with sqrtf(x) <= sqrtf(s) has x <= not_farther_than(s):
+// a float's sqrtf class [lo, hi] spans less than 2^-21 of it (the midpoints of
+// sqrtf(s) and its neighbours, squared); the margin of 2^-20 (and of the least
+// normal float, for tiny s) holds it with room for the rounding of the product.
+// The nearest-higher searches bound their candidates' squared distances with it
+// and take a square root only for those within.
+__device__ __forceinline__ float not_farther_than(float s) { return s * (1.f + 0x1p-20f) + 0x1p-126f; }
+
 // Order key of the nearest-higher relation (ClusteringKernels.hpp:132-134): point j
 // is higher than point i when rho_j > rho_i, or rho_j == rho_i > 0 and j > i.  With
 //   rank_key(rho, id) = ord(rho) << 32 | id,

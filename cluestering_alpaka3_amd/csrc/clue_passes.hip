@@ -504,8 +504,9 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher(const PassArgs a) {
   cptr<uint64_t> HK = as_const(a.hk);
   float delta = FLT_MAX;
   int32_t nh = -1;
-  // updates need s2 <= bound = min(thr(dm), max{s : sqrtf(s) < delta}): the float
-  // below the smallest squared distance whose sqrtf is delta
+  // Euclidean: updates need s2 <= bound = min(thr(dm), the best's squared
+  // distance) -- sqrtf is monotone -- and then a strictly smaller sqrtf, whose
+  // square root is taken only for such candidates
   float bound = a.thr;
   // One candidate (ClusteringKernels.hpp:129-144): j is "higher" if rho_j > rho_i,
   // or equal, positive and with a larger id (one comparison of rank keys); the
@@ -515,12 +516,12 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher(const PassArgs a) {
     if constexpr (METRIC == MEUCLID) {
       const float s2 = squared_distance<D>(q, c);
       if (higher && s2 <= bound) {
-        delta = __builtin_sqrtf(s2);
-        nh = static_cast<int32_t>(key_j & 0xffffffffu);
-        float lo2, hi2;
-        sqrt_class(delta, lo2, hi2);
-        const float lim = lo2 > 0.f ? __uint_as_float(__float_as_uint(lo2) - 1u) : -1.f;
-        bound = lim < a.thr ? lim : a.thr;
+        const float dc = __builtin_sqrtf(s2);
+        if (dc < delta) {
+          delta = dc;
+          nh = static_cast<int32_t>(key_j & 0xffffffffu);
+          bound = s2;
+        }
       }
     } else {
       const float dist = metric_distance<D, METRIC>(q, c, a.mp);
@@ -619,8 +620,9 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_wave(const PassArgs
           for (int d = 0; d < D; ++d) c[d] = a.xs[d][j];
           float dj;
           if constexpr (METRIC == MEUCLID) {
+            // squared distances: sqrtf is monotone, the square root is taken once per batch
             const float s2 = squared_distance<D>(q, c);
-            dj = s2 <= a.thr ? __builtin_sqrtf(s2) : __builtin_inff();
+            dj = s2 <= a.thr ? s2 : __builtin_inff();
           } else {
             dj = metric_distance<D, METRIC>(q, c, a.mp);
             dj = dj <= a.radius ? dj : __builtin_inff();
@@ -629,7 +631,20 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_wave(const PassArgs
         }
       }
       const float dmin = wave_min_f32(dist);
-      if (dmin < delta) {  // uniform
+      if constexpr (METRIC == MEUCLID) {
+        if (dmin <= a.thr) {  // uniform; some higher candidate within dm
+          const float dd = __builtin_sqrtf(dmin);
+          if (dd < delta) {
+            // the first lane whose sqrtf equals the least one (squared distances
+            // within the class of dmin, all at least dmin)
+            float lo2, hi2;
+            sqrt_class(dd, lo2, hi2);
+            const int l = __builtin_ctzll(__ballot(dist <= hi2));
+            delta = dd;
+            nh = static_cast<int32_t>(readlane_u64(key_j, l) & 0xffffffffu);
+          }
+        }
+      } else if (dmin < delta) {  // uniform
         const int l = __builtin_ctzll(__ballot(dist == dmin));
         delta = dmin;
         nh = static_cast<int32_t>(readlane_u64(key_j, l) & 0xffffffffu);
@@ -726,16 +741,15 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
   // no point higher than that lane's; at most the wave's least threshold, none
   // higher than any of its lanes'
   const uint64_t wthr = uniform(wave_min_u64(valid ? thr_i : ~uint64_t{0}));
-  float delta = FLT_MAX;
-  int32_t nh = -1;
-  int64_t best = INT64_MAX;
-  // The best candidate so far is at distance delta; the squared distances with
-  // sqrtf equal to delta are [lo2, hi2].  A candidate replaces it when it is
-  // nearer (s2 < lo2) or as near and earlier in the visit order (key < best), so
-  // candidates need s2 <= bound = min(search radius, hi2).
+  int64_t best = INT64_MAX;  // visit key (tile, id) of the best candidate
+  // The best candidate so far is at distance delta, squared distance s2b.  A
+  // candidate replaces it when its sqrtf is smaller, or equal and it is earlier
+  // in the visit order (key < best); it cannot when s2 > not_farther_than(s2b),
+  // so candidates need s2 <= bound = min(search radius, not_farther_than(s2b)),
+  // and their square roots are taken only then.
   const float cap = a.thr1 < a.thr ? a.thr1 : a.thr;
   float bound = cap;
-  float lo2 = __builtin_inff();
+  float delta = FLT_MAX;
   // the wave's query box, for the block prefilter
   float qlo[D], qhi[D];
 #pragma unroll
@@ -746,14 +760,15 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
 
   auto cand = [&](int t, const float* c, uint64_t key_j) {
     const float s2 = squared_distance<D>(q, c);
-    const int64_t key = (static_cast<int64_t>(t) << 32) | static_cast<uint32_t>(key_j);
-    if (key_j > thr_i && s2 <= bound && (s2 < lo2 || key < best)) {
-      delta = __builtin_sqrtf(s2);
-      nh = static_cast<int32_t>(key_j & 0xffffffffu);
-      best = key;
-      float hi2;
-      sqrt_class(delta, lo2, hi2);
-      bound = hi2 < cap ? hi2 : cap;
+    if (key_j > thr_i && s2 <= bound) {
+      const int64_t key = (static_cast<int64_t>(t) << 32) | static_cast<uint32_t>(key_j);
+      const float dc = __builtin_sqrtf(s2);
+      if (dc < delta || (dc == delta && key < best)) {
+        delta = dc;
+        best = key;
+        const float up = not_farther_than(s2);
+        bound = up < cap ? up : cap;
+      }
     }
   };
   // candidates [beg, end) of tile t, wave-uniform: eight at a time, all loads
@@ -912,6 +927,7 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
 
   // unresolved positions are compacted in Morton order (flag -> scan -> scatter)
   // so that the full search runs on spatially coherent waves
+  const int32_t nh = best == INT64_MAX ? -1 : static_cast<int32_t>(best & 0xffffffff);
   if (valid) a.pending_flag[pos] = nh == -1 ? 1 : 0;
   if (valid && nh != -1) {
     a.nh_out[id_i] = nh;
@@ -962,12 +978,12 @@ __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
     const EvGrid<D> g = load_event<D>(a.ev, e, a.wrap);
     int lo[D], hi[D];
     search_box<D>(q, a.radius, g, lo, hi);
-    float delta = FLT_MAX;
     int64_t best = INT64_MAX;
-    // candidates need s2 <= bound = min(thr(dm), hi2), and replace the lane's best
-    // when s2 < lo2 or, as near, key < best ([lo2, hi2]: the sqrtf class of delta)
+    // candidates need s2 <= bound = min(thr(dm), not_farther_than(s2b)) (s2b: the
+    // lane's best squared distance, delta its sqrtf), and replace the lane's best
+    // when their sqrtf is smaller, or equal with key < best
     float bound = a.thr;
-    float lo2 = __builtin_inff();
+    float delta = FLT_MAX;
     float group_bound = a.thr;
     // Levels of growing radius order the visits nearest first, so that the
     // bound shrinks early.  Each block is visited once, at the first level whose
@@ -1018,13 +1034,15 @@ __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
     }
     auto cand = [&](int t, const float* c, uint64_t key_j) {
       const float s2 = squared_distance<D>(q, c);
-      const int64_t key = (static_cast<int64_t>(t) << 32) | static_cast<uint32_t>(key_j);
-      if (key_j > thr_i && s2 <= bound && (s2 < lo2 || key < best)) {
-        delta = __builtin_sqrtf(s2);
-        best = key;
-        float hi2;
-        sqrt_class(delta, lo2, hi2);
-        bound = hi2 < a.thr ? hi2 : a.thr;
+      if (key_j > thr_i && s2 <= bound) {
+        const int64_t key = (static_cast<int64_t>(t) << 32) | static_cast<uint32_t>(key_j);
+        const float dc = __builtin_sqrtf(s2);
+        if (dc < delta || (dc == delta && key < best)) {
+          delta = dc;
+          best = key;
+          const float up = not_farther_than(s2);
+          bound = up < a.thr ? up : a.thr;
+        }
       }
     };
     for (int lv = 0; lv < a.nlvl; ++lv) {

@@ -10,7 +10,7 @@ import pytest
 import cluestering_alpaka3_amd as clue
 import oracle
 from cluestering_alpaka3_amd import datasets
-from parity_util import assert_same, run_both
+from parity_util import assert_same, run_both, run_engine
 
 pytestmark = pytest.mark.gpu
 
@@ -298,6 +298,58 @@ def test_small_calls_fused_and_not(ref_data, fused):
     ps = datasets.blobs(20000, 2, 60, 0.0, 200.0, 1.0, 0.0, ("const", 1.0), 99)
     got, ref, *rest = run_both(ps.coords, ps.weights, 0.6, 1.0, 5.0, seed_dc=0.6, tuning=t)
     assert ref.stats["n_dropped_seeds"] > 0
+    assert_same(got, ref, *rest)
+
+
+def ring_ties(ndim, centres, per_ring, seed):
+    """Light centres, each inside a shell of heavier points at radius 2.5-3.5
+    rounded to float: their distances to the centre fall within a few ulps, so a
+    centre's nearest-higher search meets equal sqrtf distances (ties broken by
+    the visit order) and distances one ulp apart whose squares differ by ulps."""
+    rng = np.random.default_rng(seed)
+    side = int(np.ceil(centres ** (1.0 / ndim)))
+    grid = np.stack(np.meshgrid(*[np.arange(side)] * ndim, indexing="ij")).reshape(ndim, -1)[:, :centres]
+    cen = (grid * 9.0 - 9.0 * (side - 1) / 2).astype(np.float64)
+    cen += rng.uniform(-0.3, 0.3, cen.shape)
+    pts, w = [cen.astype(np.float32)], [np.full(cen.shape[1], 0.25, np.float32)]
+    for k in range(cen.shape[1]):
+        u = rng.normal(size=(ndim, per_ring))
+        u /= np.linalg.norm(u, axis=0)
+        r = rng.uniform(2.5, 3.5)
+        pts.append((cen[:, k:k + 1] + r * u).astype(np.float32))
+        w.append(rng.choice(np.float32([1.0, 1.5, 2.0]), per_ring))
+    order = rng.permutation(sum(p.shape[1] for p in pts))
+    return np.concatenate(pts, axis=1)[:, order], np.concatenate(w)[order]
+
+
+NH_PATHS = [dict(small_call_points=0, tile_call_points=0),                       # block search, one phase
+            dict(small_call_points=0, tile_call_points=0, nh_first_radius=0.3),  # centres left to the second
+            dict(small_call_points=0, tile_call_points=0, nh_first_radius=0.3, pending_group=64),
+            dict(small_call_points=0, tile_call_points=1 << 30),                 # tile walk, a lane per query
+            dict(small_call_points=1 << 30, tile_call_points=1 << 30),           # tile walk, a wave per query
+            dict(small_call_points=1 << 30, tile_call_points=0)]
+
+
+_RING_CACHE = {}
+
+
+def ring_case(ndim):
+    """The ring set in ndim dimensions and the oracle's result (one per process)."""
+    if ndim not in _RING_CACHE:
+        c, w = ring_ties(ndim, 64, 1000 if ndim == 2 else 1500, 40 + ndim)
+        args = (0.05, 0.2, 4.0, None, 32, ("flat", 0.5), "euclidean", None, None)
+        _RING_CACHE[ndim] = (c, w, oracle.make_clusters(c, w, oracle.make_params(*args, seed_mode=0)))
+    return _RING_CACHE[ndim]
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+@pytest.mark.parametrize("tuning", NH_PATHS)
+def test_nearest_higher_sqrt_ties(ndim, tuning):
+    c, w, ref = ring_case(ndim)
+    got, *rest = run_engine(c, w, 0.05, 0.2, 4.0, ppb=32, tuning=tuning)
+    # the centres' nearest-highers are shell points at a distance shared by others
+    d = ref.delta[w == np.float32(0.25)]
+    assert np.all((d >= 2.5 * 0.99) & (d <= 3.5 * 1.01))
     assert_same(got, ref, *rest)
 
 
