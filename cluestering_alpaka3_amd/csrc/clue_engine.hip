@@ -1934,7 +1934,7 @@ clue_status clue_plan_set_tuning(clue_plan_t plan, const clue_tuning* tuning) {
       tuning->density_order < -1 || tuning->density_order > 1 || tuning->small_call_points < 0 ||
       tuning->wave_dims < 0 || tuning->wave_dims > CLUE_MAX_DIM || tuning->tile_call_points < 0 ||
       tuning->density_filter < 0 || tuning->density_filter > 1 || tuning->fused_event_points < 0 ||
-      tuning->fused_event_points > kFusedMaxPoints)
+      tuning->fused_event_points > kFusedMaxPoints || tuning->split_call_points < 0)
     return fail(CLUE_ERR_INVALID_ARGUMENT, "invalid tuning values");
   plan->tuning = *tuning;
   return CLUE_OK;
@@ -2233,6 +2233,7 @@ clue_status clue_autotune_batched(clue_plan_t plan, void* stream, clue_params* p
   try_field(set_group, {8.f, 16.f, 32.f, 64.f});
   try_field([](clue_tuning& t, float v) { t.density_order = static_cast<int32_t>(v); }, {0.f, 1.f});
   try_field([](clue_tuning& t, float v) { t.density_filter = static_cast<int32_t>(v); }, {0.f, 1.f});
+  try_field([](clue_tuning& t, float v) { t.split_call_points = static_cast<int32_t>(v); }, {0.f, 1073741824.f});
   try_field(set_radius, {cur.nh_first_radius * 0.8f, cur.nh_first_radius * 1.25f});
   (void)hipStreamSynchronize(st);
   (void)hipFree(labels);

@@ -491,7 +491,7 @@ struct clue_plan_s {
   DevBuf pending, pending_flag, qcost, qorder;
   // cluster -> points map (post-processing): sort keys and values
   DevBuf mkey, mkey2, mval, mval2;
-  clue_tuning tuning{0.f, -1, 7, 0, 0, -1, 65536, 3, 32768, 1, 12288};
+  clue_tuning tuning{0.f, -1, 7, 0, 0, -1, 65536, 3, 32768, 1, 12288, 262144};
   int64_t ldb = 0;  // leading dimension of the block boxes
   int64_t ld = 0;   // leading dimension of xs / xn
   // pinned staging ring of the event and chunk tables (a slot is reused once its

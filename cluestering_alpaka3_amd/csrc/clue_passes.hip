@@ -111,7 +111,13 @@ __global__ __launch_bounds__(kThreads) void k_box_class(const PassArgs a, float 
 // so the lanes of a wave share most of their search box); otherwise in tile order.
 // UNIT: Euclidean metric, exp argument in [-14, 0] and amplitude 1 (exp_unit);
 // 2: also a Gaussian of mean 0 and power-of-two (2 sigma) sigma.
-template <int D, int METRIC, int KERNEL, bool UNION, bool QNH, int UNIT = 0, bool FILT = false>
+// SPLIT (filtered walk only): the workgroup's four waves share one wave's 64
+// queries.  Each builds the same ring of survivors; every batch of 32 is
+// evaluated a quarter per wave into LDS, and wave 0 adds the 32 terms in ring
+// order -- the same sums, with the chain of a query's terms cut to the adds.
+// For calls whose waves do not fill the GPU, where a pass lasts as long as its
+// heaviest wave.
+template <int D, int METRIC, int KERNEL, bool UNION, bool QNH, int UNIT = 0, bool FILT = false, bool SPLIT = false>
 __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
   // the exponential's 2^(j/64) table, read through LDS by the kernel evaluations
   __shared__ double s_exp2[KERNEL == KFLAT ? 1 : 64];
@@ -121,18 +127,23 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
   }
   // the filtered union walk's per-wave ring of candidates (coordinates, value, slot)
   constexpr bool kFilter = FILT && UNION && METRIC == MEUCLID;
-  constexpr int kRing = 128;  // entries per wave; a batch never wraps (kRing % kFB == 0)
-  constexpr int kFB = 8;      // candidates per evaluation batch
+  static_assert(!SPLIT || (kFilter && QNH), "the split walk is a variant of the filtered walk");
+  constexpr int kRing = 128;         // entries per wave; a batch never wraps (kRing % kFB == 0)
+  constexpr int kFB = SPLIT ? 32 : 8;  // candidates per evaluation batch
+  constexpr int kWaves = kThreads / kWave;
+  constexpr int kShare = kFB / kWaves;  // split: candidates of a batch per wave
+  // split: the terms of a batch, [batch parity][candidate][lane]
+  __shared__ float s_terms[SPLIT ? 2 : 1][SPLIT ? kFB : 1][SPLIT ? kWave : 1];
   // kernel value 1 at distance 0, the self term's value: the self test is not needed
   constexpr bool kSelfFree = (UNIT != 0 && KERNEL == KEXP) || (UNIT == 2 && KERNEL == KGAUSS);
   __shared__ float4 s_ring[kFilter ? kThreads / kWave : 1][kFilter ? kRing : 1];
   __shared__ int32_t s_ringj[kFilter && D == 3 ? kThreads / kWave : 1][kFilter && D == 3 ? kRing : 1];
   const int qb = a.qorder ? a.qorder[blockIdx.x] : logical_block(a.xcd & 1);
-  const int pos = qb * kThreads + threadIdx.x;
-  const bool valid = pos < a.n;
-  const int pq = valid ? pos : a.n - 1;
   const int lane = __lane_id();
   const int wv = threadIdx.x / kWave;
+  const int pos = SPLIT ? qb * kWave + lane : qb * kThreads + static_cast<int>(threadIdx.x);
+  const bool valid = pos < a.n;
+  const int pq = valid ? pos : a.n - 1;
   float q[D];
   float w_self;  // the point's own term is 1 * w_i (kernel value 1 for j == i)
   int s;         // the query's slot in the tile-ordered candidate layout
@@ -164,15 +175,18 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
   constexpr bool kUsesHW = (METRIC == MEUCLID) && (KERNEL == KFLAT);
 
   // One candidate: rho_i += ((float)(d <= dc) * k) * w_j in the visit order
-  // (ClusteringKernels.hpp:38-44); out-of-radius terms are +0 and skipped.
-  auto eval = [&](auto self_c, int j, const float* c, float wj, float hwj) {
+  // (ClusteringKernels.hpp:38-44); out-of-radius terms are +0 and skipped (the
+  // density starts at +0 and an add never makes it -0, so +0 terms change
+  // nothing).  sink receives the in-radius term.
+  auto add = [&](float t) { rho += t; };
+  auto eval = [&](auto self_c, int j, const float* c, float wj, float hwj, auto&& sink) {
     constexpr bool SELF = decltype(self_c)::value;  // may this tile hold the lane's own point?
     if constexpr (METRIC == MEUCLID) {
       const float s2 = squared_distance<D>(q, c);
       if constexpr (KERNEL == KFLAT) {
         float t = hwj;
         if constexpr (SELF) t = (j == s) ? w_self : hwj;
-        if (s2 <= a.thr) rho += t;
+        if (s2 <= a.thr) sink(t);
       } else {
         if (s2 <= a.thr) {
           float k = 1.f;
@@ -184,14 +198,14 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
               k = kernel_value<KERNEL>(__builtin_sqrtf(s2), a.kp[0], a.kp[1], a.kp[2], a.kden_rcp);
             }
           }
-          rho += k * wj;
+          sink(k * wj);
         }
       }
     } else {
       const float dist = metric_distance<D, METRIC>(q, c, a.mp);
       if (dist <= a.radius) {
         const float k = (j == s) ? 1.f : kernel_value<KERNEL>(dist, a.kp[0], a.kp[1], a.kp[2], a.kden_rcp);
-        rho += k * wj;
+        sink(k * wj);
       }
     }
   };
@@ -213,7 +227,7 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
         }
         if (need) {
 #pragma unroll
-          for (int u = 0; u < U; ++u) eval(self_c, j + u, c[u], wv4[u], hv[u]);
+          for (int u = 0; u < U; ++u) eval(self_c, j + u, c[u], wv4[u], hv[u], add);
         }
       }
     }
@@ -224,7 +238,7 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
       float wj = 0.f, hwj = 0.f;
       if constexpr (kUsesHW) hwj = HW[j];
       else wj = W[j];
-      if (need) eval(self_c, j, c, wj, hwj);
+      if (need) eval(self_c, j, c, wj, hwj, add);
     }
   };
   // out-of-radius candidates with an infinite or NaN weight add 0 * w = NaN
@@ -278,31 +292,50 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
         ring[at] = r4;
         if constexpr (D == 3) ringj[at] = j;
       };
+      int tb = 0;  // split: the parity of the batch (its term buffer)
       auto drain = [&](bool take) {
         __builtin_amdgcn_wave_barrier();
-        float4 r4[kFB];
-        int jj[kFB];
+        constexpr int kOwn = SPLIT ? kShare : kFB;  // candidates this wave evaluates
+        const int first = head + (SPLIT ? wv * kShare : 0);
+        float4 r4[kOwn];
+        int jj[kOwn];
 #pragma unroll
-        for (int u = 0; u < kFB; ++u) {
-          r4[u] = ring[head + u];
-          if constexpr (D == 3) jj[u] = ringj[head + u];
+        for (int u = 0; u < kOwn; ++u) {
+          r4[u] = ring[first + u];
+          if constexpr (D == 3) jj[u] = ringj[first + u];
         }
-        if (take) {
-#pragma unroll
-          for (int u = 0; u < kFB; ++u) {
-            float c[D];
-            float v;
-            int j;
-            if constexpr (D == 1) {
-              c[0] = r4[u].x; v = r4[u].y; j = __float_as_int(r4[u].z);
-            } else if constexpr (D == 2) {
-              c[0] = r4[u].x; c[1] = r4[u].y; v = r4[u].z; j = __float_as_int(r4[u].w);
-            } else {
-              c[0] = r4[u].x; c[1] = r4[u].y; c[2] = r4[u].z; v = r4[u].w; j = jj[u];
-            }
-            if constexpr (kSelfFree) eval(std::false_type{}, j, c, v, v);
-            else eval(std::true_type{}, j, c, v, v);
+        auto cand = [&](int u, auto&& sink) {
+          float c[D];
+          float v;
+          int j;
+          if constexpr (D == 1) {
+            c[0] = r4[u].x; v = r4[u].y; j = __float_as_int(r4[u].z);
+          } else if constexpr (D == 2) {
+            c[0] = r4[u].x; c[1] = r4[u].y; v = r4[u].z; j = __float_as_int(r4[u].w);
+          } else {
+            c[0] = r4[u].x; c[1] = r4[u].y; c[2] = r4[u].z; v = r4[u].w; j = jj[u];
           }
+          if constexpr (kSelfFree) eval(std::false_type{}, j, c, v, v, sink);
+          else eval(std::true_type{}, j, c, v, v, sink);
+        };
+        if constexpr (SPLIT) {
+          // every wave sees the same ring, batches and flags (same queries, same
+          // candidates), so all reach each barrier
+#pragma unroll
+          for (int u = 0; u < kOwn; ++u) {
+            float t = 0.f;
+            if (take) cand(u, [&](float x) { t = x; });
+            s_terms[tb][wv * kShare + u][lane] = t;
+          }
+          __syncthreads();
+          if (wv == 0) {
+#pragma unroll
+            for (int u = 0; u < kFB; ++u) rho += s_terms[tb][u][lane];
+          }
+          tb ^= 1;
+        } else if (take) {
+#pragma unroll
+          for (int u = 0; u < kFB; ++u) cand(u, add);
         }
         __builtin_amdgcn_wave_barrier();
         head = (head + kFB) & (kRing - 1);
@@ -356,7 +389,7 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
     });
   });
   if (poisoned) rho = __builtin_nanf("");
-  if (valid) {
+  if (valid && (!SPLIT || wv == 0)) {
     a.rho_s_out[s] = rho;
     if constexpr (QNH) {
       const int32_t id = a.pn[pos];
@@ -1550,8 +1583,11 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
     CLUE_LAUNCH_CHECK("nearest_higher");
     return CLUE_OK;
   };
-  // small calls: one wave per query (k_density_wave), the same sums in the same order
-  if (wave_mode) {
+  // small calls: one wave per query (k_density_wave), the same sums in the same order,
+  // unless the call runs the split density
+  const bool split_first = U && METRIC == MEUCLID && blocks && a.dfilter && p->tuning.split_call_points > 0 &&
+                           a.n <= p->tuning.split_call_points;
+  if (wave_mode && !split_first) {
     const int wgrid = static_cast<int>(div_up(a.n, kThreads / kWave));
     if constexpr (METRIC == MEUCLID) {
       if (blocks) {
@@ -1589,7 +1625,11 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
   // heaviest-first pays in 2D (a shorter tail); in 3D the lost spatial locality
   // of concurrent blocks costs more than it saves (measured), hence the default
   const bool order = p->tuning.density_order > 0 || (p->tuning.density_order < 0 && D <= 2);
-  if (blocks && order && grid <= kOrderMax) {
+  // four waves per 64 queries (the filtered walk's split variant) for calls of at
+  // most split_call_points points; its query blocks are 64 queries, unordered
+  const bool split = U && METRIC == MEUCLID && blocks && a.dfilter && p->tuning.split_call_points > 0 &&
+                     a.n <= p->tuning.split_call_points;
+  if (blocks && order && grid <= kOrderMax && !split) {
     rc = launch_density_order(p, a, grid, st);
     if (rc != CLUE_OK) return rc;
     a.qorder = p->qorder.as<int32_t>();
@@ -1598,25 +1638,29 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
     int unit = 0;
     if constexpr (METRIC == MEUCLID) unit = a.exp_unit;
     constexpr int kU1 = METRIC == MEUCLID ? 1 : 0, kU2 = METRIC == MEUCLID ? 2 : 0;
-    auto launch = [&](auto filt_c) -> bool {
+    auto launch = [&](auto filt_c, auto split_c) -> bool {
       constexpr bool F = decltype(filt_c)::value;
+      constexpr bool S = decltype(split_c)::value;
+      const int g = S ? static_cast<int>(div_up(a.n, kWave)) : grid;
       switch (kernel_type) {
-        case KFLAT: k_density<D, METRIC, KFLAT, U, true, 0, F><<<grid, kThreads, 0, st>>>(a); return true;
+        case KFLAT: k_density<D, METRIC, KFLAT, U, true, 0, F, S><<<g, kThreads, 0, st>>>(a); return true;
         case KGAUSS:
-          if (unit == 2) k_density<D, METRIC, KGAUSS, U, true, kU2, F><<<grid, kThreads, 0, st>>>(a);
-          else if (unit) k_density<D, METRIC, KGAUSS, U, true, kU1, F><<<grid, kThreads, 0, st>>>(a);
-          else k_density<D, METRIC, KGAUSS, U, true, 0, F><<<grid, kThreads, 0, st>>>(a);
+          if (unit == 2) k_density<D, METRIC, KGAUSS, U, true, kU2, F, S><<<g, kThreads, 0, st>>>(a);
+          else if (unit) k_density<D, METRIC, KGAUSS, U, true, kU1, F, S><<<g, kThreads, 0, st>>>(a);
+          else k_density<D, METRIC, KGAUSS, U, true, 0, F, S><<<g, kThreads, 0, st>>>(a);
           return true;
         case KEXP:
-          if (unit) k_density<D, METRIC, KEXP, U, true, kU1, F><<<grid, kThreads, 0, st>>>(a);
-          else k_density<D, METRIC, KEXP, U, true, 0, F><<<grid, kThreads, 0, st>>>(a);
+          if (unit) k_density<D, METRIC, KEXP, U, true, kU1, F, S><<<g, kThreads, 0, st>>>(a);
+          else k_density<D, METRIC, KEXP, U, true, 0, F, S><<<g, kThreads, 0, st>>>(a);
           return true;
       }
       return false;
     };
+    constexpr std::false_type no{};
+    constexpr std::true_type yes{};
     bool ok;
-    if constexpr (U && METRIC == MEUCLID) ok = a.dfilter ? launch(std::true_type{}) : launch(std::false_type{});
-    else ok = launch(std::false_type{});
+    if constexpr (U && METRIC == MEUCLID) ok = !a.dfilter ? launch(no, no) : split ? launch(yes, yes) : launch(yes, no);
+    else ok = launch(no, no);
     if (!ok) return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown convolutional kernel type");
   } else {
     switch (kernel_type) {

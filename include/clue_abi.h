@@ -132,6 +132,13 @@ typedef struct {
  *                      (and 2,048 tiles) build each event's layout, and later
  *                      its seeds and labels, in one workgroup per event (two
  *                      launches in place of about twenty); at most 32,768;
+ *                      0: never
+ *  split_call_points   calls of at most this many points that run the
+ *                      filtered density walk (1-3 dimensions, Euclidean, with
+ *                      the Morton layout) share each wave's 64 queries among
+ *                      four waves, which shortens the longest query's chain
+ *                      when the call's waves do not fill the GPU; it takes
+ *                      precedence over small_call_points for the density pass;
  *                      0: never */
 typedef struct {
   float nh_first_radius;
@@ -145,6 +152,7 @@ typedef struct {
   int32_t tile_call_points;
   int32_t density_filter;
   int32_t fused_event_points;
+  int32_t split_call_points;
 } clue_tuning;
 
 typedef struct clue_plan_s* clue_plan_t;

@@ -132,7 +132,10 @@ TUNINGS = [dict(nh_first_radius=0.05, morton_bits=5),
            dict(small_call_points=1 << 30, tile_call_points=1 << 30),
            dict(small_call_points=0, tile_call_points=1 << 30),
            dict(density_filter=0, density_order=1),
-           dict(fused_event_points=0)]
+           dict(fused_event_points=0),
+           dict(split_call_points=1 << 30, xcd_remap=0),
+           dict(split_call_points=1 << 30, small_call_points=0, xcd_remap=7),
+           dict(split_call_points=0, small_call_points=0)]
 
 
 @pytest.mark.parametrize("tuning", TUNINGS)
@@ -327,7 +330,8 @@ NH_PATHS = [dict(small_call_points=0, tile_call_points=0),                      
             dict(small_call_points=0, tile_call_points=0, nh_first_radius=0.3, pending_group=64),
             dict(small_call_points=0, tile_call_points=1 << 30),                 # tile walk, a lane per query
             dict(small_call_points=1 << 30, tile_call_points=1 << 30),           # tile walk, a wave per query
-            dict(small_call_points=1 << 30, tile_call_points=0)]
+            dict(small_call_points=1 << 30, tile_call_points=0),
+            dict(small_call_points=0, tile_call_points=0, split_call_points=1 << 30)]
 
 
 _RING_CACHE = {}
