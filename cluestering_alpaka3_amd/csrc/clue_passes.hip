@@ -746,9 +746,17 @@ __device__ __forceinline__ float upper_bound2(const float* q, const float* lo, c
 // higher point within it is final (every candidate at or below its distance was
 // examined); the others are queued for the second phase (k_nh_pending), the
 // full search within dm.
-template <int D>
+//
+// SPLIT: the workgroup's four waves share one wave's 64 queries.  Block k of a
+// tile belongs to wave k mod 4; each wave prunes its blocks with its own best,
+// which is never better than the combined one, so the combined minimum's block
+// is scanned by its owner, and the four minima are combined at the end (the
+// minimum of (distance, visit position) does not depend on the order).
+template <int D, bool SPLIT = false>
 __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassArgs a) {
-  const int gid = logical_block((a.xcd >> 1) & 1) * kThreads + threadIdx.x;
+  const int wv = threadIdx.x / kWave;
+  const int gid = SPLIT ? logical_block((a.xcd >> 1) & 1) * kWave + __lane_id()
+                        : logical_block((a.xcd >> 1) & 1) * kThreads + static_cast<int>(threadIdx.x);
   const bool valid = gid < a.n;
   const int pos = valid ? gid : a.n - 1;
   float q[D];
@@ -911,7 +919,8 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
             const float tt = gg * gg;
             lbw = (d == D - 1) ? tt : tt + lbw;
           }
-          uint64_t m = __ballot(inb && lbw <= maxb && bk > wthr);
+          const bool own = !SPLIT || (k & (kThreads / kWave - 1)) == wv;
+          uint64_t m = __ballot(inb && own && lbw <= maxb && bk > wthr);
           while (m) {
             const int kl = __builtin_ctzll(m);
             m &= m - 1;
@@ -958,6 +967,25 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
     }
   });
 
+  if constexpr (SPLIT) {
+    // the four waves' minima, combined by wave 0
+    __shared__ float s_delta[kThreads / kWave][kWave];
+    __shared__ int64_t s_best[kThreads / kWave][kWave];
+    const int lane = __lane_id();
+    s_delta[wv][lane] = delta;
+    s_best[wv][lane] = best;
+    __syncthreads();
+    if (wv != 0) return;
+#pragma unroll
+    for (int w = 1; w < kThreads / kWave; ++w) {
+      const float dw = s_delta[w][lane];
+      const int64_t bw = s_best[w][lane];
+      if (dw < delta || (dw == delta && bw < best)) {
+        delta = dw;
+        best = bw;
+      }
+    }
+  }
   // unresolved positions are compacted in Morton order (flag -> scan -> scatter)
   // so that the full search runs on spatially coherent waves
   const int32_t nh = best == INT64_MAX ? -1 : static_cast<int32_t>(best & 0xffffffff);
@@ -1502,6 +1530,14 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
   // the nearest-higher pass (after either density pass)
   // one wave per query (k_density_wave, k_nearest_higher_wave) for small calls
   const bool wave_mode = D > p->tuning.wave_dims || (p->tuning.small_call_points > 0 && a.n <= p->tuning.small_call_points);
+  // four waves per 64 queries in the density pass (the filtered walk's split
+  // variant) and the first nearest-higher phase, for calls of at most
+  // split_call_points points with the Morton layout; the density's query blocks
+  // are then 64 queries, unordered, and the split density takes precedence over
+  // the wave-per-query one
+  const bool split = U && METRIC == MEUCLID && blocks && p->tuning.split_call_points > 0 &&
+                     a.n <= p->tuning.split_call_points;
+  const bool dsplit = split && a.dfilter;
   auto nearest_higher = [&]() -> clue_status {
     a.radius = dm;
     if constexpr (!U) {
@@ -1536,7 +1572,8 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
         }
         rc = launch_nh_maxkeys(p, a, st);
         if (rc != CLUE_OK) return rc;
-        k_nearest_higher_blocks<D><<<grid, kThreads, 0, st>>>(a);
+        if (split) k_nearest_higher_blocks<D, true><<<div_up(a.n, kWave), kThreads, 0, st>>>(a);
+        else k_nearest_higher_blocks<D><<<grid, kThreads, 0, st>>>(a);
         CLUE_LAUNCH_CHECK("nearest_higher_blocks");
         rc = scan_exclusive(a.pending_flag, a.pending_flag, a.n, p->part.as<int32_t>(), st);
         if (rc != CLUE_OK) return rc;
@@ -1585,9 +1622,7 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
   };
   // small calls: one wave per query (k_density_wave), the same sums in the same order,
   // unless the call runs the split density
-  const bool split_first = U && METRIC == MEUCLID && blocks && a.dfilter && p->tuning.split_call_points > 0 &&
-                           a.n <= p->tuning.split_call_points;
-  if (wave_mode && !split_first) {
+  if (wave_mode && !dsplit) {
     const int wgrid = static_cast<int>(div_up(a.n, kThreads / kWave));
     if constexpr (METRIC == MEUCLID) {
       if (blocks) {
@@ -1625,11 +1660,7 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
   // heaviest-first pays in 2D (a shorter tail); in 3D the lost spatial locality
   // of concurrent blocks costs more than it saves (measured), hence the default
   const bool order = p->tuning.density_order > 0 || (p->tuning.density_order < 0 && D <= 2);
-  // four waves per 64 queries (the filtered walk's split variant) for calls of at
-  // most split_call_points points; its query blocks are 64 queries, unordered
-  const bool split = U && METRIC == MEUCLID && blocks && a.dfilter && p->tuning.split_call_points > 0 &&
-                     a.n <= p->tuning.split_call_points;
-  if (blocks && order && grid <= kOrderMax && !split) {
+  if (blocks && order && grid <= kOrderMax && !dsplit) {
     rc = launch_density_order(p, a, grid, st);
     if (rc != CLUE_OK) return rc;
     a.qorder = p->qorder.as<int32_t>();
@@ -1659,7 +1690,7 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
     constexpr std::false_type no{};
     constexpr std::true_type yes{};
     bool ok;
-    if constexpr (U && METRIC == MEUCLID) ok = !a.dfilter ? launch(no, no) : split ? launch(yes, yes) : launch(yes, no);
+    if constexpr (U && METRIC == MEUCLID) ok = !a.dfilter ? launch(no, no) : dsplit ? launch(yes, yes) : launch(yes, no);
     else ok = launch(no, no);
     if (!ok) return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown convolutional kernel type");
   } else {

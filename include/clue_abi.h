@@ -133,13 +133,13 @@ typedef struct {
  *                      its seeds and labels, in one workgroup per event (two
  *                      launches in place of about twenty); at most 32,768;
  *                      0: never
- *  split_call_points   calls of at most this many points that run the
- *                      filtered density walk (1-3 dimensions, Euclidean, with
- *                      the Morton layout) share each wave's 64 queries among
- *                      four waves, which shortens the longest query's chain
- *                      when the call's waves do not fill the GPU; it takes
- *                      precedence over small_call_points for the density pass;
- *                      0: never */
+ *  split_call_points   calls of at most this many points in 1-3 dimensions
+ *                      (Euclidean, with the Morton layout) share each wave's 64
+ *                      queries among four waves in the density pass (with
+ *                      density_filter) and the first nearest-higher phase,
+ *                      which shortens the longest query's chain when the call's
+ *                      waves do not fill the GPU; it takes precedence over
+ *                      small_call_points for the density pass; 0: never */
 typedef struct {
   float nh_first_radius;
   int32_t morton_bits;
