@@ -358,10 +358,15 @@ __global__ __launch_bounds__(kThreads) void k_gather_morton(const float* xs, con
 }
 
 // offsets[t] = number of points whose tile id is below t (lower bound of t in
-// the sorted tile ids), t in [0, T]: the CSR offsets of the tiles.
-__global__ void k_tile_offsets(const int32_t* skey, int32_t n, int32_t T, int32_t* offsets) {
+// the sorted tile ids), t in [0, T]: the CSR offsets of the tiles.  Also clears
+// the tiles' non-finite flags (set by k_gather) and the per-event counts of
+// points without a nearest-higher.
+__global__ void k_tile_offsets(const int32_t* skey, int32_t n, int32_t T, int32_t* offsets, int32_t* tnf,
+                               int32_t* sc, int32_t n_events) {
   const int t = blockIdx.x * kThreads + threadIdx.x;
+  if (t < n_events) sc[t] = 0;
   if (t > T) return;
+  if (t < T) tnf[t] = 0;
   int lo = 0, hi = n;
   while (lo < hi) {
     const int mid = lo + ((hi - lo) >> 1);
@@ -371,16 +376,6 @@ __global__ void k_tile_offsets(const int32_t* skey, int32_t n, int32_t T, int32_
   offsets[t] = lo;
 }
 
-
-// Tiles holding a weight that is infinite or NaN.  The reference adds
-// ((float)(d <= dc) * k) * w_j for every candidate of the search box
-// (ClusteringKernels.hpp:42); the density pass skips the out-of-radius terms,
-// which are +-0 -- except that 0 * w_j is NaN when w_j is infinite or NaN.  The
-// density pass applies those terms in the flagged tiles (flag starts at 0).
-__global__ void k_tile_nonfinite(const float* ws, const int32_t* skey, int32_t n, int32_t* flag) {
-  const int s = blockIdx.x * kThreads + threadIdx.x;
-  if (s < n && !__builtin_isfinite(ws[s])) flag[skey[s]] = 1;
-}
 
 __global__ void k_block_count(const int32_t* offsets, int32_t T, int32_t* cnt) {
   const int t = blockIdx.x * kThreads + threadIdx.x;
@@ -468,11 +463,13 @@ __global__ __launch_bounds__(kThreads) void k_tile_boxes(const float* bmin, cons
 // nearest-higher searches: a block whose largest key does not exceed a query's
 // threshold holds no point higher than it).  kBoxLanes lanes per block read
 // every kBoxLanes-th key of its Morton positions; then kTileLanes lanes per tile
-// combine its blocks.  An empty tile gets 0, below every key.
+// combine its blocks.  An empty tile gets 0, below every key.  Also clears the
+// count of the first nearest-higher phase's unresolved points (*pending_count).
 __global__ __launch_bounds__(kThreads) void k_block_maxkey(const uint64_t* hk, const int32_t* offsets,
                                                             const int32_t* blk_off, const int32_t* btile,
-                                                            int32_t T, uint64_t* bmaxk) {
+                                                            int32_t T, uint64_t* bmaxk, int32_t* pending_count) {
   const int64_t gid = static_cast<int64_t>(logical_block(1)) * kThreads + threadIdx.x;
+  if (gid == 0) *pending_count = 0;
   const int b = static_cast<int>(gid / kBoxLanes);
   const int sub = threadIdx.x % kBoxLanes;
   if (b >= blk_off[T]) return;  // uniform over the lane group
@@ -589,13 +586,6 @@ __global__ __launch_bounds__(1024) void k_query_block_order(const int32_t* cost,
     }
     order[b] = sorted[src];
   }
-}
-
-// pending[rank] = position for every flagged position; idx is the exclusive scan
-// of the flags (idx[n] = number flagged).
-__global__ void k_compact(const int32_t* idx, int32_t n, int32_t* out) {
-  const int p = blockIdx.x * kThreads + threadIdx.x;
-  if (p < n && idx[p + 1] != idx[p]) out[idx[p]] = p;
 }
 
 // The event and chunk tables of a call over one event, written on the device
@@ -893,7 +883,7 @@ clue_status reserve(clue_plan_s* p, int32_t D, int32_t n, int32_t T, int32_t n_e
   CLUE_HIP(need(p->bmaxk, static_cast<size_t>(ldb) * 8));
   CLUE_HIP(need(p->tmaxk, static_cast<size_t>(T) * 8));
   CLUE_HIP(need(p->pending, (N + 64) * 4));
-  CLUE_HIP(need(p->pending_flag, (N + 1) * 4));
+  CLUE_HIP(need(p->pending_count, 256));
   CLUE_HIP(need(p->qcost, static_cast<size_t>(div_up(n, kThreads)) * 4));
   CLUE_HIP(need(p->qorder, static_cast<size_t>(div_up(n, kThreads)) * 4));
   return CLUE_OK;
@@ -914,17 +904,11 @@ clue_status launch_density_order(clue_plan_s* p, const PassArgs& a, int grid, hi
 // Largest rank key per Morton block and per tile (the nearest-higher filter).
 clue_status launch_nh_maxkeys(clue_plan_s* p, const PassArgs& a, hipStream_t st) {
   k_block_maxkey<<<div_up((div_up(a.n, kBlock) + a.ntiles) * kBoxLanes, kThreads), kThreads, 0, st>>>(
-      a.hk, a.offsets, a.blk_off, a.btile, a.ntiles, p->bmaxk.as<uint64_t>());
+      a.hk, a.offsets, a.blk_off, a.btile, a.ntiles, p->bmaxk.as<uint64_t>(), a.pending_count);
   CLUE_LAUNCH_CHECK("block_maxkey");
   k_tile_maxkey<<<div_up(static_cast<int64_t>(a.ntiles) * kTileLanes, kThreads), kThreads, 0, st>>>(
       p->bmaxk.as<uint64_t>(), a.blk_off, a.ntiles, p->tmaxk.as<uint64_t>());
   CLUE_LAUNCH_CHECK("tile_maxkey");
-  return CLUE_OK;
-}
-
-clue_status launch_compact(const int32_t* idx, int32_t n, int32_t* out, hipStream_t st) {
-  k_compact<<<div_up(n, kThreads), kThreads, 0, st>>>(idx, n, out);
-  CLUE_LAUNCH_CHECK("compact");
   return CLUE_OK;
 }
 
@@ -1309,18 +1293,15 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
 
   // ---- tile offsets
   CLUE_MARK(3);
-  k_tile_offsets<<<div_up(Ttot + 1, kThreads), kThreads, 0, st>>>(sorted_keys, n, Ttot, p->offsets.as<int32_t>());
+  k_tile_offsets<<<div_up(std::max<int64_t>(Ttot + 1, n_events), kThreads), kThreads, 0, st>>>(
+      sorted_keys, n, Ttot, p->offsets.as<int32_t>(), p->tnf.as<int32_t>(), p->sc.as<int32_t>(), n_events);
   CLUE_LAUNCH_CHECK("tile_offsets");
 
   // ---- tile-ordered SoA (+ Morton-ordered SoA and block boxes)
   rc = dispatch_dim(D, [&](auto dc_) {
-    return launch_gather<decltype(dc_)::value>(p, perm, n, h, blocks, st);
+    return launch_gather<decltype(dc_)::value>(p, perm, sorted_keys, n, h, blocks, st);
   });
   if (rc != CLUE_OK) return rc;
-  k_fill_i32<<<div_up(Ttot, kThreads), kThreads, 0, st>>>(p->tnf.as<int32_t>(), 0, Ttot);
-  k_tile_nonfinite<<<div_up(n, kThreads), kThreads, 0, st>>>(p->ws.as<float>(), sorted_keys, n,
-                                                              p->tnf.as<int32_t>());
-  CLUE_LAUNCH_CHECK("tile_nonfinite");
   if (blocks) {
     // Morton layout: the tile-ordered slots sorted by (tile id, Morton code inside the tile)
     rc = radix_sort(p, st, n, tbits + sub_bits * D, p->kn2.as<int32_t>(), p->kn.as<int32_t>(),
@@ -1347,9 +1328,6 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
         p->tbox_lo.as<float>(), p->tbox_hi.as<float>());
     CLUE_LAUNCH_CHECK("tile_boxes");
   }
-  // ---- density + nearest-higher
-  k_fill_i32<<<div_up(n_events, kThreads), kThreads, 0, st>>>(p->sc.as<int32_t>(), 0, n_events);
-  CLUE_LAUNCH_CHECK("fill");
   }  // the layout of calls not fused
 
   PassArgs a{};
@@ -1413,7 +1391,7 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
     a.bmaxk = p->bmaxk.as<uint64_t>();
     a.tmaxk = p->tmaxk.as<uint64_t>();
     a.pending = p->pending.as<int32_t>();
-    a.pending_flag = p->pending_flag.as<int32_t>();
+    a.pending_count = p->pending_count.as<int32_t>();
   }
   a.ntiles = Ttot;
   a.xcd = p->tuning.xcd_remap;
@@ -1461,17 +1439,17 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
   ca.seed_dc = prm->seed_dc;
   ca.rhoc = prm->rhoc;
   for (int d = 0; d < CLUE_MAX_DIM; ++d) ca.mp[d] = md.params[d];
+  // totals of this call ([2, 8) totals, [8, 11) label-round flags); [1], the
+  // grow-only capacity, carries over unless reset
+  ca.counters = counters;
+  ca.clear_cap = p->reset_capacity && persistent_capacity ? 1 : 0;
+  if (persistent_capacity) p->reset_capacity = false;
   rc = dispatch_dim(D, [&](auto dc_) {
     return dispatch_metric_classify<decltype(dc_)::value>(md.type, ca, st);
   });
   if (rc != CLUE_OK) return rc;
   rc = scan_exclusive(p->flag.as<int32_t>(), p->flag.as<int32_t>(), n, p->part.as<int32_t>(), st);
   if (rc != CLUE_OK) return rc;
-  // totals of this call; [1], the grow-only capacity, carries over unless reset
-  k_fill_i32<<<1, 64, 0, st>>>(counters, 0, p->reset_capacity && persistent_capacity ? 2 : 1);
-  if (persistent_capacity) p->reset_capacity = false;
-  k_fill_i32<<<1, 64, 0, st>>>(counters + 2, 0, 9);  // [2, 8) totals, [8, 11) label-round flags
-  CLUE_LAUNCH_CHECK("fill");
   k_capacity<<<div_up(n_events, kThreads), kThreads, 0, st>>>(
       p->events.as<EventDesc>(), n_events, p->sc.as<int32_t>(), p->flag.as<int32_t>(), counters,
       p->cap_used.as<int32_t>(), p->rank_base.as<int32_t>(), prm->seed_mode == CLUE_SEEDS_UNBOUNDED ? 1 : 0,

@@ -147,8 +147,7 @@ struct PassArgs {
   const float* tmin[kMaxDim];
   const float* tmax[kMaxDim];
   int32_t* pending;          // nearest-higher: Morton positions left for the full search
-  int32_t* pending_count;
-  int32_t* pending_flag;     // per Morton position: 1 if unresolved after the first phase
+  int32_t* pending_count;    // its length (cleared by k_block_maxkey, then atomic appends)
   float thr1;                // first-phase search radius threshold
   float box1;                // first-phase search box radius (<= dm)
   float lvl[8];              // second phase: squared radius thresholds of its levels, the last one dm's
@@ -366,6 +365,8 @@ struct ClassifyArgs {
   float seed_dc;
   float rhoc;
   float mp[kMaxDim];
+  int32_t* counters;  // call totals, cleared here: [0], [1] when clear_cap, [2, 11)
+  int clear_cap;
 };
 
 // ===========================================================================
@@ -488,7 +489,7 @@ struct clue_plan_s {
   DevBuf bmaxk, tmaxk;  // largest rank key of each block / tile
   DevBuf qk, qk2, qv, qv2;  // more than three dimensions: sort of the queries by box size
   DevBuf tnf;           // tiles holding an infinite or NaN weight
-  DevBuf pending, pending_flag, qcost, qorder;
+  DevBuf pending, pending_count, qcost, qorder;
   // cluster -> points map (post-processing): sort keys and values
   DevBuf mkey, mkey2, mval, mval2;
   clue_tuning tuning{0.f, -1, 7, 0, 0, -1, 65536, 3, 32768, 1, 12288, 262144};
@@ -525,7 +526,7 @@ struct clue_plan_s {
                    &rho_tmp, &seed_tmp, &bbox_part, &counters, &events, &chunks, &pev,    &sc,
                    &cap_used, &rank_base, &kn,   &kn2,     &pn,      &pn2,      &mid,     &xn,
                    &wn,      &hk,       &bcnt,    &blk_off, &bbox_lo,  &bbox_hi, &tbox_lo,
-                   &tbox_hi, &bmaxk, &tmaxk, &tnf, &qk, &qk2, &qv, &qv2, &pending, &pending_flag, &qcost, &qorder, &mkey,  &mkey2,
+                   &tbox_hi, &bmaxk, &tmaxk, &tnf, &qk, &qk2, &qv, &qv2, &pending, &pending_count, &qcost, &qorder, &mkey,  &mkey2,
                    &mval,    &mval2,  &btile, &packed};
     return k < static_cast<int>(sizeof(b) / sizeof(b[0])) ? b[k] : nullptr;
   }
@@ -544,14 +545,14 @@ int bits_for(int64_t count);
 // launches of the non-templated kernels the neighbour passes need (clue_engine.hip)
 clue_status launch_density_order(clue_plan_s* p, const PassArgs& a, int grid, hipStream_t st);
 clue_status launch_nh_maxkeys(clue_plan_s* p, const PassArgs& a, hipStream_t st);
-clue_status launch_compact(const int32_t* idx, int32_t n, int32_t* out, hipStream_t st);
 
 // per-dimension entry points (clue_passes.hip, one translation unit per D)
 template <int D>
 clue_status launch_bin(const Coords& c, const float* weights, int32_t n_chunks, clue_plan_s* p, const Wrap& wrap,
                        int sub_bits, hipStream_t st);
 template <int D>
-clue_status launch_gather(clue_plan_s* p, const int32_t* perm, int32_t n, float h, bool blocks, hipStream_t st);
+clue_status launch_gather(clue_plan_s* p, const int32_t* perm, const int32_t* skey, int32_t n, float h, bool blocks,
+                          hipStream_t st);
 template <int D>
 clue_status dispatch_metric_passes(int metric, const PassArgs& a, int kt, float dc, float dm,
                                    hipStream_t st, clue_plan_s* p, bool blocks);

@@ -55,10 +55,16 @@ __global__ __launch_bounds__(kThreads) void k_bin(Coords c, const float* w, cons
 
 // Tile-ordered SoA copy: xs[d][slot] = x[d][perm[slot]] (the coalesced tile
 // array), read from the packed copy: one random 16-byte read per 4 values.
+// Also flags the tiles holding a weight that is infinite or NaN (tnf, cleared by
+// k_tile_offsets).  The reference adds ((float)(d <= dc) * k) * w_j for every
+// candidate of the search box (ClusteringKernels.hpp:42); the density pass skips
+// the out-of-radius terms, which are +-0 -- except that 0 * w_j is NaN when w_j
+// is infinite or NaN, and the density pass applies those terms in flagged tiles.
 template <int D>
-__global__ __launch_bounds__(kThreads) void k_gather(const float* packed, const int32_t* perm, int32_t n,
-                                                      int64_t ld, float* xs, float* ws, float* hw, float h,
-                                                      int32_t* key_nh_s) {
+__global__ __launch_bounds__(kThreads) void k_gather(const float* packed, const int32_t* perm,
+                                                      const int32_t* skey, int32_t n, int64_t ld, float* xs,
+                                                      float* ws, float* hw, float h, int32_t* key_nh_s,
+                                                      int32_t* tnf) {
   constexpr int S = packed_stride<D>();
   const int s = logical_block(1) * kThreads + threadIdx.x;
   if (s >= n) return;
@@ -77,6 +83,7 @@ __global__ __launch_bounds__(kThreads) void k_gather(const float* packed, const 
   for (int d = 0; d < D; ++d) xs[d * ld + s] = v[d];
   ws[s] = v[D];
   hw[s] = h * v[D];  // FlatKernel term k*w with k = h, precomputed once per point
+  if (!__builtin_isfinite(v[D])) tnf[skey[s]] = 1;
   if (key_nh_s) key_nh_s[s] = __float_as_int(v[D + 1]);  // (tile, Morton code) key, in tile order
 }
 
@@ -986,10 +993,17 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
       }
     }
   }
-  // unresolved positions are compacted in Morton order (flag -> scan -> scatter)
-  // so that the full search runs on spatially coherent waves
+  // unresolved positions are appended to the pending list, a wave's together in
+  // lane (Morton) order, so that the full search runs on spatially coherent
+  // waves (the order of the waves does not change any result)
   const int32_t nh = best == INT64_MAX ? -1 : static_cast<int32_t>(best & 0xffffffff);
-  if (valid) a.pending_flag[pos] = nh == -1 ? 1 : 0;
+  const uint64_t pm = __ballot(valid && nh == -1);
+  if (pm != 0) {  // uniform; every lane of the wave is active here
+    int base = 0;
+    if (__lane_id() == 0) base = atomicAdd(a.pending_count, __builtin_popcountll(pm));
+    base = __builtin_amdgcn_readfirstlane(base);
+    if (valid && nh == -1) a.pending[base + mbcnt64(pm)] = pos;
+  }
   if (valid && nh != -1) {
     a.nh_out[id_i] = nh;
     a.delta_out[id_i] = delta;
@@ -1222,6 +1236,9 @@ __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
 template <int D, int METRIC>
 __global__ __launch_bounds__(kThreads) void k_classify(const ClassifyArgs a) {
   const int i = blockIdx.x * kThreads + threadIdx.x;
+  // the call's totals start at zero; [1], the grow-only capacity, carries over
+  // unless cleared
+  if (i < 11 && (i != 1 || a.clear_cap)) a.counters[i] = 0;
   if (i >= a.n) return;
   const int32_t nhv = a.nh[i];
   float dist;
@@ -1493,10 +1510,12 @@ clue_status launch_bin(const Coords& c, const float* weights, int32_t n_chunks, 
 }
 
 template <int D>
-clue_status launch_gather(clue_plan_s* p, const int32_t* perm, int32_t n, float h, bool blocks, hipStream_t st) {
-  k_gather<D><<<div_up(n, kThreads), kThreads, 0, st>>>(p->packed.as<float>(), perm, n, p->ld, p->xs.as<float>(),
-                                                        p->ws.as<float>(), p->hw.as<float>(), h,
-                                                        blocks ? p->kn2.as<int32_t>() : nullptr);
+clue_status launch_gather(clue_plan_s* p, const int32_t* perm, const int32_t* skey, int32_t n, float h, bool blocks,
+                          hipStream_t st) {
+  k_gather<D><<<div_up(n, kThreads), kThreads, 0, st>>>(p->packed.as<float>(), perm, skey, n, p->ld,
+                                                        p->xs.as<float>(), p->ws.as<float>(), p->hw.as<float>(), h,
+                                                        blocks ? p->kn2.as<int32_t>() : nullptr,
+                                                        p->tnf.as<int32_t>());
   CLUE_LAUNCH_CHECK("gather");
   return CLUE_OK;
 }
@@ -1575,11 +1594,6 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
         if (split) k_nearest_higher_blocks<D, true><<<div_up(a.n, kWave), kThreads, 0, st>>>(a);
         else k_nearest_higher_blocks<D><<<grid, kThreads, 0, st>>>(a);
         CLUE_LAUNCH_CHECK("nearest_higher_blocks");
-        rc = scan_exclusive(a.pending_flag, a.pending_flag, a.n, p->part.as<int32_t>(), st);
-        if (rc != CLUE_OK) return rc;
-        rc = launch_compact(a.pending_flag, a.n, a.pending, st);
-        if (rc != CLUE_OK) return rc;
-        a.pending_count = a.pending_flag + a.n;
         // enough waves to fill the GPU; surplus waves find the pending list exhausted
         // default grid of the persistent second phase: 8192 workgroups for one
         // event of 2^20 points or more (2D 1M points: 0.51 -> 0.49 ms, 3D 4M: 7.43
@@ -1763,7 +1777,8 @@ clue_status dispatch_metric_classify(int metric, const ClassifyArgs& a, hipStrea
 
 template clue_status launch_bin<CLUE_DIM>(const Coords&, const float*, int32_t, clue_plan_s*, const Wrap&, int,
                                           hipStream_t);
-template clue_status launch_gather<CLUE_DIM>(clue_plan_s*, const int32_t*, int32_t, float, bool, hipStream_t);
+template clue_status launch_gather<CLUE_DIM>(clue_plan_s*, const int32_t*, const int32_t*, int32_t, float, bool,
+                                             hipStream_t);
 template clue_status dispatch_metric_passes<CLUE_DIM>(int, const PassArgs&, int, float, float, hipStream_t,
                                                       clue_plan_s*, bool);
 template clue_status launch_fused_layout<CLUE_DIM>(const FusedArgs&, int32_t, hipStream_t);
