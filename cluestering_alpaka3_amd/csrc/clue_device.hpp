@@ -190,6 +190,34 @@ __device__ __forceinline__ float engine_expf_unit(float x, const double* table =
   return static_cast<float>(__builtin_ldexp(__builtin_fma(t, p, t), k >> 6));
 }
 
+// The unit exponential reads 2^(k/64) whole from a wide table: x in [-14, 0]
+// rounds x*64/ln2 to k in [-1293, 0], and entry k = kExp2Table[k & 63] * 2^(k >> 6)
+// is an exact scaling of the 64-entry table.
+constexpr int kExpWide = 1294;
+__device__ __forceinline__ double exp2_wide_entry(int k) {
+  return __builtin_ldexp(kExp2Table[k & 63], k >> 6);
+}
+
+// engine_expf_unit with 2^(k/64) from the wide table (wide0: its entry k = 0, so
+// that k indexes it directly): no index mask, no shift, no final ldexp.  The
+// fused multiply-add of a table entry scaled by 2^m is 2^m times that of the
+// unscaled entry (no result lies near the subnormal range), so the bits are
+// engine_expf_unit's (checked on every float in [-14, 0], together with the
+// range of k, by oracle/check_expf_exhaustive.c).
+__device__ __forceinline__ float engine_expf_unit_wide(float x, const double* wide0) {
+  const double xd = static_cast<double>(x);
+  const double sh = __builtin_fma(xd, 0x1.71547652b82fep+6, 0x1.8p+52);
+  const int k = static_cast<int>(static_cast<uint32_t>(__builtin_bit_cast(uint64_t, sh)));
+  const double kd = sh - 0x1.8p+52;
+  const double r = __builtin_fma(-kd, 0x1.62e42fefa39efp-7, xd);
+  const double h = __builtin_fma(r, 1.0 / 6.0, 0.5);
+  const double q = __builtin_fma(r, 0.2, 1.0) * (1.0 / 24.0);
+  const double r2 = r * r;
+  const double p = __builtin_fma(r2, __builtin_fma(r2, q, h), r);
+  const double t = wide0[k];
+  return static_cast<float>(__builtin_fma(t, p, t));
+}
+
 // Correctly rounded sqrt of x in [2^-96, FLT_MAX]: the hardware estimate (within
 // one ulp) corrected by the signs of the residuals of its two neighbours -- the
 // IEEE sqrtf lowering without its small-input scaling and special-value paths.
@@ -220,6 +248,9 @@ enum { KFLAT = 0, KGAUSS = 1, KEXP = 2 };
 // UNIT: also the amplitude is exactly 1 (amplitude * e == e) and the argument is
 // known to lie in [-14, 0] for every in-radius distance (checked on the host);
 // 2 (Gaussian): also the mean is 0 and (2 sigma) sigma a power of two.
+// table: with UNIT, the wide table of 2^(k/64) at its entry k = 0 (every caller
+// passes it; the default is for the other evaluations); otherwise kExp2Table or
+// a copy of it.
 template <int KERNEL, bool NONPOS = false, int UNIT = 0>
 __device__ __forceinline__ float kernel_value(float dist, float p0, float p1, float p2, double rcp_den,
                                               const double* table = kExp2Table) {
@@ -228,16 +259,16 @@ __device__ __forceinline__ float kernel_value(float dist, float p0, float p1, fl
   } else if constexpr (KERNEL == KGAUSS && UNIT == 2) {
     // mean 0 and (2 sigma) sigma = 2^e: the quotient -(d*d) / 2^e is the exact
     // scaling of d*d by -2^-e (rcp_den), so one float product gives its bits
-    return engine_expf_unit((dist * dist) * static_cast<float>(-rcp_den), table);
+    return engine_expf_unit_wide((dist * dist) * static_cast<float>(-rcp_den), table);
   } else if constexpr (KERNEL == KGAUSS) {
     const float a = dist - p0;
     const float num = (-a) * a;  // -(d-mu)*(d-mu), divided by (2*sigma)*sigma
     const float x = quotient_via_rcp(num, rcp_den);
-    if constexpr (UNIT) return engine_expf_unit(x, table);
+    if constexpr (UNIT) return engine_expf_unit_wide(x, table);
     return p2 * (NONPOS ? engine_expf_nonpos(x, table) : engine_expf(x));
   } else {
     const float x = (-p0) * dist;
-    if constexpr (UNIT) return engine_expf_unit(x, table);
+    if constexpr (UNIT) return engine_expf_unit_wide(x, table);
     return p1 * (NONPOS ? engine_expf_nonpos(x, table) : engine_expf(x));
   }
 }

@@ -730,6 +730,11 @@ __global__ void k_centroids(Coords c, int ndim, const int32_t* offsets, const in
 // Self-test arithmetic (parity of the device math with the host's).
 // ===========================================================================
 __global__ void k_selftest(int op, const float* a, const float* b, float* out, int64_t n) {
+  __shared__ double s_wide[kExpWide];  // op 9: the density pass's wide table
+  if (op == 9) {
+    for (int k = threadIdx.x; k < kExpWide; k += blockDim.x) s_wide[k] = exp2_wide_entry(k - (kExpWide - 1));
+    __syncthreads();
+  }
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float x = a[i];
@@ -741,6 +746,8 @@ __global__ void k_selftest(int op, const float* a, const float* b, float* out, i
     case 4: out[i] = quotient_via_rcp(x, 1.0 / static_cast<double>(b[i])); break;
     case 5: out[i] = engine_expf_nonpos(x); break;
     case 8: out[i] = engine_expf_unit(x); break;
+    // arguments clamped to [-14, 0], the range the density pass uses it on
+    case 9: out[i] = engine_expf_unit_wide(__builtin_fminf(__builtin_fmaxf(x, -14.f), 0.f), s_wide + (kExpWide - 1)); break;
     case 6:
     case 7: {
       float lo, hi;

@@ -126,12 +126,18 @@ __global__ __launch_bounds__(kThreads) void k_box_class(const PassArgs a, float 
 // heaviest wave.
 template <int D, int METRIC, int KERNEL, bool UNION, bool QNH, int UNIT = 0, bool FILT = false, bool SPLIT = false>
 __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
-  // the exponential's 2^(j/64) table, read through LDS by the kernel evaluations
-  __shared__ double s_exp2[KERNEL == KFLAT ? 1 : 64];
+  // the exponential's table in LDS: 2^(j/64), or for UNIT the wide table of
+  // 2^(k/64) over the unit evaluation's whole range of k
+  __shared__ double s_exp2[KERNEL == KFLAT ? 1 : (UNIT != 0 ? kExpWide : 64)];
   if constexpr (KERNEL != KFLAT) {
-    if (threadIdx.x < 64) s_exp2[threadIdx.x] = kExp2Table[threadIdx.x];
+    if constexpr (UNIT != 0) {
+      for (int i = threadIdx.x; i < kExpWide; i += kThreads) s_exp2[i] = exp2_wide_entry(i - (kExpWide - 1));
+    } else {
+      if (threadIdx.x < 64) s_exp2[threadIdx.x] = kExp2Table[threadIdx.x];
+    }
     __syncthreads();
   }
+  const double* const exp_table = UNIT != 0 ? s_exp2 + (kExpWide - 1) : s_exp2;
   // the filtered union walk's per-wave ring of candidates (coordinates, value, slot)
   constexpr bool kFilter = FILT && UNION && METRIC == MEUCLID;
   static_assert(!SPLIT || (kFilter && QNH), "the split walk is a variant of the filtered walk");
@@ -200,7 +206,7 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
           if (!SELF || j != s) {
             if ((UNIT || a.exp_nonpos) && s2 >= 0x1p-96f) {
               k = kernel_value<KERNEL, true, UNIT>(sqrt_rn_normal(s2), a.kp[0], a.kp[1], a.kp[2], a.kden_rcp,
-                                                   s_exp2);
+                                                   exp_table);
             } else {
               k = kernel_value<KERNEL>(__builtin_sqrtf(s2), a.kp[0], a.kp[1], a.kp[2], a.kden_rcp);
             }
@@ -420,11 +426,16 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
 // terms; here that chain is one add per in-radius term.
 template <int D, int METRIC, int KERNEL, bool QNH, int UNIT>
 __global__ __launch_bounds__(kThreads) void k_density_wave(const PassArgs a) {
-  __shared__ double s_exp2[KERNEL == KFLAT ? 1 : 64];
+  __shared__ double s_exp2[KERNEL == KFLAT ? 1 : (UNIT != 0 ? kExpWide : 64)];
   if constexpr (KERNEL != KFLAT) {
-    if (threadIdx.x < 64) s_exp2[threadIdx.x] = kExp2Table[threadIdx.x];
+    if constexpr (UNIT != 0) {
+      for (int i = threadIdx.x; i < kExpWide; i += kThreads) s_exp2[i] = exp2_wide_entry(i - (kExpWide - 1));
+    } else {
+      if (threadIdx.x < 64) s_exp2[threadIdx.x] = kExp2Table[threadIdx.x];
+    }
     __syncthreads();
   }
+  const double* const exp_table = UNIT != 0 ? s_exp2 + (kExpWide - 1) : s_exp2;
   const int pos = uniform(static_cast<int>(blockIdx.x) * (kThreads / kWave) + static_cast<int>(threadIdx.x / kWave));
   if (pos >= a.n) return;  // the whole wave
   const int lane = __lane_id();
@@ -484,7 +495,7 @@ __global__ __launch_bounds__(kThreads) void k_density_wave(const PassArgs a) {
               if (j != s) {
                 if ((UNIT || a.exp_nonpos) && s2 >= 0x1p-96f) {
                   k = kernel_value<KERNEL, true, UNIT>(sqrt_rn_normal(s2), a.kp[0], a.kp[1], a.kp[2], a.kden_rcp,
-                                                       s_exp2);
+                                                       exp_table);
                 } else {
                   k = kernel_value<KERNEL>(__builtin_sqrtf(s2), a.kp[0], a.kp[1], a.kp[2], a.kden_rcp);
                 }

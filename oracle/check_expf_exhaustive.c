@@ -34,6 +34,34 @@ static float expf_unit(float x) {
   return (float)ldexp(fma(t, p, t), k >> 6);
 }
 
+/* The same evaluation with 2^(k/64) read whole from the wide table the density
+ * pass builds in LDS (entry k = unit_table[k & 63] * 2^(k >> 6), k in
+ * [-1293, 0]): no mask, no shift, no final ldexp.  Its bits must be expf_unit's,
+ * and k must stay inside the table on every argument. */
+#define WIDE 1294
+static double wide_table[WIDE];
+static long long wide_out_of_range;
+static float expf_unit_wide(float x) {
+  const double xd = (double)x;
+  const double sh = fma(xd, 0x1.71547652b82fep+6, 0x1.8p+52);
+  uint64_t bits;
+  memcpy(&bits, &sh, 8);
+  const int k = (int)(uint32_t)bits;
+  if (k < -(WIDE - 1) || k > 0) {
+#pragma omp atomic
+    ++wide_out_of_range;
+    return 0.0f;
+  }
+  const double kd = sh - 0x1.8p+52;
+  const double r = fma(-kd, 0x1.62e42fefa39efp-7, xd);
+  const double h = fma(r, 1.0 / 6.0, 0.5);
+  const double q = fma(r, 0.2, 1.0) * (1.0 / 24.0);
+  const double r2 = r * r;
+  const double p = fma(r2, fma(r2, q, h), r);
+  const double t = wide_table[k + WIDE - 1];
+  return (float)fma(t, p, t);
+}
+
 int main(void) {
   long long bad = 0, total = 0;
 #pragma omp parallel for reduction(+ : bad, total) schedule(dynamic, 1 << 20)
@@ -48,17 +76,21 @@ int main(void) {
   }
   printf("exhaustive: %lld inputs in [-110, 89], %lld mismatches\n", total, bad);
   for (int i = 0; i < 64; ++i) unit_table[i] = (double)exp2l((long double)i / 64.0L);
-  long long bad_u = 0, total_u = 0;
-#pragma omp parallel for reduction(+ : bad_u, total_u) schedule(dynamic, 1 << 20)
+  for (int k = -(WIDE - 1); k <= 0; ++k) wide_table[k + WIDE - 1] = ldexp(unit_table[k & 63], k >> 6);
+  long long bad_u = 0, total_u = 0, bad_w = 0;
+#pragma omp parallel for reduction(+ : bad_u, total_u, bad_w) schedule(dynamic, 1 << 20)
   for (long long u = 0x80000000ll; u < (1ll << 32); ++u) {
     const uint32_t b = (uint32_t)u;
     float x;
     memcpy(&x, &b, 4);
     if (!(x >= -14.0f)) continue;
     ++total_u;
-    const float e = expf_unit(x), l = oracle_expf_longdouble(x);
+    const float e = expf_unit(x), l = oracle_expf_longdouble(x), w = expf_unit_wide(x);
     if (memcmp(&e, &l, 4) != 0) ++bad_u;
+    if (memcmp(&w, &e, 4) != 0) ++bad_w;
   }
   printf("exhaustive: %lld inputs in [-14, 0], unit-amplitude evaluation, %lld mismatches\n", total_u, bad_u);
-  return bad != 0 || bad_u != 0;
+  printf("exhaustive: %lld inputs in [-14, 0], wide-table evaluation against the unit one, %lld mismatches"
+         " (%lld indexes outside the table)\n", total_u, bad_w, wide_out_of_range);
+  return bad != 0 || bad_u != 0 || bad_w != 0 || wide_out_of_range != 0;
 }

@@ -52,14 +52,16 @@ def test_device_math_matches_host():
         np.testing.assert_array_equal(clue.selftest_math(5, xs).view(np.int32),
                                       oracle.expf_engine(xs).view(np.int32))
     # the density pass's evaluation for unit-amplitude kernels, arguments in [-14, 0]
-    # (every 7th float there, about 157 million, plus both ends)
+    # (every 7th float there, about 157 million, plus both ends), with the 64-entry
+    # table and with the wide table the density pass reads
     lo14 = int(np.float32(-14.0).view(np.uint32))
     unit = np.concatenate([np.arange(0x80000000, lo14 + 1, 7, dtype=np.uint64),
                            np.array([0x80000000, lo14], np.uint64)]).astype(np.uint32)
     for k in range(0, unit.size, 1 << 23):
         xs = unit[k:k + (1 << 23)].view(np.float32)
-        np.testing.assert_array_equal(clue.selftest_math(8, xs).view(np.int32),
-                                      oracle.expf_engine(xs).view(np.int32))
+        want = oracle.expf_engine(xs).view(np.int32)
+        np.testing.assert_array_equal(clue.selftest_math(8, xs).view(np.int32), want)
+        np.testing.assert_array_equal(clue.selftest_math(9, xs).view(np.int32), want)
     # sqrtf and division are correctly rounded on both sides
     rng = np.random.default_rng(1)
     a = np.abs(rng.standard_normal(200000).astype(np.float32)) * np.float32(1e3)

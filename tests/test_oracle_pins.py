@@ -535,7 +535,8 @@ def test_periodic_known_answer(ref_data):
 def test_engine_exp_exhaustive():
     # every float of [-110, 89] (the engine exp) and of [-14, 0] (the density
     # pass's unit-amplitude evaluation, restated in the checker operation by
-    # operation) against long double expl rounded once to float
+    # operation) against long double expl rounded once to float; the wide-table
+    # form of the unit evaluation equal to it bit for bit, its index in range
     import os
     import subprocess
 
@@ -543,4 +544,6 @@ def test_engine_exp_exhaustive():
     r = subprocess.run(["make", "-C", oracle_dir, "-s", "check-exp"], capture_output=True, text=True, timeout=900)
     assert r.returncode == 0, r.stdout + r.stderr
     lines = [l for l in r.stdout.splitlines() if l.startswith("exhaustive:")]
-    assert len(lines) == 2 and all(l.endswith(", 0 mismatches") for l in lines), r.stdout
+    assert len(lines) == 3, r.stdout
+    assert lines[0].endswith(", 0 mismatches") and lines[1].endswith(", 0 mismatches"), r.stdout
+    assert lines[2].endswith(", 0 mismatches (0 indexes outside the table)"), r.stdout
