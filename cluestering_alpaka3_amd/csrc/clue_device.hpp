@@ -273,6 +273,17 @@ __device__ __forceinline__ float kernel_value(float dist, float p0, float p1, fl
   }
 }
 
+// Unit evaluations that are exactly 1 at every squared distance below 2^-96,
+// where the fast square root is not exact: the host takes UNIT 2 for the
+// Gaussian only with 1/((2 sigma) sigma) <= 2^70, and UNIT for the exponential
+// only with a rate <= 2^22, so the argument stays above -2^-25 there, whose
+// correctly rounded exp is 1.  They need no test of the distance against
+// 2^-96, and no test of j == i (the self term's kernel value is 1).
+template <int KERNEL, int UNIT>
+constexpr bool unit_tiny_exact() {
+  return (UNIT != 0 && KERNEL == KEXP) || (UNIT == 2 && KERNEL == KGAUSS);
+}
+
 // Distance metrics (core/DistanceMetrics.hpp).  The squared-term sum is the
 // right fold of meta::accumulate (internal/meta/accumulate.hpp:14-15).
 enum { MEUCLID = 0, MWEUCLID = 1, MPERIODIC = 2, MMANHATTAN = 3, MCHEBYSHEV = 4, MWCHEBYSHEV = 5 };

@@ -1371,12 +1371,18 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
     }
     const float amp = kd.type == CLUE_KERNEL_GAUSSIAN ? kd.params[2] : kd.params[1];
     a.exp_unit = amp == 1.0f && lowest * 1.001 <= 14.0;
+    // the exponential's unit evaluation also runs below a squared distance of
+    // 2^-96 (where the fast square root is not exact); a rate of at most 2^22
+    // keeps the argument there above -2^-25, whose exp is exactly 1
+    if (kd.type == CLUE_KERNEL_EXPONENTIAL && !(kd.params[0] <= 0x1p22f)) a.exp_unit = 0;
     if (a.exp_unit && kd.type == CLUE_KERNEL_GAUSSIAN && kd.params[0] == 0.f) {
-      // (2 sigma) sigma = 2^e with 2^-e a float: -(d*d) / den is d*d scaled exactly
+      // (2 sigma) sigma = 2^e with 2^-e a float: -(d*d) / den is d*d scaled exactly;
+      // 2^-e <= 2^70 keeps the argument above -2^-25 at every squared distance
+      // below 2^-96, where this evaluation runs too
       const float den = (2.f * kd.params[1]) * kd.params[1];
       int e = 0;
       const float mant = std::frexp(den, &e);  // den = mant * 2^e, mant in [0.5, 1)
-      if (std::isfinite(den) && den > 0.f && mant == 0.5f && e - 1 <= 149 && e - 1 >= -127) a.exp_unit = 2;
+      if (std::isfinite(den) && den > 0.f && mant == 0.5f && e - 1 <= 149 && e - 1 >= -70) a.exp_unit = 2;
     }
   }
   for (int d = 0; d < CLUE_MAX_DIM; ++d) a.mp[d] = md.params[d];

@@ -148,7 +148,7 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
   // split: the terms of a batch, [batch parity][candidate][lane]
   __shared__ float s_terms[SPLIT ? 2 : 1][SPLIT ? kFB : 1][SPLIT ? kWave : 1];
   // kernel value 1 at distance 0, the self term's value: the self test is not needed
-  constexpr bool kSelfFree = (UNIT != 0 && KERNEL == KEXP) || (UNIT == 2 && KERNEL == KGAUSS);
+  constexpr bool kSelfFree = unit_tiny_exact<KERNEL, UNIT>();
   __shared__ float4 s_ring[kFilter ? kThreads / kWave : 1][kFilter ? kRing : 1];
   __shared__ int32_t s_ringj[kFilter && D == 3 ? kThreads / kWave : 1][kFilter && D == 3 ? kRing : 1];
   const int qb = a.qorder ? a.qorder[blockIdx.x] : logical_block(a.xcd & 1);
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
         if (s2 <= a.thr) {
           float k = 1.f;
           if (!SELF || j != s) {
-            if ((UNIT || a.exp_nonpos) && s2 >= 0x1p-96f) {
+            if (unit_tiny_exact<KERNEL, UNIT>() || ((UNIT || a.exp_nonpos) && s2 >= 0x1p-96f)) {
               k = kernel_value<KERNEL, true, UNIT>(sqrt_rn_normal(s2), a.kp[0], a.kp[1], a.kp[2], a.kden_rcp,
                                                    exp_table);
             } else {
@@ -493,7 +493,7 @@ __global__ __launch_bounds__(kThreads) void k_density_wave(const PassArgs a) {
             } else {
               float k = 1.f;
               if (j != s) {
-                if ((UNIT || a.exp_nonpos) && s2 >= 0x1p-96f) {
+                if (unit_tiny_exact<KERNEL, UNIT>() || ((UNIT || a.exp_nonpos) && s2 >= 0x1p-96f)) {
                   k = kernel_value<KERNEL, true, UNIT>(sqrt_rn_normal(s2), a.kp[0], a.kp[1], a.kp[2], a.kden_rcp,
                                                        exp_table);
                 } else {

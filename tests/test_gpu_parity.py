@@ -448,6 +448,43 @@ def test_far_and_scaled_coordinates(offset, scale):
     assert_same(*run_both(x, w, float(dc), 3.0, float(dm), ppb=1, kernel=("gaussian", 0.0, float(scale), 1.0)))
 
 
+def tiny_clusters(scale, tiny, n_wide=500, n_tiny=300, seed=9):
+    """n_wide points uniform in [0, scale]^2 and two groups of n_tiny points at the
+    origin, within `tiny` and 2^-10 `tiny` of it: squared distances below 2^-96
+    (normal and subnormal) besides ordinary ones."""
+    rng = np.random.default_rng(seed)
+    wide = rng.random((2, n_wide)) * scale
+    t1 = rng.random((2, n_tiny)) * tiny
+    t2 = rng.random((2, n_tiny)) * tiny * 2.0 ** -10
+    x = np.concatenate([wide, t1, t2], axis=1).astype(np.float32)
+    w = (rng.random(x.shape[1]) + 0.5).astype(np.float32)
+    return np.ascontiguousarray(x), w
+
+
+@pytest.mark.parametrize("case", ["gauss_unit", "exp_unit", "gauss_steep", "exp_steep"])
+def test_tiny_distances(case):
+    # squared distances below 2^-96, where the kernels' fast square root is not
+    # exact: the unit evaluations run there and must give exactly 1 (Gaussian of
+    # mean 0 with 1/((2 sigma) sigma) <= 2^70, exponential with a rate <= 2^22);
+    # steeper kernels, whose value there is not 1, keep the exact square root
+    if case == "gauss_unit":
+        x, w = tiny_clusters(5.0, 1e-16)
+        kernel, dc = ("gaussian", 0.0, 1.0, 1.0), 2.0
+    elif case == "exp_unit":
+        x, w = tiny_clusters(5.0, 1e-16)
+        kernel, dc = ("exponential", 4.0, 1.0), 2.0
+    elif case == "gauss_steep":  # (2 sigma) sigma = 2^-81: 1/den = 2^81
+        x, w = tiny_clusters(2e-12, 1e-15)
+        kernel, dc = ("gaussian", 0.0, 2.0 ** -41, 1.0), 2e-12
+    else:  # rate 2^23
+        x, w = tiny_clusters(1.5e-6, 1e-15)
+        kernel, dc = ("exponential", 2.0 ** 23, 1.0), 1.5e-6
+    got, ref, *rest = run_both(x, w, dc, 3.0, 2 * dc, kernel=kernel)
+    assert_same(got, ref, *rest)
+    d2 = (x[0, -1] - x[0, -2]) ** 2 + (x[1, -1] - x[1, -2]) ** 2
+    assert d2 < 2.0 ** -96
+
+
 def test_invalid_arguments():
     with pytest.raises(clue.InvalidArgument):
         clue.cluster_points(np.zeros((2, 4), np.float32), np.ones(4, np.float32), -1.0, 1.0)
