@@ -357,6 +357,88 @@ __global__ __launch_bounds__(kThreads) void k_gather_morton(const float* xs, con
   mid[p] = perm[s];
 }
 
+// Morton order inside each tile: the tile-order slots [offsets[t], offsets[t+1])
+// of tile t sorted by the Morton code in the low `bits` bits of their keys, one
+// kernel in place of a radix sort of all slots by (tile id, code).  One wave per
+// tile, tiles strided over the grid.  A tile of at most 64 points ranks each
+// point against every other (lane j's code read as a wave-uniform value; equal
+// codes in slot order).  A larger tile counts its codes in an LDS histogram of
+// 2^KB entries (KB >= bits), scans it into bucket starts, and takes each slot's
+// position from an LDS atomic add on its bucket's start, 64 slots at a time in
+// slot order: equal codes follow the slot order between chunks, and inside a
+// chunk the order in which the LDS serves the lanes.  Any order of the points of
+// one Morton cell is a valid layout (no result depends on it).  slot_of[p] is
+// the tile-order slot of Morton position p.  A workgroup is one wave, whose LDS
+// operations execute in program order: a wavefront-scope fence keeps the
+// compiler from moving them across each other, without a workgroup barrier's
+// wait for the wave's outstanding global stores.
+__device__ __forceinline__ void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); }
+
+template <int KB>
+__global__ __launch_bounds__(kWave) void k_tile_morton(const int32_t* keys, const int32_t* offsets, int32_t T,
+                                                       int bits, int32_t* slot_of) {
+  __shared__ int32_t hist[1 << KB];
+  const int lane = __lane_id();
+  const int32_t mask = (1 << bits) - 1;
+  // the keys of kGroup chunks are loaded together, their loads in flight at once
+  constexpr int kGroup = 8;
+  for (int t = blockIdx.x; t < T; t += gridDim.x) {
+    const int32_t o0 = uniform(offsets[t]), o1 = uniform(offsets[t + 1]);
+    const int32_t c = o1 - o0;
+    if (c <= 0) continue;
+    if (c <= kWave) {
+      const bool v = lane < c;
+      const int32_t code = keys[v ? o0 + lane : o0] & mask;
+      int rank = 0;
+      for (int j = 0; j < c; ++j) {
+        const int32_t cj = __builtin_amdgcn_readlane(code, j);
+        rank += (cj < code || (cj == code && j < lane)) ? 1 : 0;
+      }
+      if (v && rank < c) slot_of[o0 + rank] = o0 + lane;
+      continue;
+    }
+    auto load_group = [&](int32_t base, int32_t* kk) {
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u) {
+        const int32_t s = base + u * kWave + lane;
+        kk[u] = keys[s < o1 ? s : o1 - 1];  // clamped inside the tile
+      }
+    };
+    for (int i = lane; i < (1 << KB); i += kWave) hist[i] = 0;
+    wave_fence();
+    for (int32_t base = o0; base < o1; base += kGroup * kWave) {
+      int32_t kk[kGroup];
+      load_group(base, kk);
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u)
+        if (base + u * kWave + lane < o1) atomicAdd(&hist[kk[u] & mask], 1);
+    }
+    wave_fence();
+    // exclusive scan, one row of 64 entries at a time
+    int32_t carry = 0;
+    for (int i = 0; i < (1 << KB); i += kWave) {
+      const int32_t h = hist[i + lane];
+      const int32_t incl = wave_incl_scan(h);
+      hist[i + lane] = carry + incl - h;
+      carry += __builtin_amdgcn_readlane(incl, kWave - 1);
+    }
+    wave_fence();
+    for (int32_t base = o0; base < o1; base += kGroup * kWave) {
+      int32_t kk[kGroup];
+      load_group(base, kk);
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u) {
+        const int32_t s = base + u * kWave + lane;
+        if (s < o1) {
+          const int32_t pos = atomicAdd(&hist[kk[u] & mask], 1);
+          if (pos >= 0 && pos < c) slot_of[o0 + pos] = s;
+        }
+      }
+    }
+    wave_fence();
+  }
+}
+
 // offsets[t] = number of points whose tile id is below t (lower bound of t in
 // the sorted tile ids), t in [0, T]: the CSR offsets of the tiles.  Also clears
 // the tiles' non-finite flags (set by k_gather) and the per-event counts of
@@ -1264,7 +1346,8 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
   }
   int32_t* sorted_keys = nullptr;
   int32_t* perm = nullptr;     // slot -> point id
-  int32_t* mkeys = nullptr;
+  int32_t* mkeys = nullptr;    // tile id << morton_shift of each Morton position
+  int morton_shift = sub_bits * D;
   int32_t* slot_of = nullptr;  // Morton position -> tile-order slot
   if (fused) {
     CLUE_MARK(0);
@@ -1310,10 +1393,27 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
   });
   if (rc != CLUE_OK) return rc;
   if (blocks) {
-    // Morton layout: the tile-ordered slots sorted by (tile id, Morton code inside the tile)
-    rc = radix_sort(p, st, n, tbits + sub_bits * D, p->kn2.as<int32_t>(), p->kn.as<int32_t>(),
-                    p->pn.as<int32_t>(), p->pn2.as<int32_t>(), true, &mkeys, &slot_of);
-    if (rc != CLUE_OK) return rc;
+    // Morton layout: the tile-ordered slots sorted by (tile id, Morton code inside
+    // the tile) -- one kernel sorting each tile's slots when the code has at most
+    // 12 bits (the tile of Morton position p is then the tile of slot p, whose
+    // sorted keys the first sort left), else a radix sort of all of them
+    if (sub_bits * D <= 12) {
+      slot_of = p->pn.as<int32_t>();
+      const int tgrid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(Ttot, 8192)));
+      if (sub_bits * D <= 10)
+        k_tile_morton<10><<<tgrid, kWave, 0, st>>>(p->kn2.as<int32_t>(), p->offsets.as<int32_t>(), Ttot,
+                                                    sub_bits * D, slot_of);
+      else
+        k_tile_morton<12><<<tgrid, kWave, 0, st>>>(p->kn2.as<int32_t>(), p->offsets.as<int32_t>(), Ttot,
+                                                    sub_bits * D, slot_of);
+      CLUE_LAUNCH_CHECK("tile_morton");
+      mkeys = sorted_keys;
+      morton_shift = 0;
+    } else {
+      rc = radix_sort(p, st, n, tbits + sub_bits * D, p->kn2.as<int32_t>(), p->kn.as<int32_t>(),
+                      p->pn.as<int32_t>(), p->pn2.as<int32_t>(), true, &mkeys, &slot_of);
+      if (rc != CLUE_OK) return rc;
+    }
     k_gather_morton<<<div_up(n, kThreads), kThreads, 0, st>>>(p->xs.as<float>(), p->ws.as<float>(), perm, slot_of,
                                                               n, D, p->ld, p->xn.as<float>(), p->wn.as<float>(),
                                                               p->mid.as<int32_t>());
@@ -1398,7 +1498,7 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
     a.sn = slot_of;
     a.pn = p->mid.as<int32_t>();
     a.kn = mkeys;
-    a.mshift = sub_bits * D;
+    a.mshift = morton_shift;
     a.blk_off = p->blk_off.as<int32_t>();
     a.btile = p->btile.as<int32_t>();
     a.bmaxk = p->bmaxk.as<uint64_t>();

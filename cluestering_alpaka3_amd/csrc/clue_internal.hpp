@@ -137,7 +137,7 @@ struct PassArgs {
   const float* wn;
   const int32_t* sn;         // Morton position -> tile-order slot
   const int32_t* pn;         // Morton position -> point id
-  const int32_t* kn;         // sorted Morton keys (tile id << mshift | code)
+  const int32_t* kn;         // per Morton position: its tile id << mshift (with the Morton code below when mshift > 0)
   int32_t mshift;
   const int32_t* blk_off;    // first block of each tile
   const int32_t* btile;      // tile of each block
