@@ -232,6 +232,15 @@ __device__ __forceinline__ float sqrt_rn_normal(float x) {
   return y;
 }
 
+// Correctly rounded sqrtf of a finite s >= +0: below 2^-96 the fast evaluation
+// runs on s * 2^64, whose square root is exactly 2^32 times sqrtf(s) (both are
+// normal floats there, and scaling by a power of two commutes with rounding).
+__device__ __forceinline__ float sqrt_rn_finite(float s) {
+  const bool tiny = s < 0x1p-96f;
+  const float r = sqrt_rn_normal(tiny ? s * 0x1p64f : s);
+  return tiny ? r * 0x1p-32f : r;
+}
+
 // Gaussian kernel argument -(a*a) / den as the correctly rounded float quotient,
 // computed as (float)((double)num * (1/den)) with 1/den rounded to double on the
 // host: a quotient of two floats lies at least ~2^-48 (relative) from any float

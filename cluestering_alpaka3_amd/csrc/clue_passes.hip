@@ -817,17 +817,17 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
     qhi[d] = wave_max_f32(valid ? q[d] : -__builtin_inff());
   }
 
+  // (the update by selects: no branch inside the bound test)
   auto cand = [&](int t, const float* c, uint64_t key_j) {
     const float s2 = squared_distance<D>(q, c);
     if (key_j > thr_i && s2 <= bound) {
       const int64_t key = (static_cast<int64_t>(t) << 32) | static_cast<uint32_t>(key_j);
-      const float dc = __builtin_sqrtf(s2);
-      if (dc < delta || (dc == delta && key < best)) {
-        delta = dc;
-        best = key;
-        const float up = not_farther_than(s2);
-        bound = up < cap ? up : cap;
-      }
+      const float dc = sqrt_rn_finite(s2);
+      const bool better = (dc < delta) | ((dc == delta) & (key < best));
+      const float up = not_farther_than(s2);
+      delta = better ? dc : delta;
+      best = better ? key : best;
+      bound = better ? (up < cap ? up : cap) : bound;
     }
   };
   // candidates [beg, end) of tile t, wave-uniform: eight at a time, all loads
@@ -1122,13 +1122,12 @@ __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
       const float s2 = squared_distance<D>(q, c);
       if (key_j > thr_i && s2 <= bound) {
         const int64_t key = (static_cast<int64_t>(t) << 32) | static_cast<uint32_t>(key_j);
-        const float dc = __builtin_sqrtf(s2);
-        if (dc < delta || (dc == delta && key < best)) {
-          delta = dc;
-          best = key;
-          const float up = not_farther_than(s2);
-          bound = up < a.thr ? up : a.thr;
-        }
+        const float dc = sqrt_rn_finite(s2);
+        const bool better = (dc < delta) | ((dc == delta) & (key < best));
+        const float up = not_farther_than(s2);
+        delta = better ? dc : delta;
+        best = better ? key : best;
+        bound = better ? (up < a.thr ? up : a.thr) : bound;
       }
     };
     for (int lv = 0; lv < a.nlvl; ++lv) {

@@ -20,6 +20,16 @@ def test_fast_sqrt_exhaustive():
     first = int(np.float32(2.0 ** -96).view(np.uint32))
     last = int(np.finfo(np.float32).max.view(np.uint32))
     assert clue.selftest_sqrt(first, last - first + 1) == 0
+    # the nearest-higher searches' square root of any finite distance: below 2^-96
+    # (every 11th float there, subnormals included, and both ends) on an argument
+    # scaled by 2^64 into the range above, and the range above itself
+    tiny = np.concatenate([np.arange(0, first, 11, dtype=np.uint64),
+                           np.array([0, 1, first - 1, first], np.uint64)]).astype(np.uint32)
+    for k in range(0, tiny.size, 1 << 23):
+        xs = tiny[k:k + (1 << 23)].view(np.float32)
+        np.testing.assert_array_equal(clue.selftest_math(10, xs).view(np.int32), np.sqrt(xs).view(np.int32))
+    big = np.arange(first, last + 1, 997, dtype=np.uint64).astype(np.uint32).view(np.float32)
+    np.testing.assert_array_equal(clue.selftest_math(10, big).view(np.int32), np.sqrt(big).view(np.int32))
 
 
 def test_device_math_matches_host():
