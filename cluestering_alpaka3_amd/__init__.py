@@ -73,7 +73,8 @@ class ClueTuning(ctypes.Structure):
                 ("pending_group", ctypes.c_int32), ("density_order", ctypes.c_int32),
                 ("small_call_points", ctypes.c_int32), ("wave_dims", ctypes.c_int32),
                 ("tile_call_points", ctypes.c_int32), ("density_filter", ctypes.c_int32),
-                ("fused_event_points", ctypes.c_int32), ("split_call_points", ctypes.c_int32)]
+                ("fused_event_points", ctypes.c_int32), ("split_call_points", ctypes.c_int32),
+                ("nh_lone_lanes", ctypes.c_int32)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

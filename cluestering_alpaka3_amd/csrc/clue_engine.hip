@@ -1510,6 +1510,7 @@ clue_status enqueue_batch(clue_plan_s* p, hipStream_t st, const clue_params* prm
   a.ntiles = Ttot;
   a.xcd = p->tuning.xcd_remap;
   a.dfilter = p->tuning.density_filter;
+  a.nh_lone = p->tuning.nh_lone_lanes;
   a.tnf = p->tnf.as<int32_t>();
   // rank keys of the densities: Morton order for the block search, tile order for the tile walk
   a.hk_out = p->hk.as<uint64_t>();
@@ -2026,7 +2027,8 @@ clue_status clue_plan_set_tuning(clue_plan_t plan, const clue_tuning* tuning) {
       tuning->density_order < -1 || tuning->density_order > 1 || tuning->small_call_points < 0 ||
       tuning->wave_dims < 0 || tuning->wave_dims > CLUE_MAX_DIM || tuning->tile_call_points < 0 ||
       tuning->density_filter < 0 || tuning->density_filter > 1 || tuning->fused_event_points < 0 ||
-      tuning->fused_event_points > kFusedMaxPoints || tuning->split_call_points < 0)
+      tuning->fused_event_points > kFusedMaxPoints || tuning->split_call_points < 0 ||
+      tuning->nh_lone_lanes < 0 || tuning->nh_lone_lanes > kWave)
     return fail(CLUE_ERR_INVALID_ARGUMENT, "invalid tuning values");
   plan->tuning = *tuning;
   return CLUE_OK;
@@ -2321,6 +2323,7 @@ clue_status clue_autotune_batched(clue_plan_t plan, void* stream, clue_params* p
   auto set_xcd = [](clue_tuning& t, float v) { t.xcd_remap = static_cast<int32_t>(v); };
   auto set_group = [](clue_tuning& t, float v) { t.pending_group = static_cast<int32_t>(v); };
   try_field(set_radius, {0.08f, 0.12f, 0.18f, 0.25f, 0.35f, 0.5f, 0.75f, 1.0f});
+  try_field([](clue_tuning& t, float v) { t.nh_lone_lanes = static_cast<int32_t>(v); }, {0.f, 2.f, 8.f});
   try_field(set_xcd, {0.f, 1.f, 2.f, 3.f, 4.f, 5.f, 6.f, 7.f});
   try_field(set_group, {8.f, 16.f, 32.f, 64.f});
   try_field([](clue_tuning& t, float v) { t.density_order = static_cast<int32_t>(v); }, {0.f, 1.f});

@@ -169,6 +169,8 @@ struct PassArgs {
   const int32_t* tnf;        // density: 1 for tiles holding an infinite or NaN weight
   const int32_t* qperm;      // tile-order walks: query slots by growing search-box size (or null)
   int dfilter;               // density: filtered union walk (tuning density_filter)
+  int nh_lone;               // first nearest-higher phase: blocks that at most this many lanes need
+                             // are scanned across the wave (tuning nh_lone_lanes)
 };
 
 
@@ -492,7 +494,7 @@ struct clue_plan_s {
   DevBuf pending, pending_count, qcost, qorder;
   // cluster -> points map (post-processing): sort keys and values
   DevBuf mkey, mkey2, mval, mval2;
-  clue_tuning tuning{0.f, -1, 7, 0, 0, -1, 65536, 3, 32768, 1, 12288, 262144};
+  clue_tuning tuning{0.f, -1, 7, 0, 0, -1, 65536, 3, 32768, 1, 12288, 262144, 4};
   int64_t ldb = 0;  // leading dimension of the block boxes
   int64_t ld = 0;   // leading dimension of xs / xn
   // pinned staging ring of the event and chunk tables (a slot is reused once its

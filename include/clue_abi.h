@@ -139,7 +139,13 @@ typedef struct {
  *                      density_filter) and the first nearest-higher phase,
  *                      which shortens the longest query's chain when the call's
  *                      waves do not fill the GPU; it takes precedence over
- *                      small_call_points for the density pass; 0: never */
+ *                      small_call_points for the density pass; 0: never
+ *  nh_lone_lanes       first nearest-higher phase: a Morton block of the
+ *                      search that at most this many lanes of a wave still
+ *                      need is scanned across the wave (each needing lane's
+ *                      32 candidates on half the lanes, then a minimum over
+ *                      them) instead of one candidate at a time for the whole
+ *                      wave; 0: never, at most 64 */
 typedef struct {
   float nh_first_radius;
   int32_t morton_bits;
@@ -153,6 +159,7 @@ typedef struct {
   int32_t density_filter;
   int32_t fused_event_points;
   int32_t split_call_points;
+  int32_t nh_lone_lanes;
 } clue_tuning;
 
 typedef struct clue_plan_s* clue_plan_t;

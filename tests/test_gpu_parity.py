@@ -147,7 +147,10 @@ TUNINGS = [dict(nh_first_radius=0.05, morton_bits=5),
            dict(fused_event_points=0),
            dict(split_call_points=1 << 30, xcd_remap=0),
            dict(split_call_points=1 << 30, small_call_points=0, xcd_remap=7),
-           dict(split_call_points=0, small_call_points=0)]
+           dict(split_call_points=0, small_call_points=0),
+           dict(nh_lone_lanes=0),
+           dict(nh_lone_lanes=64, nh_first_radius=1.0, split_call_points=0),
+           dict(nh_lone_lanes=64, split_call_points=1 << 30)]
 
 
 @pytest.mark.parametrize("tuning", TUNINGS)
@@ -343,7 +346,10 @@ NH_PATHS = [dict(small_call_points=0, tile_call_points=0),                      
             dict(small_call_points=0, tile_call_points=1 << 30),                 # tile walk, a lane per query
             dict(small_call_points=1 << 30, tile_call_points=1 << 30),           # tile walk, a wave per query
             dict(small_call_points=1 << 30, tile_call_points=0),
-            dict(small_call_points=0, tile_call_points=0, split_call_points=1 << 30)]
+            dict(small_call_points=0, tile_call_points=0, split_call_points=1 << 30),
+            dict(small_call_points=0, tile_call_points=0, nh_lone_lanes=0),  # point by point only
+            dict(small_call_points=0, tile_call_points=0, nh_lone_lanes=64, nh_first_radius=1.0),  # across only
+            dict(small_call_points=0, tile_call_points=0, nh_lone_lanes=64, split_call_points=1 << 30)]
 
 
 _RING_CACHE = {}
