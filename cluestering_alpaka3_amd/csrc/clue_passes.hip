@@ -760,9 +760,9 @@ __device__ __forceinline__ float upper_bound2(const float* q, const float* lo, c
 //
 // This first phase (one lane per query, a wave per 64 Morton-consecutive
 // queries) limits the search to the radius thr1: the blocks of its lanes'
-// search boxes, the wave's own tile first (a first scan of the wave's own 64
-// points only repeated part of that tile's scan: 1-5 % slower).  A lane that finds a
-// higher point within it is final (every candidate at or below its distance was
+// search boxes, the wave's own tile first.  (A first scan of the wave's own 64
+// points repeated part of that tile's scan and was 1-5 % slower.)  A lane that
+// finds a higher point within it is final (every candidate at or below its distance was
 // examined); the others are queued for the second phase (k_nh_pending), the
 // full search within dm.
 //
@@ -818,7 +818,9 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
     qhi[d] = wave_max_f32(valid ? q[d] : -__builtin_inff());
   }
 
-  // (the update by selects: no branch inside the bound test)
+  // One candidate: a higher point within the bound replaces the best when it is
+  // nearer, or as near and earlier in the visit order (the update by selects,
+  // no branch inside the bound test).
   auto cand = [&](int t, const float* c, uint64_t key_j) {
     const float s2 = squared_distance<D>(q, c);
     if (key_j > thr_i && s2 <= bound) {
