@@ -196,8 +196,8 @@ def batched_bench(clue, datasets, args):
     outs = [clue.DeviceArray((N,), t) for t in (np.int32, np.int32, np.float32, np.int32)]
     plan = clue.Plan(0)
 
-    # autotune on the first events (about 4M points), points per tile included
-    k = int(np.searchsorted(off, 4_000_000))
+    # autotune on the first events (about 12M points), points per tile included
+    k = int(np.searchsorted(off, 12_000_000))
     k = max(1, min(k, E))
     t0 = time.perf_counter()
     tuning, ppb, tune_ms = plan.autotune_batched(stream, params, D, off[: k + 1], cp, dw.ptr, kd, md,

@@ -2242,7 +2242,7 @@ clue_status clue_autotune_candidates(int32_t* points_per_tile, int32_t capacity,
 }
 
 // Coordinate descent over the launch tuning (and, when asked, the points per
-// tile) on the given batch: every candidate is run once untimed and three times
+// tile) on the given batch: every candidate is run once untimed and five times
 // timed with HIP events on `stream`; the fastest median wins.
 clue_status clue_autotune_batched(clue_plan_t plan, void* stream, clue_params* params, int32_t ndim,
                                   int32_t n_events, const int64_t* event_offsets,
@@ -2277,11 +2277,11 @@ clue_status clue_autotune_batched(clue_plan_t plan, void* stream, clue_params* p
   };
   auto measure = [&](float* ms) -> clue_status {
     clue_status r = run_once(nullptr);
-    float v[3];
-    for (int k = 0; r == CLUE_OK && k < 3; ++k) r = run_once(&v[k]);
+    float v[5];
+    for (int k = 0; r == CLUE_OK && k < 5; ++k) r = run_once(&v[k]);
     if (r != CLUE_OK) return r;
-    std::sort(v, v + 3);
-    *ms = v[1];
+    std::sort(v, v + 5);
+    *ms = v[2];
     return CLUE_OK;
   };
   clue_tuning cur = saved;
