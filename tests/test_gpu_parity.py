@@ -495,8 +495,13 @@ def test_tiny_distances(case):
     else:  # rate 2^23
         x, w = tiny_clusters(1.5e-6, 1e-15)
         kernel, dc = ("exponential", 2.0 ** 23, 1.0), 1.5e-6
-    got, ref, *rest = run_both(x, w, dc, 3.0, 2 * dc, kernel=kernel)
-    assert_same(got, ref, *rest)
+    # one wave per query (small calls), one lane per query with the block search
+    # and with the tile walk, and the split walk
+    for tuning in (None, dict(small_call_points=0, tile_call_points=0, split_call_points=0),
+                   dict(small_call_points=0, tile_call_points=1 << 30),
+                   dict(small_call_points=0, tile_call_points=0, split_call_points=1 << 30)):
+        got, ref, *rest = run_both(x, w, dc, 3.0, 2 * dc, kernel=kernel, tuning=tuning)
+        assert_same(got, ref, *rest)
     d2 = (x[0, -1] - x[0, -2]) ** 2 + (x[1, -1] - x[1, -2]) ** 2
     assert d2 < 2.0 ** -96
 
