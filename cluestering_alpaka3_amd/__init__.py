@@ -754,7 +754,8 @@ def selftest_math(op, a, b=None, device=0):
     4 a/b through a double reciprocal, 5 expf of non-positive arguments, 6 / 7 the largest /
     smallest float whose sqrtf equals sqrtf(a), 8 / 9 expf of arguments in [-14, 0] as the density
     pass evaluates it for unit-amplitude kernels (9: its wide table, arguments clamped to [-14, 0]),
-    10 the nearest-higher searches' sqrtf of finite a >= 0."""
+    10 the nearest-higher searches' sqrtf of finite a >= 0, 11 their bound below which a squared
+    distance has a smaller sqrtf than sqrtf(a) (finite a >= 0)."""
     a = np.ascontiguousarray(a, np.float32)
     da = DeviceArray.from_host(a, device)
     db = DeviceArray.from_host(np.ascontiguousarray(b, np.float32), device) if b is not None else None

@@ -398,6 +398,22 @@ __device__ __forceinline__ void sqrt_class(float d, float& lo, float& hi) {
 // and take a square root only for those within.
 __device__ __forceinline__ float not_farther_than(float s) { return s * (1.f + 0x1p-20f) + 0x1p-126f; }
 
+// Every squared distance x >= 0 with x < strictly_nearer(s) has sqrtf(x) < sqrtf(s)
+// (s finite, >= 0): the class of sqrtf(s) reaches less than 2^-21 of s below it,
+// and the margin of 2^-20 minus the least normal float (one rounding) stays below
+// that; for s below 2^-126 the result is negative, for +inf it is +inf.  Checked
+// on every finite float.
+// The nearest-higher searches replace their best by a candidate below it without
+// taking a square root; only candidates between it and not_farther_than(s) need
+// the square roots compared.
+__device__ __forceinline__ float strictly_nearer(float s) { return __builtin_fmaf(s, 1.f - 0x1p-20f, -0x1p-126f); }
+
+// (sqrtf(s2), key) < (sqrtf(s2b), best) in lexicographic order (s2, s2b finite, >= 0)
+__device__ __forceinline__ bool sqrt_nearer_or_earlier(float s2, int64_t key, float s2b, int64_t best) {
+  const float dc = sqrt_rn_finite(s2), db = sqrt_rn_finite(s2b);
+  return (dc < db) | ((dc == db) & (key < best));
+}
+
 // Order key of the nearest-higher relation (ClusteringKernels.hpp:132-134): point j
 // is higher than point i when rho_j > rho_i, or rho_j == rho_i > 0 and j > i.  With
 //   rank_key(rho, id) = ord(rho) << 32 | id,

@@ -831,6 +831,7 @@ __global__ void k_selftest(int op, const float* a, const float* b, float* out, i
     // arguments clamped to [-14, 0], the range the density pass uses it on
     case 9: out[i] = engine_expf_unit_wide(__builtin_fminf(__builtin_fmaxf(x, -14.f), 0.f), s_wide + (kExpWide - 1)); break;
     case 10: out[i] = sqrt_rn_finite(x); break;  // finite x >= 0
+    case 11: out[i] = strictly_nearer(x); break;  // finite x >= 0
     case 6:
     case 7: {
       float lo, hi;

@@ -124,8 +124,13 @@ __global__ __launch_bounds__(kThreads) void k_box_class(const PassArgs a, float 
 // order -- the same sums, with the chain of a query's terms cut to the adds.
 // For calls whose waves do not fill the GPU, where a pass lasts as long as its
 // heaviest wave.
+//
+// Occupancy: in 2D with the flat kernel, eight waves per SIMD (64 VGPRs and 78
+// SGPRs, no spills; seven at 68 VGPRs and 102 SGPRs without the request): the
+// 4096-event calorimeter batch's pass 4.15 -> 3.84 ms.
 template <int D, int METRIC, int KERNEL, bool UNION, bool QNH, int UNIT = 0, bool FILT = false, bool SPLIT = false>
-__global__ __launch_bounds__(kThreads) void k_density(const PassArgs a) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(D == 2 && KERNEL == KFLAT ? 8 : 1)))
+void k_density(const PassArgs a) {
   // the exponential's table in LDS: 2^(j/64), or for UNIT the wide table of
   // 2^(k/64) over the unit evaluation's whole range of k
   __shared__ double s_exp2[KERNEL == KFLAT ? 1 : (UNIT != 0 ? kExpWide : 64)];
@@ -771,8 +776,14 @@ __device__ __forceinline__ float upper_bound2(const float* q, const float* lo, c
 // which is never better than the combined one, so the combined minimum's block
 // is scanned by its owner, and the four minima are combined at the end (the
 // minimum of (distance, visit position) does not depend on the order).
+//
+// Occupancy: in 2D the request of at least seven waves per SIMD keeps the SGPRs
+// at 94 (106 without it, which allows seven waves), so that with 63 VGPRs the
+// kernel runs eight (the batch's first phase about 5 % faster, the 2D 1M-point
+// one 2-3 %); in 3D it would spill.
 template <int D, bool SPLIT = false>
-__global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassArgs a) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(D == 2 ? 7 : 1)))
+void k_nearest_higher_blocks(const PassArgs a) {
   const int wv = threadIdx.x / kWave;
   const int gid = SPLIT ? logical_block((a.xcd >> 1) & 1) * kWave + __lane_id()
                         : logical_block((a.xcd >> 1) & 1) * kThreads + static_cast<int>(threadIdx.x);
@@ -802,14 +813,16 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
   // higher than any of its lanes'
   const uint64_t wthr = uniform(wave_min_u64(valid ? thr_i : ~uint64_t{0}));
   int64_t best = INT64_MAX;  // visit key (tile, id) of the best candidate
-  // The best candidate so far is at distance delta, squared distance s2b.  A
-  // candidate replaces it when its sqrtf is smaller, or equal and it is earlier
-  // in the visit order (key < best); it cannot when s2 > not_farther_than(s2b),
-  // so candidates need s2 <= bound = min(search radius, not_farther_than(s2b)),
-  // and their square roots are taken only then.
+  // The best candidate so far is at squared distance s2b (its distance, the
+  // output, is sqrtf(s2b)).  A candidate replaces it when its sqrtf is smaller,
+  // or equal and it is earlier in the visit order (key < best); it cannot when
+  // s2 > not_farther_than(s2b), so candidates need s2 <= bound = min(search
+  // radius, not_farther_than(s2b)).  Below strictly_nearer(s2b) (+inf while
+  // there is no best) a candidate replaces the best without a square root;
+  // between that and the bound the two square roots decide.
   const float cap = a.thr1 < a.thr ? a.thr1 : a.thr;
   float bound = cap;
-  float delta = FLT_MAX;
+  float s2b = __builtin_inff();
   // the wave's query box, for the block prefilter
   float qlo[D], qhi[D];
 #pragma unroll
@@ -819,16 +832,15 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
   }
 
   // One candidate: a higher point within the bound replaces the best when it is
-  // nearer, or as near and earlier in the visit order (the update by selects,
-  // no branch inside the bound test).
+  // nearer, or as near and earlier in the visit order (the update by selects).
   auto cand = [&](int t, const float* c, uint64_t key_j) {
     const float s2 = squared_distance<D>(q, c);
     if (key_j > thr_i && s2 <= bound) {
       const int64_t key = (static_cast<int64_t>(t) << 32) | static_cast<uint32_t>(key_j);
-      const float dc = sqrt_rn_finite(s2);
-      const bool better = (dc < delta) | ((dc == delta) & (key < best));
+      bool better = s2 < strictly_nearer(s2b);
+      if (!better) better = sqrt_nearer_or_earlier(s2, key, s2b, best);
       const float up = not_farther_than(s2);
-      delta = better ? dc : delta;
+      s2b = better ? s2 : s2b;
       best = better ? key : best;
       bound = better ? (up < cap ? up : cap) : bound;
     }
@@ -864,9 +876,8 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
   // accepts (higher, squared distance within its bound), and the needing lane
   // merges it into its best.  The minimum of (distance, visit position) does not
   // depend on the order of examination, and inside one tile the visit position
-  // orders as the id, so the result is the point-by-point scan's.  After an
-  // update the bound holds every square whose sqrtf is at most delta: such a
-  // square lies below next_up(delta)^2.
+  // orders as the id, so the result is the point-by-point scan's.  The lane of
+  // each half that holds its half's least passes its squared distance along.
   auto scan_across = [&](int t, int beg, int end, uint64_t nm) {
     constexpr int kHalf = kWave / 2;
     static_assert(kBlock <= kHalf, "a block fits a half-wave");
@@ -898,9 +909,10 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
       const float b2 = readlane_f32(bound, l2);
       const float hb = upper ? b2 : b1;
       const float s2 = squared_distance<D>(hq, c);
-      uint64_t pk = ~uint64_t{0};
-      if (jv && kj > hthr && s2 <= hb)
-        pk = (static_cast<uint64_t>(__float_as_uint(sqrt_rn_finite(s2))) << 32) | static_cast<uint32_t>(kj);
+      const bool ok = jv && kj > hthr && s2 <= hb;
+      uint64_t pk =
+          ok ? (static_cast<uint64_t>(__float_as_uint(sqrt_rn_finite(s2))) << 32) | static_cast<uint32_t>(kj)
+             : ~uint64_t{0};
 #pragma unroll
       for (int off = kHalf / 2; off > 0; off >>= 1) {
         const uint64_t o = shfl_xor_u64(pk, off);
@@ -908,14 +920,23 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
       }
       const uint64_t r1 = readlane_u64(pk, 0);
       const uint64_t r2 = readlane_u64(pk, kHalf);
+      // the winners' lanes (ids differ inside a half: at most one lane each)
+      const uint64_t wm = __ballot(ok && static_cast<uint32_t>(kj) == static_cast<uint32_t>(upper ? r2 : r1));
+      const uint32_t wlo = static_cast<uint32_t>(wm), whi = static_cast<uint32_t>(wm >> 32);
+      const float sw1 = readlane_f32(s2, wlo ? __builtin_ctz(wlo) : 0);
+      const float sw2 = readlane_f32(s2, whi ? kHalf + __builtin_ctz(whi) : 0);
       const uint64_t r = lane == l1 ? r1 : r2;
       if ((lane == l1 || lane == l2) && r != ~uint64_t{0}) {
-        const float dr = __uint_as_float(static_cast<uint32_t>(r >> 32));
+        const float sw = lane == l1 ? sw1 : sw2;
         const int64_t key = (static_cast<int64_t>(t) << 32) | static_cast<uint32_t>(r);
-        const bool better = (dr < delta) | ((dr == delta) & (key < best));
-        const float dn = __uint_as_float(__float_as_uint(dr) + 1u);
-        const float up = not_farther_than(dn * dn);
-        delta = better ? dr : delta;
+        bool better = sw < strictly_nearer(s2b);
+        if (!better) {
+          const float dr = __uint_as_float(static_cast<uint32_t>(r >> 32));  // sqrtf(sw)
+          const float db = sqrt_rn_finite(s2b);
+          better = (dr < db) | ((dr == db) & (key < best));
+        }
+        const float up = not_farther_than(sw);
+        s2b = better ? sw : s2b;
         best = better ? key : best;
         bound = better ? (up < cap ? up : cap) : bound;
       }
@@ -1048,6 +1069,7 @@ __global__ __launch_bounds__(kThreads) void k_nearest_higher_blocks(const PassAr
     }
   });
 
+  float delta = best == INT64_MAX ? FLT_MAX : sqrt_rn_finite(s2b);
   if constexpr (SPLIT) {
     // the four waves' minima, combined by wave 0
     __shared__ float s_delta[kThreads / kWave][kWave];
@@ -1103,8 +1125,12 @@ __device__ __forceinline__ float group_min_f32(float v) {
   return v;
 }
 
+//
+// Occupancy: a single 2D event asks for six waves per SIMD (80 VGPRs, no
+// spills; five at 83 without the request); batched 2D events and 3D would spill.
 template <int D, int G, bool SINGLE>
-__global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(D == 2 && SINGLE ? 6 : 1)))
+void k_nh_pending(const PassArgs a) {
   static_assert(kWave % G == 0, "groups must tile the wave");
   const int npend = uniform(*a.pending_count);
   const int lane = __lane_id();
@@ -1129,10 +1155,11 @@ __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
     search_box<D>(q, a.radius, g, lo, hi);
     int64_t best = INT64_MAX;
     // candidates need s2 <= bound = min(thr(dm), not_farther_than(s2b)) (s2b: the
-    // lane's best squared distance, delta its sqrtf), and replace the lane's best
-    // when their sqrtf is smaller, or equal with key < best
+    // lane's best squared distance), and replace the lane's best when their sqrtf
+    // is smaller (certainly so below strictly_nearer(s2b); s2b is +inf while
+    // there is no best), or equal with key < best
     float bound = a.thr;
-    float delta = FLT_MAX;
+    float s2b = __builtin_inff();
     float group_bound = a.thr;
     // Levels of growing radius order the visits nearest first, so that the
     // bound shrinks early.  Each block is visited once, at the first level whose
@@ -1185,10 +1212,10 @@ __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
       const float s2 = squared_distance<D>(q, c);
       if (key_j > thr_i && s2 <= bound) {
         const int64_t key = (static_cast<int64_t>(t) << 32) | static_cast<uint32_t>(key_j);
-        const float dc = sqrt_rn_finite(s2);
-        const bool better = (dc < delta) | ((dc == delta) & (key < best));
+        bool better = s2 < strictly_nearer(s2b);
+        if (!better) better = sqrt_nearer_or_earlier(s2, key, s2b, best);
         const float up = not_farther_than(s2);
-        delta = better ? dc : delta;
+        s2b = better ? s2 : s2b;
         best = better ? key : best;
         bound = better ? (up < a.thr ? up : a.thr) : bound;
       }
@@ -1286,6 +1313,7 @@ __global__ __launch_bounds__(kThreads) void k_nh_pending(const PassArgs a) {
       prev = cap;
     }
     // lexicographic (distance, key) minimum over the group
+    float delta = best == INT64_MAX ? FLT_MAX : sqrt_rn_finite(s2b);
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1) {
       const float od = __shfl_xor(delta, off);

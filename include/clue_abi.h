@@ -426,7 +426,9 @@ const char* clue_stage_name(int32_t stage);
  *      equals sqrtf(a), 8 expf of arguments in [-14, 0] as the density pass
  *      evaluates it for unit-amplitude kernels with the 64-entry table, 9 the
  *      same with the wide table the density pass reads, arguments clamped to
- *      [-14, 0], 10 the nearest-higher searches' sqrtf of finite a >= 0). --------------------------------------------------- */
+ *      [-14, 0], 10 the nearest-higher searches' sqrtf of finite a >= 0, 11
+ *      their bound below which a squared distance has a smaller sqrtf than
+ *      sqrtf(a), finite a >= 0). --------------------------------------------------- */
 clue_status clue_selftest_math(int32_t op, const float* a_dev, const float* b_dev, float* out_dev,
                                int64_t n, void* stream);
 /* Number of floats with bit patterns in [first_bits, first_bits + count) on

@@ -116,6 +116,13 @@ def test_sqrt_class_bounds():
         assert np.all(np.sqrt(np.nextafter(hi, f32(np.inf))) > dd)
         below = np.nextafter(lo, f32(-np.inf))
         assert np.all((lo == 0) | (np.sqrt(np.maximum(below, f32(0))) < dd))
+        # op 11: every square below strictly_nearer(s) has a smaller sqrtf, and the
+        # bound lies within 2^-19 of s (it only spares square roots)
+        nn = clue.selftest_math(11, ss)
+        pos = nn > 0
+        assert np.all(np.sqrt(np.nextafter(nn[pos], f32(-np.inf))) < dd[pos])
+        big = ss >= f32(2.0 ** -100)
+        assert np.all(nn[big] >= ss[big] * f32(1 - 2.0 ** -19))
 
 
 # reduced sizes with the local density of the full workloads (box and blob count

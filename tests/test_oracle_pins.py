@@ -547,3 +547,20 @@ def test_engine_exp_exhaustive():
     assert len(lines) == 3, r.stdout
     assert lines[0].endswith(", 0 mismatches") and lines[1].endswith(", 0 mismatches"), r.stdout
     assert lines[2].endswith(", 0 mismatches (0 indexes outside the table)"), r.stdout
+
+
+def test_sqrt_bounds_exhaustive():
+    # the nearest-higher searches' bounds on squared distances, on every finite
+    # float: above not_farther_than(s) every sqrtf is larger than sqrtf(s), below
+    # strictly_nearer(s) every sqrtf is smaller
+    import os
+    import subprocess
+
+    oracle_dir = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
+    r = subprocess.run(["make", "-C", oracle_dir, "-s", "check-sqrt"], capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = [l for l in r.stdout.splitlines() if l.startswith("exhaustive:")]
+    assert len(lines) == 2, r.stdout
+    assert all(l.endswith(", 0 violations") for l in lines), r.stdout
+    # most floats have a positive lower bound (all from 2^-126 * (1 + 2^-19) up)
+    assert int(lines[1].split(" over ")[1].split()[0]) > 2_000_000_000, r.stdout
