@@ -711,9 +711,11 @@ __global__ void k_capacity(const EventDesc* ev, int n_events, const int32_t* sc,
 
 // One synchronous pointer-jumping step, parent <- parent of parent: halves the
 // distance of every node to its root (other threads' concurrent updates only
-// replace a parent by one of its ancestors).
+// replace a parent by one of its ancestors).  Each XCD takes a contiguous eighth
+// of the points: parents lie in the point's event, so the random reads of one
+// XCD stay within the events it walks, in its own L2.
 __global__ __launch_bounds__(kThreads) void k_jump(int32_t* root, int32_t n) {
-  const int i = blockIdx.x * kThreads + threadIdx.x;
+  const int i = logical_block(1) * kThreads + threadIdx.x;
   if (i < n) root[i] = root[root[i]];
 }
 
@@ -732,8 +734,9 @@ __global__ __launch_bounds__(kThreads) void k_labels(int32_t* root, int32_t n, i
                                                       const int32_t* rank_base, int32_t* cluster,
                                                       int32_t* nh_out, int32_t* flags, int round) {
   if (round > 0 && flags[round - 1] == 0) return;  // every thread finished before
-  // later rounds run on a smaller grid, each thread striding over the points
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+  // later rounds run on a smaller grid, each thread striding over the points;
+  // blocks mapped to XCDs as in k_jump
+  for (int i = logical_block(1) * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
     const int e = pev[i];
     if (round == 0) {
       const int32_t nhv = nh_out[i];
