@@ -986,7 +986,9 @@ void k_nearest_higher_blocks(const PassArgs a) {
     const int lane = __lane_id();
     // one tile of the union: the blocks whose box and largest key can still
     // improve a lane that needs the tile (need: the tile is in the lane's box)
-    auto tile_pass = [&](int t, bool need) {
+    // (pfirst: a Morton position inside tile t whose block is walked first, the
+    // tile's blocks following in rotated order; -1 for the tile's own order)
+    auto tile_pass = [&](int t, bool need, int pfirst) {
       float tl[D], th[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
@@ -998,10 +1000,15 @@ void k_nearest_higher_blocks(const PassArgs a) {
       if (maxb >= 0.f) {
         const int k0 = BOFF[t], k1 = BOFF[t + 1];
         const int off0 = OFF[t], off1 = OFF[t + 1];
-        for (int kb = k0; kb < k1; kb += kWave) {
-          // one block per lane against the wave's query box
-          const int k = kb + lane;
-          const bool inb = k < k1;
+        const int nb = k1 - k0;
+        const int rel = (pfirst >= off0 && pfirst < off1) ? (pfirst - off0) / kBlock : 0;
+        for (int ci = 0; ci < nb; ci += kWave) {
+          // one block per lane against the wave's query box: lane l takes the
+          // tile's block (rel + ci + l) mod nb
+          const bool inb = ci + lane < nb;
+          int r = rel + ci + lane;
+          r = r >= nb ? r - nb : r;
+          const int k = k0 + (inb ? r : 0);
           float bl[D], bh[D];
 #pragma unroll
           for (int d = 0; d < D; ++d) {
@@ -1033,7 +1040,9 @@ void k_nearest_higher_blocks(const PassArgs a) {
             const bool nk = need && readlane_u64(bk, kl) > thr_i && lower_bound2<D>(q, ul, uh) <= bound;
             const uint64_t nm = __ballot(nk);
             if (nm != 0) {
-              const int start = off0 + (kb + kl - k0) * kBlock;
+              int rk = rel + ci + kl;
+              rk = rk >= nb ? rk - nb : rk;
+              const int start = off0 + rk * kBlock;
               const int end = min(start + kBlock, off1);
               if (__builtin_popcountll(nm) <= a.nh_lone) scan_across(t, start, end, nm);
               else if (nk) scan(t, start, end);
@@ -1042,12 +1051,12 @@ void k_nearest_higher_blocks(const PassArgs a) {
         }
       }
     };
-    // the wave's own tile first (its blocks hold the nearest candidates, so the
-    // lanes' bounds shrink before the neighbouring tiles are walked), then the
-    // rest of the union's tiles in lexicographic order
+    // the wave's own tile first, from the block of the wave's first query on
+    // (the nearest candidates, so that the lanes' bounds shrink before the rest
+    // is walked), then the rest of the union's tiles in lexicographic order
     if (t0 >= 0) {
       const bool need0 = in_box(t0);
-      if (__ballot(need0) != 0) tile_pass(t0, need0);
+      if (__ballot(need0) != 0) tile_pass(t0, need0, uniform(gid & ~(kWave - 1)));
     }
     int b[D];
 #pragma unroll
@@ -1058,7 +1067,7 @@ void k_nearest_higher_blocks(const PassArgs a) {
       for (int d = 0; d < D; ++d) need = need && b[d] >= lo[d] && b[d] <= hi[d];
       if (__ballot(need) != 0) {
         const int t = uniform(tile_index<D>(b, g));
-        if (t != t0) tile_pass(t, need);
+        if (t != t0) tile_pass(t, need, -1);
       }
       int d = D - 1;
       for (; d >= 0; --d) {
