@@ -2330,6 +2330,8 @@ clue_status clue_autotune_batched(clue_plan_t plan, void* stream, clue_params* p
   try_field([](clue_tuning& t, float v) { t.nh_lone_lanes = static_cast<int32_t>(v); }, {0.f, 2.f, 8.f});
   try_field(set_xcd, {0.f, 1.f, 2.f, 3.f, 4.f, 5.f, 6.f, 7.f});
   try_field(set_group, {8.f, 16.f, 32.f, 64.f});
+  try_field([](clue_tuning& t, float v) { t.pending_blocks = static_cast<int32_t>(v); },
+            {2048.f, 4096.f, 8192.f, 16384.f});
   try_field([](clue_tuning& t, float v) { t.density_order = static_cast<int32_t>(v); }, {0.f, 1.f});
   try_field([](clue_tuning& t, float v) { t.density_filter = static_cast<int32_t>(v); }, {0.f, 1.f});
   try_field([](clue_tuning& t, float v) { t.split_call_points = static_cast<int32_t>(v); }, {0.f, 1073741824.f});

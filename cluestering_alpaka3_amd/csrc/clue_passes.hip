@@ -1705,12 +1705,13 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
         else k_nearest_higher_blocks<D><<<grid, kThreads, 0, st>>>(a);
         CLUE_LAUNCH_CHECK("nearest_higher_blocks");
         // enough waves to fill the GPU; surplus waves find the pending list exhausted
-        // default grid of the persistent second phase: 8192 workgroups for one
-        // event of 2^20 points or more (2D 1M points: 0.51 -> 0.49 ms, 3D 4M: 7.43
-        // -> 7.31 ms), 2048 otherwise (the 4096-event batch: 7.08 ms against 7.23 ms
-        // with 8192; small served requests lose more than they gain)
+        // default grid of the persistent second phase: 8192 workgroups for calls
+        // of 2^20 points or more (2D 1M points: 0.51 -> 0.49 ms, 3D 4M: 7.43 ->
+        // 7.31 ms; the 4096-event batch's nearest-higher stage 5.28 -> 4.93 ms
+        // against 2048), 2048 otherwise (small served requests lose more than
+        // they gain)
         const int pblocks = p->tuning.pending_blocks > 0 ? p->tuning.pending_blocks
-                                                         : (a.n_events == 1 && a.n >= (1 << 20) ? 8192 : 2048);
+                                                         : (a.n >= (1 << 20) ? 8192 : 2048);
         const int nblk = static_cast<int>(std::min<int64_t>(pblocks, div_up(a.n, kThreads / kWave)));
         const bool single = a.n_events == 1;
         switch (group) {

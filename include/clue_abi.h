@@ -105,7 +105,7 @@ typedef struct {
  *                      neighbouring queries share one L2; bit 2: the same for
  *                      the density's heaviest-first order (each XCD four
  *                      separate chunks of the Morton order, heaviest first)
- *  pending_blocks      workgroups of the second nearest-higher phase, 0 auto (8192 for one event of 2^20 points or more, else 2048)
+ *  pending_blocks      workgroups of the second nearest-higher phase, 0 auto (8192 for calls of 2^20 points or more, else 2048)
  *  pending_group       lanes per unresolved point in that phase: 8, 16, 32, 64;
  *                      0 auto (8 in 3D, 16 otherwise)
  *  density_order       1: launch the density pass's query blocks heaviest first
