@@ -1681,9 +1681,11 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
     if constexpr (METRIC == MEUCLID && D <= 3) {
       if (blocks) {
         // automatic first radius and group: measured per dimension (3D 4M points:
-        // 3.76 ms at 0.15 dm and 16 lanes, 3.0 ms at 0.2 dm and 8 lanes; 2D 1M
-        // points: 0.47 ms at 0.15 dm and 16 lanes, 0.50 ms at 0.2 dm and 8 lanes)
-        const float r1 = p->tuning.nh_first_radius > 0.f ? p->tuning.nh_first_radius : (D == 3 ? 0.2f : 0.15f);
+        // 3.76 ms at 0.15 dm and 16 lanes, 3.0 ms at 0.2 dm and 8 lanes, and with
+        // the first phase walking its own tile from the wave's block, 2.70 ms at
+        // 0.2 dm, 2.61 at 0.3 and 0.4; 2D 1M points: 0.47 ms at 0.15 dm and 16
+        // lanes, 0.50 ms at 0.2 dm and 8 lanes, now 0.35-0.37 ms from 0.15 to 0.4)
+        const float r1 = p->tuning.nh_first_radius > 0.f ? p->tuning.nh_first_radius : (D == 3 ? 0.3f : 0.15f);
         const int group = p->tuning.pending_group > 0 ? p->tuning.pending_group : (D == 3 ? 8 : 16);
         a.thr1 = sqrt_le_bound(dm * r1);
         a.box1 = dm * r1 * (1.f + 1.f / 1024.f);

@@ -98,7 +98,7 @@ typedef struct {
 /* Launch tuning of a plan.  Every value leaves the results unchanged (they only
  * steer work order and pruning); the autotuner picks them by timing.
  *  nh_first_radius     first nearest-higher phase radius, as a fraction of dm;
- *                      0 auto (0.2 in 3D, 0.15 otherwise)
+ *                      0 auto (0.3 in 3D, 0.15 otherwise)
  *  morton_bits         bits per dimension of the in-tile Morton order, -1 auto
  *  xcd_remap           bit 0: density, bit 1: nearest-higher -- give each
  *                      XCD a contiguous range of the pass's query blocks, so
