@@ -1684,9 +1684,11 @@ clue_status launch_passes(const PassArgs& base_args, int kernel_type, float dc, 
         // 3.76 ms at 0.15 dm and 16 lanes, 3.0 ms at 0.2 dm and 8 lanes, and with
         // the first phase walking its own tile from the wave's block, 2.70 ms at
         // 0.2 dm, 2.61 at 0.3 and 0.4; 2D 1M points: 0.47 ms at 0.15 dm and 16
-        // lanes, 0.50 ms at 0.2 dm and 8 lanes, now 0.35-0.37 ms from 0.15 to 0.4)
-        const float r1 = p->tuning.nh_first_radius > 0.f ? p->tuning.nh_first_radius : (D == 3 ? 0.3f : 0.15f);
-        const int group = p->tuning.pending_group > 0 ? p->tuning.pending_group : (D == 3 ? 8 : 16);
+        // lanes, 0.50 ms at 0.2 dm and 8 lanes, now 0.360 ms at 0.15 dm and 16 lanes,
+        // 0.350 at 0.3 dm and 8, 0.356 at 0.25 and 8, 0.371 at 0.15 and 8)
+        const bool d23 = D == 2 || D == 3;
+        const float r1 = p->tuning.nh_first_radius > 0.f ? p->tuning.nh_first_radius : (d23 ? 0.3f : 0.15f);
+        const int group = p->tuning.pending_group > 0 ? p->tuning.pending_group : (d23 ? 8 : 16);
         a.thr1 = sqrt_le_bound(dm * r1);
         a.box1 = dm * r1 * (1.f + 1.f / 1024.f);
         // second phase levels: twice the radius each time, dm last

@@ -98,7 +98,7 @@ typedef struct {
 /* Launch tuning of a plan.  Every value leaves the results unchanged (they only
  * steer work order and pruning); the autotuner picks them by timing.
  *  nh_first_radius     first nearest-higher phase radius, as a fraction of dm;
- *                      0 auto (0.3 in 3D, 0.15 otherwise)
+ *                      0 auto (0.3 in 2D and 3D, 0.15 otherwise)
  *  morton_bits         bits per dimension of the in-tile Morton order, -1 auto
  *  xcd_remap           bit 0: density, bit 1: nearest-higher -- give each
  *                      XCD a contiguous range of the pass's query blocks, so
@@ -107,7 +107,7 @@ typedef struct {
  *                      separate chunks of the Morton order, heaviest first)
  *  pending_blocks      workgroups of the second nearest-higher phase, 0 auto (8192 for calls of 2^20 points or more, else 2048)
  *  pending_group       lanes per unresolved point in that phase: 8, 16, 32, 64;
- *                      0 auto (8 in 3D, 16 otherwise)
+ *                      0 auto (8 in 2D and 3D, 16 otherwise)
  *  density_order       1: launch the density pass's query blocks heaviest first
  *                      (estimated from their tiles' sizes), 0: in Morton order,
  *                      -1: heaviest first in 1D and 2D only
