@@ -1229,17 +1229,29 @@ void k_nh_pending(const PassArgs a) {
         bound = better ? (up < a.thr ? up : a.thr) : bound;
       }
     };
+    // at every level the point's own tile is walked first (box index own_ti
+    // swapped with 0): its blocks hold the nearest candidates, so the bound
+    // shrinks before the other tiles are walked (any order gives the same result)
+    int own_ti = 0;
+    {
+      int idx = 0;
+#pragma unroll
+      for (int d = 0; d < D; ++d)
+        idx = idx * dims[d] + (tile_bin(q[d], g.mn[d], g.mx[d], g.ts[d], g.npd, false) - lo[d]);
+      own_ti = (idx >= 0 && idx < nt) ? idx : 0;
+    }
     for (int lv = 0; lv < a.nlvl; ++lv) {
       const float cap = a.lvl[lv];
       int ti = 0;
       bool more = active && !done;
       while (__ballot(more) != 0) {
         if (more) {  // uniform within a group
+          const int tv = ti == 0 ? own_ti : (ti == own_ti ? 0 : ti);
           int t, k0, k1, off0, off1;
           uint64_t tk;
           float tl[D], th[D];
           if (cached) {
-            const int src = grp_shift + ti;
+            const int src = grp_shift + tv;
             t = __shfl(my_t, src);
             tk = (static_cast<uint64_t>(__shfl(static_cast<uint32_t>(my_tk >> 32), src)) << 32) |
                  __shfl(static_cast<uint32_t>(my_tk), src);
@@ -1253,7 +1265,7 @@ void k_nh_pending(const PassArgs a) {
               th[d] = __shfl(my_th[d], src);
             }
           } else {
-            t = tile_of(ti);
+            t = tile_of(tv);
 #pragma unroll
             for (int d = 0; d < D; ++d) {
               tl[d] = a.tmin[d][t];
